@@ -1,0 +1,77 @@
+"""Writes the golden vectors of the compiled reference (Euler3D hot path).
+
+Run in the development container, where the reference sources exist:
+    python tests/golden/make_golden.py
+It builds oracle/_ref/ref_harness from the reference's own sources
+(oracle/Makefile), generates each case's UGRID mesh with
+cfdsolutions_amd.mesh, runs the harness, and stores its .npy outputs plus the
+case metadata in tests/golden/<case>.npz.  The fixtures hold data only.
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from cfdsolutions_amd import mesh as cmesh  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+# name: mesh spec, harness args
+CASES = {
+    "bump10_roe1": (dict(kind="bump", n=(10, 3, 3), h=0.1),
+                    dict(flux="roe", order=1, lim="none", mach=0.5, alpha=0.0, seed=11, amp=0.05, sweeps=3,
+                         iter=5, efix=0, k=5.0, lorder=1)),
+    "bump10_roe2_venkat": (dict(kind="bump", n=(10, 3, 3), h=0.1),
+                           dict(flux="roe", order=2, lim="venkat", mach=0.5, alpha=2.0, seed=12, amp=0.05, sweeps=4,
+                                iter=60, efix=1, k=5.0, lorder=1)),
+    "bump10_hllc1": (dict(kind="bump", n=(10, 3, 3), h=0.1),
+                     dict(flux="hllc", order=1, lim="none", mach=0.7, alpha=2.0, seed=13, amp=0.05, sweeps=3,
+                          iter=3, efix=0, k=5.0, lorder=1)),
+    "bump10_hllc2_venkat_lo2": (dict(kind="bump", n=(10, 3, 3), h=0.1),
+                                dict(flux="hllc", order=2, lim="venkat", mach=0.7, alpha=2.0, seed=14, amp=0.05,
+                                     sweeps=3, iter=3, efix=0, k=0.3, lorder=2)),
+    "cube5_roe2_bj_super": (dict(kind="cube", n=(5, 5, 5), jitter=0.2),
+                            dict(flux="roe", order=2, lim="bj", mach=1.5, alpha=0.0, seed=15, amp=0.1, sweeps=3,
+                                 iter=10, efix=1, k=5.0, lorder=1)),
+    "cube5_roe2_nolim": (dict(kind="cube", n=(5, 5, 5), jitter=0.2),
+                         dict(flux="roe", order=2, lim="none", mach=0.8, alpha=1.0, seed=16, amp=0.05, sweeps=3,
+                              iter=0, efix=0, k=5.0, lorder=2)),
+    "cube5_roe2_venkat_strong": (dict(kind="cube", n=(5, 5, 5), jitter=0.2),
+                                 dict(flux="roe", order=2, lim="venkat", mach=2.5, alpha=0.0, seed=17, amp=0.35,
+                                      sweeps=2, iter=1, efix=1, k=50.0, lorder=1)),
+}
+
+
+def make_mesh(spec, path):
+    if spec["kind"] == "bump":
+        nx, ny, nz = spec["n"]
+        return cmesh.bump_channel(nx, ny, nz, bump_h=spec["h"], ugrid_path=path)
+    n = spec["n"][0]
+    return cmesh.jittered_cube(n, jitter=spec["jitter"], ugrid_path=path)
+
+
+def main():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "-j8"], check=True)
+    for name, (mspec, a) in CASES.items():
+        with tempfile.TemporaryDirectory() as td:
+            ug = os.path.join(td, "mesh.ugrid")
+            make_mesh(mspec, ug)
+            args = [HARNESS, ug, td, a["flux"], str(a["order"]), a["lim"], repr(a["mach"]), repr(a["alpha"]),
+                    str(a["seed"]), repr(a["amp"]), str(a["sweeps"]), str(a["iter"]), str(a["efix"]), repr(a["k"]),
+                    str(a["lorder"])]
+            subprocess.run(args, check=True, stdout=subprocess.DEVNULL, cwd=td)
+            data = {f[:-4]: np.load(os.path.join(td, f)) for f in os.listdir(td) if f.endswith(".npy")}
+        data["meta"] = np.array(json.dumps({"mesh": mspec, "args": a}))
+        out = os.path.join(os.path.dirname(os.path.abspath(__file__)), name + ".npz")
+        np.savez_compressed(out, **data)
+        print(name, {k: v.shape for k, v in data.items() if k in ("xyz", "edge", "jac_a")}, os.path.getsize(out))
+
+
+if __name__ == "__main__":
+    main()
