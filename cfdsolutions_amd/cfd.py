@@ -1,0 +1,292 @@
+"""Python binding of the C ABI in include/cfd_hip.h (libcfd_hip.so).
+
+The compute path is the HIP library; this module only marshals arrays.  It
+raises if the library is not built: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _build
+
+OK = 0
+E_ARG, E_HIP, E_OOM, E_NONFINITE, E_NEGATIVE_PRESSURE, E_SINGULAR, E_HLLC, E_STATE = -1, -2, -3, -4, -5, -6, -7, -8
+FLUX_ROE, FLUX_HLLC, FLUX_AUSMP = 0, 1, 2
+LIM_NONE, LIM_VENKAT, LIM_BJ = 0, 1, 2
+GRAD_LSQ, GRAD_GG = 0, 1
+FP64, FP32 = 0, 1
+PREC_NONE, PREC_BJACOBI, PREC_ILU0 = 0, 1, 2
+SOLVER_FGMRES, SOLVER_BLOCK_GS, SOLVER_LUSGS = 0, 1, 2
+FIELD_Q, FIELD_GRAD, FIELD_PHI, FIELD_RES_CONV, FIELD_RES_DISS, FIELD_DT_ACC, FIELD_DT, FIELD_JAC_B, FIELD_X = range(9)
+
+_FLUX_NAMES = {"roe": FLUX_ROE, "hllc": FLUX_HLLC, "ausm+": FLUX_AUSMP, "ausmp": FLUX_AUSMP}
+
+
+class CfdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("cfd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("n_node", ctypes.c_int32), ("n_edge", ctypes.c_int32), ("n_bedge", ctypes.c_int32),
+                ("edge", ctypes.c_void_p), ("edge_area", ctypes.c_void_p), ("bedge", ctypes.c_void_p),
+                ("bedge_area", ctypes.c_void_p), ("bedge_type", ctypes.c_void_p), ("bedge_tag", ctypes.c_void_p),
+                ("xyz", ctypes.c_void_p), ("vol", ctypes.c_void_p), ("n2n_ia", ctypes.c_void_p),
+                ("n2n_ja", ctypes.c_void_p)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("inf_rho", ctypes.c_double), ("inf_u", ctypes.c_double),
+                ("inf_v", ctypes.c_double), ("inf_w", ctypes.c_double), ("inf_p", ctypes.c_double),
+                ("outflow_p", ctypes.c_double), ("flux", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("limiter", ctypes.c_int32), ("limiter_order", ctypes.c_int32), ("entropy_fix", ctypes.c_int32),
+                ("gradient", ctypes.c_int32), ("zpg_iter", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("venkat_k", ctypes.c_double), ("cfl_min", ctypes.c_double), ("cfl_max", ctypes.c_double),
+                ("cfl_ramp", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class LinsolveOpts(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int32), ("precond", ctypes.c_int32), ("krylov_m", ctypes.c_int32),
+                ("max_restarts", ctypes.c_int32), ("rtol", ctypes.c_double), ("sweeps", ctypes.c_int32),
+                ("mixed", ctypes.c_int32)]
+
+
+class LinsolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("restarts", ctypes.c_int32), ("final_res", ctypes.c_double),
+                ("b_norm", ctypes.c_double)]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int32), ("tol", ctypes.c_double), ("first_order_iters", ctypes.c_int32),
+                ("relaxation", ctypes.c_double), ("wall_tag", ctypes.c_int32), ("alpha_deg", ctypes.c_double),
+                ("lin", LinsolveOpts)]
+
+
+class SolveResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("krylov_total", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("first_rms", ctypes.c_double), ("final_rms", ctypes.c_double),
+                ("force", ctypes.c_double * 3), ("lift", ctypes.c_double), ("drag", ctypes.c_double),
+                ("pressure_integral", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = _build.lib_path("libcfd_hip.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libcfd_hip.so is not built (run __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(path)
+        L.cfd_last_error.restype = ctypes.c_char_p
+        L.cfd_mesh_device_bytes.restype = ctypes.c_int64
+        L.cfd_footprint_bytes.restype = ctypes.c_int64
+        L.cfd_footprint_bytes.argtypes = [ctypes.c_int32] * 5
+        L.cfd_mesh_stream.restype = ctypes.c_void_p
+        L.cfd_bsr_structure.restype = ctypes.c_int32
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_params(mach=0.5, alpha_deg=0.0, gamma=1.4, flux="roe", order=1, limiter=LIM_NONE, limiter_order=1,
+                entropy_fix=0, gradient=GRAD_LSQ, precision=FP64, venkat_k=5.0, cfl_min=1.0, cfl_max=1000.0,
+                cfl_ramp=50, zpg_iter=0, inf=None):
+    """Freestream of the GENERIC non-dimensionalisation (Euler3D/src/Material.cpp:112-210):
+    rho = 1, T = gamma, R = 1/gamma, p = R*rho*T, c = sqrt(gamma*R*T)."""
+    if inf is None:
+        r = 1.0 / gamma
+        rho, T = 1.0, gamma
+        p = r * rho * T
+        c = np.sqrt(gamma * r * (T + 0.0))
+        a = alpha_deg * np.pi / 180.0
+        u = mach * c * np.cos(a) * np.cos(0.0)
+        v = mach * c * np.sin(a)
+        w = mach * c * np.sin(0.0) * np.cos(a)
+        outp = 0.935 * p
+    else:
+        rho, u, v, w, p, outp = inf[0], inf[1], inf[2], inf[3], inf[4], inf[10]
+    f = _FLUX_NAMES[flux] if isinstance(flux, str) else flux
+    return Params(gamma, rho, u, v, w, p, outp, f, order, limiter, limiter_order, entropy_fix, gradient, zpg_iter,
+                  precision, venkat_k, cfl_min, cfl_max, cfl_ramp, 0)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        rc = lib().cfd_ctx_create(int(device), ctypes.byref(self.h))
+        if rc:
+            raise CfdError(rc, "cfd_ctx_create failed (no GPU?)")
+
+    def check(self, rc):
+        if rc:
+            raise CfdError(rc, lib().cfd_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            lib().cfd_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceMesh:
+    """A mesh resident in HBM with its state, residuals and Jacobian workspace."""
+
+    def __init__(self, ctx, mesh, params, bedge_type=None):
+        self.ctx = ctx
+        c = np.ascontiguousarray
+        self._keep = dict(
+            edge=c(mesh.edge, np.int32), edge_area=c(mesh.edge_area, np.float64), bedge=c(mesh.bedge, np.int32),
+            bedge_area=c(mesh.bedge_area, np.float64),
+            bedge_type=c(mesh.bedge_type if bedge_type is None else bedge_type, np.int32),
+            bedge_tag=c(mesh.bedge_tag, np.int32), xyz=c(mesh.xyz, np.float64), vol=c(mesh.vol, np.float64),
+            n2n_ia=c(mesh.n2n_ia, np.int32), n2n_ja=c(mesh.n2n_ja, np.int32))
+        k = self._keep
+        self.N = k["vol"].size
+        self.E = k["edge"].size // 2
+        self.B = k["bedge"].size // 2
+        d = MeshDesc(self.N, self.E, self.B, _p(k["edge"]), _p(k["edge_area"]), _p(k["bedge"]), _p(k["bedge_area"]),
+                     _p(k["bedge_type"]), _p(k["bedge_tag"]), _p(k["xyz"]), _p(k["vol"]), _p(k["n2n_ia"]),
+                     _p(k["n2n_ja"]))
+        self.h = ctypes.c_void_p()
+        ctx.check(lib().cfd_mesh_create(ctx.h, ctypes.byref(d), ctypes.byref(params), ctypes.byref(self.h)))
+
+    @property
+    def NT(self):
+        return self.N + self.B
+
+    def close(self):
+        if self.h:
+            lib().cfd_mesh_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_bytes(self):
+        return lib().cfd_mesh_device_bytes(self.h)
+
+    def stream(self):
+        return lib().cfd_mesh_stream(self.h)
+
+    def sync(self):
+        self.ctx.check(lib().cfd_mesh_sync(self.h))
+
+    _SHAPES = {FIELD_Q: lambda s: (5, s.NT), FIELD_GRAD: lambda s: (15, s.N), FIELD_PHI: lambda s: (5, s.N),
+               FIELD_RES_CONV: lambda s: (5, s.N), FIELD_RES_DISS: lambda s: (5, s.N), FIELD_DT_ACC: lambda s: (s.N,),
+               FIELD_DT: lambda s: (s.N,), FIELD_JAC_B: lambda s: (s.N, 5), FIELD_X: lambda s: (s.N, 5)}
+
+    def set(self, field, arr):
+        shape = self._SHAPES[field](self)
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(shape)
+        self.ctx.check(lib().cfd_field_set(self.h, field, _p(a)))
+
+    def get(self, field):
+        out = np.zeros(self._SHAPES[field](self))
+        self.ctx.check(lib().cfd_field_get(self.h, field, _p(out)))
+        return out
+
+    # hot path ------------------------------------------------------------
+    def bc(self, p, it):
+        self.ctx.check(lib().cfd_bc(self.h, ctypes.byref(p), int(it)))
+
+    def gradient(self, p):
+        self.ctx.check(lib().cfd_gradient(self.h, ctypes.byref(p)))
+
+    def limiter(self, p):
+        self.ctx.check(lib().cfd_limiter(self.h, ctypes.byref(p)))
+
+    def residual(self, p, stats=False):
+        eig = np.zeros(6) if stats else None
+        self.ctx.check(lib().cfd_residual(self.h, ctypes.byref(p), 1, _p(eig)))
+        return eig
+
+    def delta_t(self, p, it):
+        cfl = ctypes.c_double()
+        self.ctx.check(lib().cfd_delta_t(self.h, ctypes.byref(p), int(it), ctypes.byref(cfl)))
+        return cfl.value
+
+    def rms(self):
+        r = ctypes.c_double()
+        self.ctx.check(lib().cfd_residual_rms(self.h, ctypes.byref(r)))
+        return r.value
+
+    def jacobian(self, p, rhs_mode=1, mixed=False):
+        self.ctx.check(lib().cfd_jacobian(self.h, ctypes.byref(p), int(rhs_mode), int(bool(mixed))))
+
+    def bsr_structure(self):
+        nnz = lib().cfd_bsr_structure(self.h, None, None, None)
+        ia = np.zeros(self.N + 1, np.int32)
+        ja = np.zeros(nnz, np.int32)
+        iau = np.zeros(self.N, np.int32)
+        lib().cfd_bsr_structure(self.h, _p(ia), _p(ja), _p(iau))
+        return ia, ja, iau
+
+    def jacobian_get(self):
+        nnz = lib().cfd_bsr_structure(self.h, None, None, None)
+        a = np.zeros((nnz, 25))
+        b = np.zeros((self.N, 5))
+        self.ctx.check(lib().cfd_jacobian_get(self.h, _p(a), _p(b)))
+        return a, b
+
+    def jacobian_set(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        self.ctx.check(lib().cfd_jacobian_set(self.h, _p(a), _p(b)))
+
+    def linsolve(self, solver=SOLVER_FGMRES, precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, sweeps=0,
+                 mixed=False):
+        o = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
+        st = LinsolveStats()
+        self.ctx.check(lib().cfd_linsolve(self.h, ctypes.byref(o), ctypes.byref(st)))
+        return st
+
+    def update(self, relax=1.0):
+        self.ctx.check(lib().cfd_update(self.h, ctypes.c_double(relax)))
+
+    def forces(self, p, wall_tag=3, alpha_deg=0.0):
+        f = (ctypes.c_double * 3)()
+        lift, drag, pint = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(lib().cfd_forces(self.h, ctypes.byref(p), int(wall_tag), ctypes.c_double(alpha_deg), f,
+                                        ctypes.byref(lift), ctypes.byref(drag), ctypes.byref(pint)))
+        return np.array(f[:]), lift.value, drag.value, pint.value
+
+    def solve_steady(self, p, max_iter=200, tol=1e-10, first_order_iters=0, relaxation=1.0, wall_tag=3,
+                     alpha_deg=0.0, precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, mixed=False,
+                     solver=SOLVER_FGMRES, sweeps=0, check=True):
+        lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
+        o = SolveOpts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, lin)
+        r = SolveResult()
+        hist = np.zeros(max_iter)
+        rc = lib().cfd_solve_steady(self.h, ctypes.byref(p), ctypes.byref(o), ctypes.byref(r), _p(hist))
+        if check:
+            self.ctx.check(rc)
+        return r, hist[: r.iterations + 1]
+
+
+def footprint_bytes(n_node, n_edge, n_bedge, krylov_m=30, mixed=True):
+    return lib().cfd_footprint_bytes(int(n_node), int(n_edge), int(n_bedge), int(krylov_m), int(bool(mixed)))
+
+
+def gpu_available():
+    try:
+        n = ctypes.c_int()
+        hip = ctypes.CDLL("libamdhip64.so")
+        return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    except OSError:
+        return False

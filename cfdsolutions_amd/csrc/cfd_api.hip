@@ -1,0 +1,730 @@
+// C ABI of libcfd_hip.so: context and mesh handles, host-side structure
+// builders (incidence lists, BSR pattern, level schedules), field transfer,
+// and the steady implicit driver.
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "cfd_internal.h"
+
+// ---------------------------------------------------------------------------
+// errors and allocation
+// ---------------------------------------------------------------------------
+int cfd_set_error(cfd_ctx *ctx, int code, const char *fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int cfd_hip_check(cfd_ctx *ctx, hipError_t e, const char *what) {
+    return cfd_set_error(ctx, e == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+void *cfd_dalloc(cfd_mesh *m, size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    m->allocs.push_back(p);
+    m->bytes += (int64_t)bytes;
+    return p;
+}
+
+extern "C" int cfd_ctx_create(int device, cfd_ctx **out) {
+    if (!out) return CFD_E_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return CFD_E_HIP;
+    cfd_ctx *c = new (std::nothrow) cfd_ctx();
+    if (!c) return CFD_E_OOM;
+    c->device = device;
+    *out = c;
+    return CFD_OK;
+}
+
+extern "C" void cfd_ctx_destroy(cfd_ctx *ctx) { delete ctx; }
+
+extern "C" const char *cfd_last_error(const cfd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// ---------------------------------------------------------------------------
+// mesh creation
+// ---------------------------------------------------------------------------
+static void build_levels(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau, int N,
+                         bool upper, std::vector<int> &ptr, std::vector<int> &rows, int &maxw) {
+    std::vector<int> lev(N, 0);
+    int nl = 0;
+    if (!upper) {
+        for (int i = 0; i < N; i++) {
+            int l = 0;
+            for (int k = ia[i]; k < iau[i]; k++) l = std::max(l, lev[ja[k]] + 1);
+            lev[i] = l;
+            nl = std::max(nl, l + 1);
+        }
+    } else {
+        for (int i = N - 1; i >= 0; i--) {
+            int l = 0;
+            for (int k = iau[i] + 1; k < ia[i + 1]; k++) l = std::max(l, lev[ja[k]] + 1);
+            lev[i] = l;
+            nl = std::max(nl, l + 1);
+        }
+    }
+    ptr.assign(nl + 1, 0);
+    for (int i = 0; i < N; i++) ptr[lev[i] + 1]++;
+    maxw = 0;
+    for (int l = 0; l < nl; l++) {
+        maxw = std::max(maxw, ptr[l + 1]);
+        ptr[l + 1] += ptr[l];
+    }
+    rows.assign(N, 0);
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < N; i++) rows[pos[lev[i]]++] = i;
+}
+
+template <class T>
+static T *upload(cfd_mesh *m, const T *h, size_t n, int *rc) {
+    T *d = (T *)cfd_dalloc(m, sizeof(T) * n);
+    if (!d) {
+        *rc = cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory");
+        return nullptr;
+    }
+    if (n && hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        *rc = cfd_set_error(m->ctx, CFD_E_HIP, "hipMemcpy failed");
+        return nullptr;
+    }
+    return d;
+}
+
+extern "C" void cfd_mesh_destroy(cfd_mesh *m) {
+    if (!m) return;
+    hipSetDevice(m->ctx->device);
+    if (m->stream) hipStreamSynchronize(m->stream);
+    for (void *p : m->allocs) hipFree(p);
+    if (m->hred) hipHostFree(m->hred);
+    if (m->hflag) hipHostFree(m->hflag);
+    if (m->stream) hipStreamDestroy(m->stream);
+    delete m;
+}
+
+static void freestream_q(const cfd_params *p, double *q5) {
+    // Solver_Set_Initial_Conditions + ComputeFarFieldCondition with
+    // Get_TotalEnergy (Euler3D/src/Solver.cpp:340-362, Material.cpp:185-210)
+    double et = p->inf_u * p->inf_u + p->inf_v * p->inf_v + p->inf_w * p->inf_w;
+    et = p->inf_p / ((p->gamma - 1.0) * p->inf_rho) + 0.5 * et;
+    q5[0] = p->inf_rho;
+    q5[1] = p->inf_rho * p->inf_u;
+    q5[2] = p->inf_rho * p->inf_v;
+    q5[3] = p->inf_rho * p->inf_w;
+    q5[4] = p->inf_rho * et;
+}
+
+extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params *p, cfd_mesh **out) {
+    if (!ctx || !d || !p || !out) return CFD_E_ARG;
+    const int N = d->n_node, E = d->n_edge, B = d->n_bedge;
+    if (N <= 0 || E < 0 || B < 0 || !d->edge || !d->edge_area || (B && (!d->bedge || !d->bedge_area || !d->bedge_type)) ||
+        !d->xyz || !d->vol || !d->n2n_ia || !d->n2n_ja)
+        return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: invalid descriptor");
+    if ((int64_t)N + B > INT32_MAX / 2 || 2LL * E + B > INT32_MAX / 2)
+        return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: mesh too large for int32 indices");
+    // validate indices on the host before anything reaches a kernel
+    for (int e = 0; e < E; e++) {
+        const int l = d->edge[2 * e], r = d->edge[2 * e + 1];
+        if (l < 0 || r < 0 || l >= N || r >= N || l >= r)
+            return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: edge %d has invalid nodes (%d, %d)", e, l, r);
+    }
+    for (int b = 0; b < B; b++) {
+        const int l = d->bedge[2 * b], g = d->bedge[2 * b + 1];
+        if (l < 0 || l >= N || g != N + b)
+            return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: boundary edge %d must pair a node with ghost %d", b,
+                                 N + b);
+    }
+    if (d->n2n_ia[0] != 0) return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: n2n_ia[0] != 0");
+    for (int i = 0; i < N; i++) {
+        if (d->n2n_ia[i + 1] < d->n2n_ia[i]) return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: bad n2n_ia");
+        for (int k = d->n2n_ia[i]; k < d->n2n_ia[i + 1]; k++)
+            if (d->n2n_ja[k] < 0 || d->n2n_ja[k] >= N || d->n2n_ja[k] == i)
+                return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: bad neighbour in n2n_ja");
+    }
+    if (d->n2n_ia[N] != 2 * E)
+        return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: n2n lists (%d entries) do not match 2E = %d",
+                             d->n2n_ia[N], 2 * E);
+    if (hipSetDevice(ctx->device) != hipSuccess) return cfd_set_error(ctx, CFD_E_HIP, "hipSetDevice failed");
+
+    cfd_mesh *m = new (std::nothrow) cfd_mesh();
+    if (!m) return CFD_E_OOM;
+    m->ctx = ctx;
+    m->N = N;
+    m->E = E;
+    m->B = B;
+    m->NT = N + B;
+    int rc = CFD_OK;
+    try {
+        if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete m;
+            return cfd_set_error(ctx, CFD_E_HIP, "hipStreamCreate failed");
+        }
+        // node incidence lists in edge order (interior edges, then boundary)
+        std::vector<int> ip(N + 1, 0);
+        for (int e = 0; e < E; e++) {
+            ip[d->edge[2 * e] + 1]++;
+            ip[d->edge[2 * e + 1] + 1]++;
+        }
+        for (int b = 0; b < B; b++) ip[d->bedge[2 * b] + 1]++;
+        for (int i = 0; i < N; i++) ip[i + 1] += ip[i];
+        std::vector<int2> inc(ip[N]);
+        {
+            std::vector<int> pos(ip.begin(), ip.end() - 1);
+            for (int e = 0; e < E; e++) {
+                const int l = d->edge[2 * e], r = d->edge[2 * e + 1];
+                inc[pos[l]++] = make_int2(r, e);
+                inc[pos[r]++] = make_int2(l, e);
+            }
+            for (int b = 0; b < B; b++) inc[pos[d->bedge[2 * b]]++] = make_int2(N + b, E + b);
+        }
+        // BSR pattern, Create_CRS_SolverBlockMatrix (Euler3D/src/Solver.cpp:612-675):
+        // row i holds i and its neighbours in ascending column order.
+        m->h_ia.assign(N + 1, 0);
+        for (int i = 0; i < N; i++) m->h_ia[i + 1] = m->h_ia[i] + (d->n2n_ia[i + 1] - d->n2n_ia[i]) + 1;
+        m->nnz = m->h_ia[N];
+        m->h_ja.resize(m->nnz);
+        m->h_iau.resize(N);
+        for (int i = 0; i < N; i++) {
+            int k = m->h_ia[i];
+            m->h_ja[k++] = i;
+            for (int q = d->n2n_ia[i]; q < d->n2n_ia[i + 1]; q++) m->h_ja[k++] = d->n2n_ja[q];
+            std::sort(m->h_ja.begin() + m->h_ia[i], m->h_ja.begin() + m->h_ia[i + 1]);
+            for (int q = m->h_ia[i]; q < m->h_ia[i + 1]; q++)
+                if (m->h_ja[q] == i) m->h_iau[i] = q;
+        }
+        std::vector<int> off(inc.size(), -1);
+        for (int i = 0; i < N; i++)
+            for (int k = ip[i]; k < ip[i + 1]; k++) {
+                const int j = inc[k].x;
+                if (j >= N) continue;
+                auto b0 = m->h_ja.begin() + m->h_ia[i], b1 = m->h_ja.begin() + m->h_ia[i + 1];
+                auto it = std::lower_bound(b0, b1, j);
+                if (it == b1 || *it != j) {
+                    cfd_mesh_destroy(m);
+                    return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: edge (%d,%d) missing from n2n lists", i, j);
+                }
+                off[k] = (int)(it - m->h_ja.begin());
+            }
+        std::vector<int> lrows, urows;
+        build_levels(m->h_ia, m->h_ja, m->h_iau, N, false, m->lower.ptr, lrows, m->lower.maxwidth);
+        build_levels(m->h_ia, m->h_ja, m->h_iau, N, true, m->upper.ptr, urows, m->upper.maxwidth);
+        m->lower.nlevels = (int)m->lower.ptr.size() - 1;
+        m->upper.nlevels = (int)m->upper.ptr.size() - 1;
+
+        // packed node records: q (freestream), coordinates
+        std::vector<double> qa((size_t)kQA * m->NT, 0.0), sa((size_t)4 * (E + B), 0.0);
+        double q5[5];
+        freestream_q(p, q5);
+        for (int i = 0; i < m->NT; i++) {
+            for (int k = 0; k < 5; k++) qa[(size_t)kQA * i + k] = q5[k];
+            if (i < N)
+                for (int k = 0; k < 3; k++) qa[(size_t)kQA * i + 5 + k] = d->xyz[3 * (size_t)i + k];
+        }
+        for (int e = 0; e < E; e++)
+            for (int k = 0; k < 3; k++) sa[4 * (size_t)e + k] = d->edge_area[3 * (size_t)e + k];
+        for (int b = 0; b < B; b++)
+            for (int k = 0; k < 3; k++) sa[4 * ((size_t)E + b) + k] = d->bedge_area[3 * (size_t)b + k];
+        std::vector<int> bnode(B), btag(B, 0);
+        for (int b = 0; b < B; b++) {
+            bnode[b] = d->bedge[2 * b];
+            if (d->bedge_tag) btag[b] = d->bedge_tag[b];
+        }
+
+        m->inc_ptr = upload(m, ip.data(), ip.size(), &rc);
+        if (!rc) m->inc = upload(m, inc.data(), inc.size(), &rc);
+        if (!rc) m->inc_off = upload(m, off.data(), off.size(), &rc);
+        if (!rc) m->n2n_ia = upload(m, d->n2n_ia, (size_t)N + 1, &rc);
+        if (!rc) m->n2n_ja = upload(m, d->n2n_ja, (size_t)d->n2n_ia[N], &rc);
+        if (!rc) m->bnode = upload(m, bnode.data(), bnode.size(), &rc);
+        if (!rc) m->btype = upload(m, d->bedge_type, (size_t)B, &rc);
+        if (!rc) m->btag = upload(m, btag.data(), btag.size(), &rc);
+        if (!rc) m->d.qa = upload(m, qa.data(), qa.size(), &rc);
+        if (!rc) m->d.sa = upload(m, sa.data(), sa.size(), &rc);
+        if (!rc) m->d.vol = upload(m, d->vol, (size_t)N, &rc);
+        if (!rc) m->ia = upload(m, m->h_ia.data(), m->h_ia.size(), &rc);
+        if (!rc) m->ja = upload(m, m->h_ja.data(), m->h_ja.size(), &rc);
+        if (!rc) m->iau = upload(m, m->h_iau.data(), m->h_iau.size(), &rc);
+        if (!rc) m->lower.rows = upload(m, lrows.data(), lrows.size(), &rc);
+        if (!rc) m->upper.rows = upload(m, urows.data(), urows.size(), &rc);
+        if (rc) {
+            cfd_mesh_destroy(m);
+            return rc;
+        }
+        m->d.qb = (double *)cfd_dalloc(m, sizeof(double) * kQB * (size_t)N);
+        m->d.lsq = (double *)cfd_dalloc(m, sizeof(double) * 8 * (size_t)N);
+        m->d.rc = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
+        m->d.rd = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
+        m->d.dta = (double *)cfd_dalloc(m, sizeof(double) * (size_t)N);
+        m->dt = (double *)cfd_dalloc(m, sizeof(double) * (size_t)N);
+        m->b = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
+        m->x = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
+        m->mark = (int *)cfd_dalloc(m, sizeof(int) * (size_t)N);
+        m->red = (double *)cfd_dalloc(m, sizeof(double) * 1024 * 65);
+        m->redout = (double *)cfd_dalloc(m, sizeof(double) * 4 * 65);
+        m->flag = (int *)cfd_dalloc(m, sizeof(int) * 4);
+        m->eigbits = (unsigned long long *)cfd_dalloc(m, sizeof(unsigned long long) * 6);
+        if (!m->d.qb || !m->d.lsq || !m->d.rc || !m->d.rd || !m->d.dta || !m->dt || !m->b || !m->x || !m->mark ||
+            !m->red || !m->redout || !m->flag || !m->eigbits) {
+            cfd_mesh_destroy(m);
+            return cfd_set_error(ctx, CFD_E_OOM, "cfd_mesh_create: out of device memory");
+        }
+        if (hipHostMalloc((void **)&m->hred, sizeof(double) * 4 * 65, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&m->hflag, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess) {
+            cfd_mesh_destroy(m);
+            return cfd_set_error(ctx, CFD_E_OOM, "cfd_mesh_create: pinned allocation failed");
+        }
+        hipMemsetAsync(m->d.qb, 0, sizeof(double) * kQB * (size_t)N, m->stream);
+        hipMemsetAsync(m->mark, 0, sizeof(int) * (size_t)N, m->stream);
+        hipMemsetAsync(m->flag, 0, sizeof(int) * 4, m->stream);
+        hipMemsetAsync(m->d.rc, 0, sizeof(double) * 5 * (size_t)N, m->stream);
+        hipMemsetAsync(m->d.rd, 0, sizeof(double) * 5 * (size_t)N, m->stream);
+        rc = launch_lsq_coef(m);
+        if (!rc && hipStreamSynchronize(m->stream) != hipSuccess) rc = cfd_set_error(ctx, CFD_E_HIP, "LSQ setup failed");
+        if (rc) {
+            cfd_mesh_destroy(m);
+            return rc;
+        }
+    } catch (const std::bad_alloc &) {
+        cfd_mesh_destroy(m);
+        return cfd_set_error(ctx, CFD_E_OOM, "cfd_mesh_create: host out of memory");
+    }
+    *out = m;
+    return CFD_OK;
+}
+
+extern "C" int64_t cfd_mesh_device_bytes(const cfd_mesh *m) { return m ? m->bytes : 0; }
+
+extern "C" int64_t cfd_footprint_bytes(int32_t N, int32_t E, int32_t B, int32_t km, int32_t mixed) {
+    const int64_t n = N, e = E, b = B, nnz = n + 2 * e;
+    const int64_t s = mixed ? 4 : 8;
+    int64_t bytes = 0;
+    bytes += 8 * (kQA * (n + b) + kQB * n + 4 * (e + b) + n + 8 * n + 12 * n + 5 * n);  // state, geometry, residuals
+    bytes += 12 * (2 * e + b) + 4 * (n + 1) + 8 * e + 4 * (n + 1 + nnz + n) + 12 * b + 4 * n;  // topology
+    bytes += 25 * s * nnz * 2 + 25 * s * n;        // Jacobian, ILU(0) factors, inverse pivots
+    if (mixed) bytes += 8 * 25 * 0;                // FP64 values are not kept in mixed mode
+    bytes += s * 5 * n * (2 * (int64_t)km + 2) + 8 * 5 * n * 3;  // Krylov basis, work, b, x, r
+    return bytes;
+}
+
+extern "C" void *cfd_mesh_stream(cfd_mesh *m) { return m ? (void *)m->stream : nullptr; }
+
+extern "C" int cfd_mesh_sync(cfd_mesh *m) {
+    if (!m) return CFD_E_ARG;
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    return CFD_OK;
+}
+
+int cfd_ensure_fp32(cfd_mesh *m) {
+    if (!m->f_ready) {
+        const size_t N = m->N, NT = m->NT, EB = (size_t)m->E + m->B;
+        m->f.qa = (float *)cfd_dalloc(m, sizeof(float) * kQA * NT);
+        m->f.qb = (float *)cfd_dalloc(m, sizeof(float) * kQB * N);
+        m->f.sa = (float *)cfd_dalloc(m, sizeof(float) * 4 * EB);
+        m->f.vol = (float *)cfd_dalloc(m, sizeof(float) * N);
+        m->f.lsq = (float *)cfd_dalloc(m, sizeof(float) * 8 * N);
+        m->f.rc = (float *)cfd_dalloc(m, sizeof(float) * 5 * N);
+        m->f.rd = (float *)cfd_dalloc(m, sizeof(float) * 5 * N);
+        m->f.dta = (float *)cfd_dalloc(m, sizeof(float) * N);
+        if (!m->f.qa || !m->f.qb || !m->f.sa || !m->f.vol || !m->f.lsq || !m->f.rc || !m->f.rd || !m->f.dta)
+            return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (FP32 buffers)");
+        int rc = launch_convert_geometry_fp32(m);
+        if (rc) return rc;
+        m->f_ready = true;
+        m->f_version = -1;
+    }
+    if (m->f_version != m->q_version) {
+        int rc = launch_convert_state_to_fp32(m);
+        if (rc) return rc;
+        m->f_version = m->q_version;
+    }
+    return CFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fields
+// ---------------------------------------------------------------------------
+extern "C" int cfd_field_set(cfd_mesh *m, int field, const double *h) {
+    if (!m || !h) return CFD_E_ARG;
+    hipSetDevice(m->ctx->device);
+    const size_t N = m->N, NT = m->NT;
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    if (field == CFD_FIELD_Q || field == CFD_FIELD_GRAD || field == CFD_FIELD_PHI) {
+        const bool isq = field == CFD_FIELD_Q;
+        const size_t rec = isq ? kQA : kQB, n = isq ? NT : N;
+        double *dev = isq ? m->d.qa : m->d.qb;
+        std::vector<double> buf(rec * n);
+        CFD_HIP(m, hipMemcpy(buf.data(), dev, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        const int nf = field == CFD_FIELD_GRAD ? 15 : 5, o = field == CFD_FIELD_PHI ? kPHI : 0;
+        for (int f = 0; f < nf; f++)
+            for (size_t i = 0; i < n; i++) buf[rec * i + o + f] = h[f * n + i];
+        CFD_HIP(m, hipMemcpy(dev, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+        m->q_version++;
+        return CFD_OK;
+    }
+    double *dev = nullptr;
+    size_t n = 0;
+    switch (field) {
+    case CFD_FIELD_RES_CONV: dev = m->d.rc; n = 5 * N; break;
+    case CFD_FIELD_RES_DISS: dev = m->d.rd; n = 5 * N; break;
+    case CFD_FIELD_DT_ACC: dev = m->d.dta; n = N; break;
+    case CFD_FIELD_DT: dev = m->dt; n = N; break;
+    case CFD_FIELD_JAC_B: dev = m->b; n = 5 * N; break;
+    case CFD_FIELD_X: dev = m->x; n = 5 * N; break;
+    default: return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_field_set: unknown field %d", field);
+    }
+    CFD_HIP(m, hipMemcpy(dev, h, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (field == CFD_FIELD_RES_CONV || field == CFD_FIELD_RES_DISS) m->res_prec = CFD_FP64;
+    return CFD_OK;
+}
+
+extern "C" int cfd_field_get(cfd_mesh *m, int field, double *h) {
+    if (!m || !h) return CFD_E_ARG;
+    hipSetDevice(m->ctx->device);
+    const size_t N = m->N, NT = m->NT;
+    if ((field == CFD_FIELD_RES_CONV || field == CFD_FIELD_RES_DISS || field == CFD_FIELD_DT_ACC) &&
+        m->res_prec == CFD_FP32) {
+        int rc = launch_residual_to_fp64(m);
+        if (rc) return rc;
+        m->res_prec = CFD_FP64;
+    }
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    if (field == CFD_FIELD_Q || field == CFD_FIELD_GRAD || field == CFD_FIELD_PHI) {
+        const bool isq = field == CFD_FIELD_Q;
+        const size_t rec = isq ? kQA : kQB, n = isq ? NT : N;
+        std::vector<double> buf(rec * n);
+        if (!isq && m->f_ready && m->res_prec == CFD_FP32) {
+            // not reached: FP32 fields are exposed through the FP64 mirror only
+        }
+        CFD_HIP(m, hipMemcpy(buf.data(), isq ? m->d.qa : m->d.qb, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        const int nf = field == CFD_FIELD_GRAD ? 15 : 5, o = field == CFD_FIELD_PHI ? kPHI : 0;
+        for (int f = 0; f < nf; f++)
+            for (size_t i = 0; i < n; i++) h[f * n + i] = buf[rec * i + o + f];
+        return CFD_OK;
+    }
+    const double *dev = nullptr;
+    size_t n = 0;
+    switch (field) {
+    case CFD_FIELD_RES_CONV: dev = m->d.rc; n = 5 * N; break;
+    case CFD_FIELD_RES_DISS: dev = m->d.rd; n = 5 * N; break;
+    case CFD_FIELD_DT_ACC: dev = m->d.dta; n = N; break;
+    case CFD_FIELD_DT: dev = m->dt; n = N; break;
+    case CFD_FIELD_JAC_B: dev = m->b; n = 5 * N; break;
+    case CFD_FIELD_X: dev = m->x; n = 5 * N; break;
+    default: return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_field_get: unknown field %d", field);
+    }
+    CFD_HIP(m, hipMemcpy(h, dev, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return CFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hot-path entry points
+// ---------------------------------------------------------------------------
+static int check_params(cfd_mesh *m, const cfd_params *p) {
+    if (!m || !p) return CFD_E_ARG;
+    if (p->flux < 0 || p->flux > 2 || (p->order != 1 && p->order != 2) || p->limiter < 0 || p->limiter > 2 ||
+        p->gradient < 0 || p->gradient > 1 || (p->precision != CFD_FP64 && p->precision != CFD_FP32) ||
+        !(p->gamma > 1.0))
+        return cfd_set_error(m->ctx, CFD_E_ARG, "invalid cfd_params");
+    hipSetDevice(m->ctx->device);
+    return CFD_OK;
+}
+
+static int read_flags(cfd_mesh *m) {
+    CFD_HIP(m, hipMemcpyAsync(m->hflag, m->flag, 4 * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    return CFD_OK;
+}
+
+extern "C" int cfd_bc(cfd_mesh *m, const cfd_params *p, int iter) {
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    CFD_HIP(m, hipMemsetAsync(m->flag, 0, 4 * sizeof(int), m->stream));
+    if (p->precision == CFD_FP32) {
+        rc = cfd_ensure_fp32(m);
+        if (!rc) rc = launch_bc<float>(m, m->f, p, iter);
+    } else {
+        rc = launch_bc<double>(m, m->d, p, iter);
+        m->q_version++;
+    }
+    if (rc) return rc;
+    rc = read_flags(m);
+    if (rc) return rc;
+    if (m->hflag[3])
+        return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_bc: %d boundary edges with an unusable wave configuration",
+                             m->hflag[3]);
+    if (m->hflag[0])
+        return cfd_set_error(m->ctx, CFD_E_NEGATIVE_PRESSURE, "cfd_bc: negative pressure on %d boundary edges",
+                             m->hflag[0]);
+    if (p->precision == CFD_FP32) m->f_version = m->q_version;
+    return CFD_OK;
+}
+
+extern "C" int cfd_gradient(cfd_mesh *m, const cfd_params *p) {
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    if (p->precision == CFD_FP32) {
+        rc = cfd_ensure_fp32(m);
+        return rc ? rc : launch_gradient<float>(m, m->f, p);
+    }
+    return launch_gradient<double>(m, m->d, p);
+}
+
+extern "C" int cfd_limiter(cfd_mesh *m, const cfd_params *p) {
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    if (p->precision == CFD_FP32) {
+        rc = cfd_ensure_fp32(m);
+        return rc ? rc : launch_limiter<float>(m, m->f, p);
+    }
+    return launch_limiter<double>(m, m->d, p);
+}
+
+extern "C" int cfd_residual(cfd_mesh *m, const cfd_params *p, int add_time, double *eig6) {
+    (void)add_time;  // the accumulator is always produced; it is only consumed with AddTime
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    const bool stats = eig6 != nullptr;
+    if (stats) {
+        unsigned long long init[6];
+        const double mn = DBL_MAX, mx = DBL_MIN;
+        for (int t = 0; t < 3; t++) {
+            memcpy(&init[2 * t], &mn, 8);
+            memcpy(&init[2 * t + 1], &mx, 8);
+        }
+        CFD_HIP(m, hipMemcpyAsync(m->eigbits, init, sizeof(init), hipMemcpyHostToDevice, m->stream));
+    }
+    CFD_HIP(m, hipMemsetAsync(m->flag, 0, 4 * sizeof(int), m->stream));
+    if (p->precision == CFD_FP32) {
+        rc = cfd_ensure_fp32(m);
+        if (!rc) rc = launch_residual<float>(m, m->f, p, stats);
+        m->res_prec = CFD_FP32;
+    } else {
+        rc = launch_residual<double>(m, m->d, p, stats);
+        m->res_prec = CFD_FP64;
+    }
+    if (rc) return rc;
+    m->residual_valid = true;
+    if (stats) {
+        unsigned long long bits[6];
+        CFD_HIP(m, hipMemcpyAsync(bits, m->eigbits, sizeof(bits), hipMemcpyDeviceToHost, m->stream));
+        CFD_HIP(m, hipStreamSynchronize(m->stream));
+        for (int t = 0; t < 6; t++) memcpy(&eig6[t], &bits[t], 8);
+    }
+    if (p->flux == CFD_FLUX_HLLC) {
+        rc = read_flags(m);
+        if (rc) return rc;
+        if (m->hflag[1]) return cfd_set_error(m->ctx, CFD_E_HLLC, "HLLC: no wave configuration on %d nodes", m->hflag[1]);
+    }
+    return CFD_OK;
+}
+
+static double cfl_of(const cfd_params *p, int iter) {
+    // Compute_DeltaT, Euler3D/src/Time_Step.cpp:27-35
+    if ((p->cfl_ramp > 1) && (p->cfl_max > p->cfl_min)) {
+        if (iter < p->cfl_ramp)
+            return p->cfl_min + (p->cfl_max - p->cfl_min) * (((double)iter) / ((double)(p->cfl_ramp - 1)));
+        return p->cfl_max;
+    }
+    return p->cfl_max;
+}
+
+extern "C" int cfd_delta_t(cfd_mesh *m, const cfd_params *p, int iter, double *cfl) {
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    if (m->res_prec != CFD_FP64) {
+        rc = launch_residual_to_fp64(m);
+        if (rc) return rc;
+        m->res_prec = CFD_FP64;
+    }
+    const double c = cfl_of(p, iter);
+    if (cfl) *cfl = c;
+    return launch_delta_t(m, c);
+}
+
+extern "C" int cfd_residual_rms(cfd_mesh *m, double *rms) {
+    if (!m || !rms) return CFD_E_ARG;
+    hipSetDevice(m->ctx->device);
+    if (m->res_prec != CFD_FP64) {
+        int rc = launch_residual_to_fp64(m);
+        if (rc) return rc;
+        m->res_prec = CFD_FP64;
+    }
+    double s[5];
+    int rc = launch_rms(m, s);
+    if (rc) return rc;
+    double t = s[0] + s[1] + s[2] + s[3] + s[4];
+    *rms = sqrt(t / (5.0 * (double)m->N));
+    return CFD_OK;
+}
+
+static int ensure_jacobian_storage(cfd_mesh *m, int mixed) {
+    if (mixed) {
+        if (!m->af) m->af = (float *)cfd_dalloc(m, sizeof(float) * 25 * (size_t)m->nnz);
+        if (!m->af) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (FP32 Jacobian)");
+    } else {
+        if (!m->a) m->a = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->nnz);
+        if (!m->a) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (Jacobian)");
+    }
+    return CFD_OK;
+}
+
+extern "C" int cfd_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed) {
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    if (!m->residual_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "cfd_jacobian: call cfd_residual first");
+    if (m->res_prec != CFD_FP64) {
+        rc = launch_residual_to_fp64(m);
+        if (rc) return rc;
+        m->res_prec = CFD_FP64;
+    }
+    rc = ensure_jacobian_storage(m, mixed);
+    if (rc) return rc;
+    return launch_jacobian(m, p, rhs_mode, mixed);
+}
+
+extern "C" int32_t cfd_bsr_structure(cfd_mesh *m, int32_t *ia, int32_t *ja, int32_t *iau) {
+    if (!m) return CFD_E_ARG;
+    if (ia) memcpy(ia, m->h_ia.data(), sizeof(int) * m->h_ia.size());
+    if (ja) memcpy(ja, m->h_ja.data(), sizeof(int) * m->h_ja.size());
+    if (iau) memcpy(iau, m->h_iau.data(), sizeof(int) * m->h_iau.size());
+    return m->nnz;
+}
+
+extern "C" int cfd_jacobian_get(cfd_mesh *m, double *a, double *b) {
+    if (!m) return CFD_E_ARG;
+    if (!m->jac_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "cfd_jacobian_get: no Jacobian");
+    hipSetDevice(m->ctx->device);
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    if (a) {
+        if (m->jac_mixed) {
+            std::vector<float> tmp(25 * (size_t)m->nnz);
+            CFD_HIP(m, hipMemcpy(tmp.data(), m->af, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < tmp.size(); i++) a[i] = tmp[i];
+        } else {
+            CFD_HIP(m, hipMemcpy(a, m->a, sizeof(double) * 25 * (size_t)m->nnz, hipMemcpyDeviceToHost));
+        }
+    }
+    if (b) CFD_HIP(m, hipMemcpy(b, m->b, sizeof(double) * 5 * (size_t)m->N, hipMemcpyDeviceToHost));
+    return CFD_OK;
+}
+
+extern "C" int cfd_jacobian_set(cfd_mesh *m, const double *a, const double *b) {
+    if (!m || !a || !b) return CFD_E_ARG;
+    hipSetDevice(m->ctx->device);
+    int rc = ensure_jacobian_storage(m, 0);
+    if (rc) return rc;
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    CFD_HIP(m, hipMemcpy(m->a, a, sizeof(double) * 25 * (size_t)m->nnz, hipMemcpyHostToDevice));
+    CFD_HIP(m, hipMemcpy(m->b, b, sizeof(double) * 5 * (size_t)m->N, hipMemcpyHostToDevice));
+    m->jac_valid = true;
+    m->jac_mixed = false;
+    m->prec_valid = false;
+    return CFD_OK;
+}
+
+extern "C" int cfd_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st) {
+    if (!m || !o) return CFD_E_ARG;
+    hipSetDevice(m->ctx->device);
+    cfd_linsolve_stats tmp;
+    return run_linsolve(m, o, st ? st : &tmp);
+}
+
+extern "C" int cfd_update(cfd_mesh *m, double relax) {
+    if (!m) return CFD_E_ARG;
+    hipSetDevice(m->ctx->device);
+    m->q_version++;
+    return launch_update(m, relax);
+}
+
+extern "C" int cfd_forces(cfd_mesh *m, const cfd_params *p, int wall_tag, double alpha_deg, double *force3,
+                          double *lift, double *drag, double *pint) {
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    double v[4];
+    rc = launch_forces(m, p, wall_tag, v);
+    if (rc) return rc;
+    const double a = alpha_deg * M_PI / 180.0;
+    if (force3) {
+        force3[0] = v[0];
+        force3[1] = v[1];
+        force3[2] = v[2];
+    }
+    if (drag) *drag = v[0] * cos(a) + v[1] * sin(a);
+    if (lift) *lift = -v[0] * sin(a) + v[1] * cos(a);
+    if (pint) *pint = v[3];
+    return CFD_OK;
+}
+
+extern "C" int cfd_solve_steady(cfd_mesh *m, const cfd_params *p0, const cfd_solve_opts *o, cfd_solve_result *res,
+                                double *rms_hist) {
+    int rc = check_params(m, p0);
+    if (rc) return rc;
+    if (!o || !res) return CFD_E_ARG;
+    const auto t0 = std::chrono::steady_clock::now();
+    *res = cfd_solve_result{};
+    cfd_params p = *p0;
+    p.precision = CFD_FP64;  // the nonlinear residual is always FP64
+    for (int it = 0; it < o->max_iter; it++) {
+        cfd_params pi = p;
+        if (pi.order == 2 && it < o->first_order_iters) pi.order = 1;
+        rc = cfd_bc(m, &pi, it);
+        if (rc) break;
+        if (pi.order == 2) {
+            rc = cfd_gradient(m, &pi);
+            if (!rc && pi.limiter != CFD_LIM_NONE) rc = cfd_limiter(m, &pi);
+            if (rc) break;
+        }
+        rc = cfd_residual(m, &pi, 1, nullptr);
+        if (rc) break;
+        double rms = 0.0;
+        rc = cfd_residual_rms(m, &rms);
+        if (rc) break;
+        if (rms_hist) rms_hist[it] = rms;
+        if (it == 0) res->first_rms = rms;
+        res->final_rms = rms;
+        res->iterations = it;
+        if (!(rms == rms) || std::isinf(rms)) {
+            rc = cfd_set_error(m->ctx, CFD_E_NONFINITE, "non-finite residual at iteration %d", it);
+            break;
+        }
+        if (rms < o->tol) {
+            res->converged = 1;
+            break;
+        }
+        rc = cfd_delta_t(m, &pi, it, nullptr);
+        if (!rc) rc = cfd_jacobian(m, &pi, 1, o->lin.mixed);
+        if (rc) break;
+        cfd_linsolve_stats st{};
+        rc = cfd_linsolve(m, &o->lin, &st);
+        if (rc) break;
+        res->krylov_total += st.iterations;
+        rc = cfd_update(m, o->relaxation);
+        if (rc) break;
+        res->iterations = it + 1;
+    }
+    res->status = rc;
+    if (!rc) {
+        int rc2 = cfd_forces(m, &p, o->wall_tag, o->alpha_deg, res->force, &res->lift, &res->drag,
+                             &res->pressure_integral);
+        if (rc2) rc = rc2;
+    }
+    hipStreamSynchronize(m->stream);
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}

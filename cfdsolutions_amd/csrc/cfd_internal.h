@@ -1,0 +1,144 @@
+// Internal definitions shared by the translation units of libcfd_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "cfd_hip.h"
+
+struct cfd_ctx {
+    int device = 0;
+    std::string err;
+};
+
+// Device buffers of one precision for the residual path.  Node data used by
+// neighbour gathers is packed per node so one gather touches whole sectors:
+//   qa [NT][8]  = q1..q5, x, y, z        (ghost records hold q only)
+//   qb [N][24]  = grad (Q1x,Q1y,Q1z,...,Q5z), phi1..phi5, 4 pad
+//   sa [E+B][4] = area vector x, y, z, pad (interior edges, then boundary)
+//   lsq [N][8]  = r11 r12 r13 r22 r23 ro22 ro33 pad
+// Outputs keep the reference layout: rc, rd [5][N], dta [N].
+template <class T>
+struct ResidualBufs {
+    T *qa = nullptr;
+    T *qb = nullptr;
+    T *sa = nullptr;
+    T *vol = nullptr;
+    T *lsq = nullptr;
+    T *rc = nullptr, *rd = nullptr;
+    T *dta = nullptr;
+};
+constexpr int kQA = 8;
+constexpr int kQB = 24;
+constexpr int kPHI = 15;  // offset of phi in a qb record
+
+struct LevelSchedule {
+    std::vector<int> ptr;   // host: level l holds rows[ptr[l]..ptr[l+1])
+    int *rows = nullptr;    // device: rows sorted by level
+    int nlevels = 0;
+    int maxwidth = 0;
+};
+
+struct cfd_mesh {
+    cfd_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    int N = 0, E = 0, B = 0, NT = 0;
+    int64_t bytes = 0;
+    std::vector<void *> allocs;
+
+    // topology
+    int *inc_ptr = nullptr;   // [N+1]
+    int2 *inc = nullptr;      // [2E+B] {neighbour (ghost >= N), area index}, in edge order
+    int *inc_off = nullptr;   // [2E+B] BSR block of the off-diagonal (i, nbr), -1 for ghosts
+    int *n2n_ia = nullptr, *n2n_ja = nullptr;
+    int *bnode = nullptr, *btype = nullptr, *btag = nullptr;  // [B]
+    int *mark = nullptr;      // [N] limiter correction marks
+    int mark_epoch = 0;
+
+    ResidualBufs<double> d;   // FP64
+    ResidualBufs<float> f;    // FP32 (allocated on first use)
+    bool f_ready = false;
+
+    // time step and linear system (FP64 unless noted)
+    double *dt = nullptr;     // [N]
+    int nnz = 0;
+    std::vector<int> h_ia, h_ja, h_iau;
+    int *ia = nullptr, *ja = nullptr, *iau = nullptr;
+    double *a = nullptr, *b = nullptr, *x = nullptr;
+    float *af = nullptr;      // FP32 Jacobian (mixed)
+    double *lu = nullptr, *dinv = nullptr;
+    float *luf = nullptr, *dinvf = nullptr;
+    double *gslu = nullptr;   // [N][25] no-pivot LU of the diagonal blocks (block GS)
+    double *rowbuf = nullptr; // [N] per-row sums of the GS sweep
+    LevelSchedule lower, upper;
+    bool jac_valid = false, jac_mixed = false;
+    bool prec_valid = false;
+    int prec_kind = -1;
+    bool prec_mixed = false;
+
+    // Krylov workspace
+    int kry_m = 0;
+    bool kry_mixed = false;
+    double *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr;
+    float *Vf = nullptr, *Zf = nullptr, *wf = nullptr;
+    // reductions
+    double *red = nullptr;     // device scratch for partial sums
+    double *redout = nullptr;  // device final results (up to 128)
+    double *hred = nullptr;    // pinned host mirror
+    int *flag = nullptr;       // device error flags [4]
+    int *hflag = nullptr;      // pinned host mirror
+    unsigned long long *eigbits = nullptr;  // [6]
+    bool residual_valid = false;
+    int res_prec = CFD_FP64;   // precision of the last residual
+    int64_t q_version = 0, f_version = -1;  // FP32 mirror of the state is current when equal
+};
+
+// ---------------------------------------------------------------------------
+// helpers implemented in cfd_api.hip
+// ---------------------------------------------------------------------------
+int cfd_set_error(cfd_ctx *ctx, int code, const char *fmt, ...);
+int cfd_hip_check(cfd_ctx *ctx, hipError_t e, const char *what);
+void *cfd_dalloc(cfd_mesh *m, size_t bytes);  // device allocation tracked on the mesh
+int cfd_ensure_fp32(cfd_mesh *m);
+
+#define CFD_HIP(m, call)                                                   \
+    do {                                                                   \
+        hipError_t e_ = (call);                                            \
+        if (e_ != hipSuccess) return cfd_hip_check((m)->ctx, e_, #call);   \
+    } while (0)
+
+// Launch geometry: blocks are remapped so each of the 8 XCDs (dispatched
+// round-robin by blockIdx.x) works on a contiguous range of nodes, keeping the
+// neighbour gathers of a block in its own XCD's L2.
+constexpr int kXcd = 8;
+inline int xcd_grid(int nblocks) { return ((nblocks + kXcd - 1) / kXcd) * kXcd; }
+__device__ __forceinline__ int xcd_block() {
+    const int per = gridDim.x / kXcd;
+    return (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+}
+
+// ---------------------------------------------------------------------------
+// kernel launchers implemented in residual.hip / linsolve.hip
+// ---------------------------------------------------------------------------
+int launch_lsq_coef(cfd_mesh *m);
+template <class T>
+int launch_bc(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, int iter);
+template <class T>
+int launch_gradient(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);
+template <class T>
+int launch_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);
+template <class T>
+int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats);
+int launch_delta_t(cfd_mesh *m, double cfl);
+int launch_rms(cfd_mesh *m, double *sums5);
+int launch_update(cfd_mesh *m, double relax);
+int launch_forces(cfd_mesh *m, const cfd_params *p, int wall_tag, double *out4);
+int launch_convert_state_to_fp32(cfd_mesh *m);
+int launch_convert_geometry_fp32(cfd_mesh *m);
+int launch_residual_to_fp64(cfd_mesh *m);
+
+int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed);
+int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);

@@ -1,0 +1,885 @@
+// Implicit-side kernels: approximate Roe Jacobian assembly into BSR, block
+// SpMV, block ILU(0) and its triangular solves, block-Jacobi, the reference's
+// block Gauss-Seidel and LU-SGS sweeps, and restarted FGMRES.
+//
+// Row-sequential algorithms (ILU(0), Gauss-Seidel, LU-SGS, triangular solves)
+// run level by level over the dependency DAG of the BSR pattern: rows of one
+// level do not depend on each other and every row performs the same
+// operations in the same order as the sequential CPU code, so factors and
+// sweeps are bit-identical to it.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "cfd_internal.h"
+#include "flux.h"
+
+using namespace cfd;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxKrylov = 64;
+constexpr int kDotParts = 512;
+
+// ---------------------------------------------------------------------------
+// Jacobian blocks
+// ---------------------------------------------------------------------------
+// ConservativeEulerFluxJacobian, Euler3D/src/CompressibleUtils.cpp:558-609
+__device__ __forceinline__ void flux_jac(double g, const double *Qc, double nx, double ny, double nz, double *J) {
+    const double rho = Qc[0], u = Qc[1] / rho, v = Qc[2] / rho, w = Qc[3] / rho, rhoet = Qc[4];
+    const double ek = 0.5 * (u * u + v * v + w * w);
+    const double p = (g - 1.0) * (rhoet - rho * ek);
+    const double ht = (rhoet + p) / rho;
+    const double ubar = u * nx + v * ny + w * nz;
+    J[0] = 0.0;
+    J[5] = (g - 1.0) * ek * nx - u * ubar;
+    J[10] = (g - 1.0) * ek * ny - v * ubar;
+    J[15] = (g - 1.0) * ek * nz - w * ubar;
+    J[20] = ((g - 1.0) * ek - ht) * ubar;
+    J[1] = nx;
+    J[6] = ubar - (g - 2.0) * u * nx;
+    J[11] = v * nx - (g - 1.0) * u * ny;
+    J[16] = w * nx - (g - 1.0) * u * nz;
+    J[21] = ht * nx - (g - 1.0) * u * ubar;
+    J[2] = ny;
+    J[7] = u * ny - (g - 1.0) * v * nx;
+    J[12] = ubar - (g - 2.0) * v * ny;
+    J[17] = w * ny - (g - 1.0) * v * nz;
+    J[22] = ht * ny - (g - 1.0) * v * ubar;
+    J[3] = nz;
+    J[8] = u * nz - (g - 1.0) * w * nx;
+    J[13] = v * nz - (g - 1.0) * w * ny;
+    J[18] = ubar - (g - 2.0) * w * nz;
+    J[23] = ht * nz - (g - 1.0) * w * ubar;
+    J[4] = 0.0;
+    J[9] = (g - 1.0) * nx;
+    J[14] = (g - 1.0) * ny;
+    J[19] = (g - 1.0) * nz;
+    J[24] = g * ubar;
+}
+
+// MC_Matrix_Mul_Matrix, MATH/src/MC_Iterative_Jacobi.c:192-207
+__device__ __forceinline__ void mm5(const double *A, const double *B, double *C) {
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 5; k++) s += A[5 * i + k] * B[5 * k + j];
+            C[5 * i + j] = s;
+        }
+}
+
+// Compute_RoeAJacobian, Euler3D/src/Roe_Fluxes.cpp:149-273: A = M P |Lambda| P^-1 M^-1
+__device__ __forceinline__ void roe_ajac(double g, int efix, const double *QL, const double *QR, double nx, double ny,
+                                         double nz, double *A) {
+    Eos<double> L, R;
+    eos_face(g, QL, nx, ny, nz, L);
+    eos_face(g, QR, nx, ny, nz, R);
+    const double rho = sqrt(R.rho * L.rho);
+    const double sigma = rho / (L.rho + rho);
+    const double u = L.u + sigma * (R.u - L.u);
+    const double v = L.v + sigma * (R.v - L.v);
+    const double w = L.w + sigma * (R.w - L.w);
+    const double ht = L.ht + sigma * (R.ht - L.ht);
+    const double c = roe_speed_sound(g, u, v, w, ht);
+    const double ubar = u * nx + v * ny + w * nz;
+    const double phi = 0.5 * (g - 1.0) * (u * u + v * v + w * w);
+    const double M[25] = {1.0, 0.0, 0.0, 0.0, 0.0, u, rho, 0.0, 0.0, 0.0, v, 0.0, rho, 0.0, 0.0, w, 0.0, 0.0, rho, 0.0,
+                          phi / (g - 1.0), rho * u, rho * v, rho * w, 1.0 / (g - 1.0)};
+    const double Mi[25] = {1.0, 0.0, 0.0, 0.0, 0.0,
+                           -u / rho, 1.0 / rho, 0.0, 0.0, 0.0,
+                           -v / rho, 0.0, 1.0 / rho, 0.0, 0.0,
+                           -w / rho, 0.0, 0.0, 1.0 / rho, 0.0,
+                           phi, -u * (g - 1.0), -v * (g - 1.0), -w * (g - 1.0), (g - 1.0)};
+    const double P[25] = {nx, ny, nz, rho / c, rho / c, 0.0, -nz, ny, nx, -nx, nz, 0.0, -nx, ny, -ny,
+                          -ny, nx, 0.0, nz, -nz, 0.0, 0.0, 0.0, rho * c, rho * c};
+    const double Pi[25] = {nx, 0.0, nz, -ny, -nx / (c * c),
+                           ny, -nz, 0.0, nx, -ny / (c * c),
+                           nz, ny, -nx, 0.0, -nz / (c * c),
+                           0.0, 0.5 * nx, 0.5 * ny, 0.5 * nz, 1.0 / (2.0 * rho * c),
+                           0.0, -0.5 * nx, -0.5 * ny, -0.5 * nz, 1.0 / (2.0 * rho * c)};
+    double E0, E1, E2;
+    roe_eigen(efix, L.ubar, L.c, R.ubar, R.c, ubar, c, E0, E1, E2);
+    double T[25], Ti[25];
+    mm5(M, P, T);
+    mm5(Pi, Mi, Ti);
+    // |Lambda| * Ti: the diagonal product of the reference's full multiply
+    const double Ed[5] = {E0, E0, E0, E1, E2};
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+#pragma unroll
+        for (int j = 0; j < 5; j++) Ti[5 * i + j] = Ed[i] * Ti[5 * i + j];
+    mm5(T, Ti, A);
+}
+
+template <class TA>
+struct JacArgs {
+    int N, NT;
+    const int *ip;
+    const int2 *inc;
+    const int *off, *iau;
+    const double *sa, *qa, *vol, *dt, *rc, *rd;
+    TA *a;
+    double *b;
+    double g;
+    int efix, rhs_mode;
+};
+
+// Compute_Jacobian_Approximate_Roe, Euler3D/src/Roe_Jacobian.cpp:29-228,
+// assembled by rows: node i writes its diagonal block (accumulated in edge
+// order) and the off-diagonal block of every neighbour.
+template <class TA>
+__global__ __launch_bounds__(128) void k_jacobian(JacArgs<TA> a) {
+    const int i = xcd_block() * 128 + threadIdx.x;
+    if (i >= a.N) return;
+    double qi[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) qi[k] = a.qa[kQA * (size_t)i + k];
+    double D[25];
+#pragma unroll
+    for (int t = 0; t < 25; t++) D[t] = 0.0;
+    const double vd = a.vol[i] / a.dt[i];
+#pragma unroll
+    for (int t = 0; t < 5; t++) D[6 * t] = vd;
+    for (int k = a.ip[i]; k < a.ip[i + 1]; k++) {
+        const int2 en = a.inc[k];
+        const int j = en.x;
+        const double *s = a.sa + 4 * (size_t)en.y;
+        double nx, ny, nz;
+        const double area = area_normal(s[0], s[1], s[2], nx, ny, nz);
+        const double *rj = a.qa + kQA * (size_t)j;
+        const bool iL = j > i;
+        double QL[5], QR[5];
+#pragma unroll
+        for (int t = 0; t < 5; t++) {
+            QL[t] = iL ? qi[t] : rj[t];
+            QR[t] = iL ? rj[t] : qi[t];
+        }
+        double AR[25], J[25];
+        roe_ajac(a.g, a.efix, QL, QR, nx, ny, nz, AR);
+        if (j >= a.N) {
+            flux_jac(a.g, QL, nx, ny, nz, J);
+#pragma unroll
+            for (int t = 0; t < 25; t++) D[t] += 0.5 * (J[t] + AR[t]) * area;
+            continue;
+        }
+        TA *O = a.a + 25 * (size_t)a.off[k];
+        if (iL) {
+            flux_jac(a.g, QL, nx, ny, nz, J);
+#pragma unroll
+            for (int t = 0; t < 25; t++) D[t] += 0.5 * (J[t] + AR[t]) * area;
+            flux_jac(a.g, QR, nx, ny, nz, J);
+#pragma unroll
+            for (int t = 0; t < 25; t++) O[t] = (TA)(0.0 + 0.5 * (J[t] - AR[t]) * area);
+        } else {
+            flux_jac(a.g, QR, nx, ny, nz, J);
+#pragma unroll
+            for (int t = 0; t < 25; t++) D[t] -= 0.5 * (J[t] - AR[t]) * area;
+            flux_jac(a.g, QL, nx, ny, nz, J);
+#pragma unroll
+            for (int t = 0; t < 25; t++) O[t] = (TA)(0.0 - 0.5 * (J[t] + AR[t]) * area);
+        }
+    }
+    TA *Dd = a.a + 25 * (size_t)a.iau[i];
+#pragma unroll
+    for (int t = 0; t < 25; t++) Dd[t] = (TA)D[t];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        double r = a.rc[(size_t)k * a.N + i];
+        if (a.rhs_mode == 1) r = r + a.rd[(size_t)k * a.N + i];
+        a.b[5 * (size_t)i + k] = -r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense 5x5 helpers (same operation order as the CPU oracle)
+// ---------------------------------------------------------------------------
+// Gauss-Jordan inverse with partial pivoting; returns false on a zero pivot.
+__device__ bool inv5(const double *A, double *Ai) {
+    double a[5][10];
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            a[r][c] = A[5 * r + c];
+            a[r][c + 5] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int c = 0; c < 5; c++) {
+        int piv = c;
+        for (int r = c + 1; r < 5; r++)
+            if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+        if (piv != c)
+            for (int j = 0; j < 10; j++) {
+                double t = a[c][j];
+                a[c][j] = a[piv][j];
+                a[piv][j] = t;
+            }
+        const double d = a[c][c];
+        if (d == 0.0) return false;
+        for (int j = 0; j < 10; j++) a[c][j] = a[c][j] / d;
+        for (int r = 0; r < 5; r++) {
+            if (r == c) continue;
+            const double f = a[r][c];
+            for (int j = 0; j < 10; j++) a[r][j] = a[r][j] - f * a[c][j];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+#pragma unroll
+        for (int c = 0; c < 5; c++) Ai[5 * r + c] = a[r][c + 5];
+    return true;
+}
+
+template <class TM>
+__device__ __forceinline__ void bmul(const TM *A, const TM *B, TM *C) {
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            TM s = TM(0.0);
+#pragma unroll
+            for (int t = 0; t < 5; t++) s += A[5 * r + t] * B[5 * t + c];
+            C[5 * r + c] = s;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Block ILU(0), one level: Saad Alg. 10.4 in block form, identical to
+// orc_ilu0.  Factors in TM; the pivot inverse is computed in FP64.
+// ---------------------------------------------------------------------------
+template <class TM>
+__global__ __launch_bounds__(64) void k_ilu0_level(int nrows, const int *__restrict__ rows, const int *__restrict__ ia,
+                                                   const int *__restrict__ ja, const int *__restrict__ iau, TM *lu,
+                                                   TM *dinv, int *flag) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= nrows) return;
+    const int i = rows[t];
+    TM L[25], U[25], P[25];
+    for (int kk = ia[i]; kk < iau[i]; kk++) {
+        const int k = ja[kk];
+        TM *Lk = lu + 25 * (size_t)kk;
+        for (int q = 0; q < 25; q++) U[q] = Lk[q];
+        bmul(U, dinv + 25 * (size_t)k, L);
+        for (int q = 0; q < 25; q++) Lk[q] = L[q];
+        int pp = iau[k] + 1, qq = kk + 1;
+        const int pe = ia[k + 1], qe = ia[i + 1];
+        while (pp < pe && qq < qe) {
+            const int cp = ja[pp], cq = ja[qq];
+            if (cp == cq) {
+                const TM *Up = lu + 25 * (size_t)pp;
+                for (int q = 0; q < 25; q++) U[q] = Up[q];
+                bmul(L, U, P);
+                TM *T = lu + 25 * (size_t)qq;
+                for (int q = 0; q < 25; q++) T[q] = T[q] - P[q];
+                pp++;
+                qq++;
+            } else if (cp < cq) {
+                pp++;
+            } else {
+                qq++;
+            }
+        }
+    }
+    double Dg[25], Di[25];
+    const TM *Dp = lu + 25 * (size_t)iau[i];
+    for (int q = 0; q < 25; q++) Dg[q] = (double)Dp[q];
+    if (!inv5(Dg, Di)) {
+        atomicAdd(flag + 2, 1);
+        for (int q = 0; q < 25; q++) Di[q] = 0.0;
+    }
+    for (int q = 0; q < 25; q++) dinv[25 * (size_t)i + q] = (TM)Di[q];
+}
+
+// Inverse diagonal blocks (block-Jacobi).
+template <class TA, class TM>
+__global__ __launch_bounds__(kBlock) void k_block_inv(int N, const int *iau, const TA *a, TM *dinv, int *flag) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    double Dg[25], Di[25];
+    for (int q = 0; q < 25; q++) Dg[q] = (double)a[25 * (size_t)iau[i] + q];
+    if (!inv5(Dg, Di)) {
+        atomicAdd(flag + 2, 1);
+        for (int q = 0; q < 25; q++) Di[q] = 0.0;
+    }
+    for (int q = 0; q < 25; q++) dinv[25 * (size_t)i + q] = (TM)Di[q];
+}
+
+// Triangular solves of ILU(0), one level; 5 lanes per row (lane r owns
+// component r).  Forward: z_i = r_i - sum_{k<i} L_ik z_k.  Backward:
+// z_i = D_i^-1 (z_i - sum_{k>i} U_ik z_k).
+template <class TM, class TV>
+__global__ __launch_bounds__(kBlock) void k_trsv_level(int nrows, const int *__restrict__ rows,
+                                                       const int *__restrict__ ia, const int *__restrict__ ja,
+                                                       const int *__restrict__ iau, const TM *__restrict__ lu,
+                                                       const TM *__restrict__ dinv, const TV *__restrict__ rhs, TV *z,
+                                                       int backward) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / 5, r = lane - 5 * slot;
+    const int t = (blockIdx.x * (kBlock / 64) + wave) * 12 + slot;
+    const bool active = slot < 12 && t < nrows;
+    const int i = active ? rows[t] : 0;
+    TV y = TV(0.0);
+    if (active) {
+        if (!backward) {
+            y = rhs[5 * (size_t)i + r];
+            for (int kk = ia[i]; kk < iau[i]; kk++) {
+                const TM *A = lu + 25 * (size_t)kk + 5 * r;
+                const TV *zj = z + 5 * (size_t)ja[kk];
+                TV s = TV(0.0);
+#pragma unroll
+                for (int c = 0; c < 5; c++) s += (TV)A[c] * zj[c];
+                y = y - s;
+            }
+        } else {
+            y = z[5 * (size_t)i + r];
+            for (int kk = iau[i] + 1; kk < ia[i + 1]; kk++) {
+                const TM *A = lu + 25 * (size_t)kk + 5 * r;
+                const TV *zj = z + 5 * (size_t)ja[kk];
+                TV s = TV(0.0);
+#pragma unroll
+                for (int c = 0; c < 5; c++) s += (TV)A[c] * zj[c];
+                y = y - s;
+            }
+        }
+    }
+    if (!backward) {
+        if (active) z[5 * (size_t)i + r] = y;
+        return;
+    }
+    // gather the row's 5 components from its lanes, then apply D_i^-1
+    const int base = lane - r;
+    TV yy[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) yy[c] = __shfl(y, base + c, 64);
+    if (active) {
+        const TM *D = dinv + 25 * (size_t)i + 5 * r;
+        TV s = TV(0.0);
+#pragma unroll
+        for (int c = 0; c < 5; c++) s += (TV)D[c] * yy[c];
+        z[5 * (size_t)i + r] = s;
+    }
+}
+
+// z = D^-1 r (block-Jacobi apply)
+template <class TM, class TV>
+__global__ __launch_bounds__(kBlock) void k_bjac_apply(int N, const TM *dinv, const TV *rhs, TV *z) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= 5 * N) return;
+    const int i = t / 5, r = t - 5 * i;
+    const TM *D = dinv + 25 * (size_t)i + 5 * r;
+    const TV *x = rhs + 5 * (size_t)i;
+    TV s = TV(0.0);
+#pragma unroll
+    for (int c = 0; c < 5; c++) s += (TV)D[c] * x[c];
+    z[t] = s;
+}
+
+// y = A x, one thread per (row, component); sum over blocks then columns.
+template <class TA, class TX, class TY>
+__global__ __launch_bounds__(kBlock) void k_spmv(int N, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                 const TA *__restrict__ a, const TX *__restrict__ x, TY *__restrict__ y,
+                                                 const double *__restrict__ bsub) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= 5 * N) return;
+    const int i = t / 5, r = t - 5 * i;
+    TY s = TY(0.0);
+    for (int kk = ia[i]; kk < ia[i + 1]; kk++) {
+        const TA *A = a + 25 * (size_t)kk + 5 * r;
+        const TX *xj = x + 5 * (size_t)ja[kk];
+#pragma unroll
+        for (int c = 0; c < 5; c++) s += (TY)A[c] * (TY)xj[c];
+    }
+    if (bsub) s = (TY)bsub[t] - s;  // residual form r = b - A x
+    y[t] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Reference sweeps (MATH/src/MC_Iterative_Jacobi.c), one level per launch
+// ---------------------------------------------------------------------------
+// No-pivot LU of the diagonal blocks, MC_Iterative_Jacobi.c:381-396
+__global__ __launch_bounds__(kBlock) void k_gs_lu(int N, const int *iau, const double *a, double *lu) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= N) return;
+    double L[25];
+    for (int q = 0; q < 25; q++) L[q] = a[25 * (size_t)iau[r] + q];
+    for (int k = 0; k < 4; k++) {
+        for (int j = k + 1; j < 5; j++) L[5 * j + k] = L[5 * j + k] / L[5 * k + k];
+        for (int j = k + 1; j < 5; j++)
+            for (int i = k + 1; i < 5; i++) L[5 * i + j] -= L[5 * i + k] * L[5 * k + j];
+    }
+    for (int q = 0; q < 25; q++) lu[25 * (size_t)r + q] = L[q];
+}
+
+// One forward Gauss-Seidel row update, MC_Iterative_Jacobi.c:410-450
+__global__ __launch_bounds__(kBlock) void k_gs_level(int nrows, const int *rows, const int *ia, const int *ja,
+                                                     const int *iau, const double *a, const double *b,
+                                                     const double *lu, double *x, double *rowrms) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nrows) return;
+    const int r = rows[t];
+    double rhs[5], tv[5];
+    for (int m = 0; m < 5; m++) rhs[m] = b[5 * (size_t)r + m];
+    for (int j = ia[r]; j < ia[r + 1]; j++) {
+        const double *A = a + 25 * (size_t)j, *xj = x + 5 * (size_t)ja[j];
+        for (int ii = 0; ii < 5; ii++) tv[ii] = 0.0;
+        for (int ii = 0; ii < 5; ii++)
+            for (int jj = 0; jj < 5; jj++) tv[ii] += A[5 * ii + jj] * xj[jj];
+        for (int m = 0; m < 5; m++) rhs[m] -= tv[m];
+    }
+    double s = 0.0;
+    for (int m = 0; m < 5; m++) s += rhs[m] * rhs[m];
+    rowrms[r] = s;
+    const double *D = a + 25 * (size_t)iau[r];
+    double *xr = x + 5 * (size_t)r;
+    for (int ii = 0; ii < 5; ii++) tv[ii] = 0.0;
+    for (int ii = 0; ii < 5; ii++)
+        for (int jj = 0; jj < 5; jj++) tv[ii] += D[5 * ii + jj] * xr[jj];
+    for (int m = 0; m < 5; m++) rhs[m] += tv[m];
+    const double *L = lu + 25 * (size_t)r;
+    for (int ii = 0; ii < 5; ii++) {
+        tv[ii] = rhs[ii];
+        for (int jj = 0; jj < ii; jj++) tv[ii] -= L[5 * ii + jj] * tv[jj];
+    }
+    double xn[5];
+    for (int ii = 4; ii >= 0; ii--) {
+        xn[ii] = tv[ii];
+        for (int jj = 4; jj > ii; jj--) xn[ii] -= L[5 * ii + jj] * xn[jj];
+        xn[ii] = xn[ii] / L[5 * ii + ii];
+    }
+    for (int ii = 0; ii < 5; ii++) xr[ii] = xn[ii];
+}
+
+// LU-SGS row update with pivoted LU of the diagonal recomputed per sweep,
+// MC_Iterative_Jacobi.c:586-678 and MATH/src/LinearAlgebra.c:20-41,117-155
+__global__ __launch_bounds__(kBlock) void k_lusgs_level(int nrows, const int *rows, const int *ia, const int *ja,
+                                                        const int *iau, const double *a, const double *b, double *x) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nrows) return;
+    const int r = rows[t];
+    double rhs[5], tv[5];
+    for (int j = 0; j < 5; j++) rhs[j] = b[5 * (size_t)r + j];
+    const int diag = iau[r];
+    for (int j = ia[r]; j < ia[r + 1]; j++) {
+        if (j == diag) continue;
+        const double *A = a + 25 * (size_t)j, *xj = x + 5 * (size_t)ja[j];
+        for (int ii = 0; ii < 5; ii++) tv[ii] = 0.0;
+        for (int ii = 0; ii < 5; ii++)
+            for (int jj = 0; jj < 5; jj++) tv[ii] += A[5 * ii + jj] * xj[jj];
+        for (int m = 0; m < 5; m++) rhs[m] -= tv[m];
+    }
+    double A[5][5];
+    int P[5];
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) A[i][j] = a[25 * (size_t)diag + 5 * i + j];
+    for (int i = 0; i < 5; i++) P[i] = i;
+    for (int c = 0; c < 5; c++) {
+        double pm = A[P[c]][c];
+        int pi = c;
+        for (int i = c + 1; i < 5; i++)
+            if (fabs(A[P[i]][c]) > fabs(pm)) {
+                pm = A[P[i]][c];
+                pi = i;
+            }
+        if (pi != c) {
+            int tt = P[c];
+            P[c] = P[pi];
+            P[pi] = tt;
+        }
+        for (int i = c + 1; i < 5; i++) A[P[i]][c] /= A[P[c]][c];
+        for (int i = c + 1; i < 5; i++)
+            for (int j = c + 1; j < 5; j++) A[P[i]][j] -= A[P[i]][c] * A[P[c]][j];
+    }
+    double y[5], xs[5];
+    y[0] = rhs[P[0]];
+    for (int i = 1; i < 5; i++) {
+        y[i] = rhs[P[i]];
+        for (int j = 0; j < i; j++) y[i] -= y[j] * ((i == j) ? 1.0 : A[P[i]][j]);
+    }
+    xs[4] = y[4] / A[P[4]][4];
+    for (int i = 3; i >= 0; i--) {
+        xs[i] = y[i];
+        for (int j = 4; j > i; j--) xs[i] -= xs[j] * A[P[i]][j];
+        xs[i] /= A[P[i]][i];
+    }
+    for (int j = 0; j < 5; j++) x[5 * (size_t)r + j] = xs[j];
+}
+
+__global__ __launch_bounds__(kBlock) void k_lusgs_init(int N, const double *b, double *x) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= N) return;
+    double s = 0.0;
+    for (int j = 0; j < 5; j++) s += b[5 * (size_t)r + j];
+    s /= 5;
+    for (int j = 0; j < 5; j++) x[5 * (size_t)r + j] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Krylov vector kernels (deterministic reductions)
+// ---------------------------------------------------------------------------
+// part[blk][k] = sum over the block's elements of V_k . w, k < nv
+template <class TV>
+__global__ __launch_bounds__(kBlock) void k_mdot(size_t n, int nv, const TV *__restrict__ V, size_t ld,
+                                                 const TV *__restrict__ w, double *__restrict__ part) {
+    __shared__ double sh[kBlock];
+    double acc[kMaxKrylov + 1];
+    for (int k = 0; k < nv; k++) acc[k] = 0.0;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const double wi = (double)w[i];
+        for (int k = 0; k < nv; k++) acc[k] += (double)V[ld * k + i] * wi;
+    }
+    for (int k = 0; k < nv; k++) {
+        sh[threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * (kMaxKrylov + 1) + k] = sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_mdot_final(int nparts, int nv, const double *part, double *out) {
+    const int k = threadIdx.x;
+    if (k >= nv) return;
+    double s = 0.0;
+    for (int p = 0; p < nparts; p++) s += part[(size_t)p * (kMaxKrylov + 1) + k];
+    out[k] = s;
+}
+
+// w -= sum_k h_k V_k (k ascending, as CGS in the oracle)
+template <class TV>
+__global__ __launch_bounds__(kBlock) void k_maxpy(size_t n, int nv, const TV *__restrict__ V, size_t ld,
+                                                  const double *__restrict__ h, TV *__restrict__ w) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        double wi = (double)w[i];
+        for (int k = 0; k < nv; k++) wi = wi - h[k] * (double)V[ld * k + i];
+        w[i] = (TV)wi;
+    }
+}
+
+// dst = src * (1/scale) with scale read from the device (scale = out[idx])
+template <class TS, class TD>
+__global__ __launch_bounds__(kBlock) void k_scale(size_t n, const TS *src, TD *dst, double denom) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        dst[i] = (TD)((double)src[i] / denom);
+}
+
+// x += sum_k y_k Z_k
+template <class TV>
+__global__ __launch_bounds__(kBlock) void k_xupdate(size_t n, int nv, const TV *__restrict__ Z, size_t ld,
+                                                    const double *__restrict__ y, double *__restrict__ x) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        double xi = x[i];
+        for (int k = 0; k < nv; k++) xi = xi + y[k] * (double)Z[ld * k + i];
+        x[i] = xi;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_vsum_partial(size_t n, const double *v, double *part) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) acc += v[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * (kMaxKrylov + 1)] = sh[0];
+}
+
+__global__ void k_fill(size_t n, double v, double *x) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) x[i] = v;
+}
+
+int grid_cap(size_t n, int cap) {
+    size_t g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    return (int)(g < (size_t)cap ? g : cap);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed) {
+    const int nb = xcd_grid((m->N + 127) / 128);
+    if (mixed) {
+        JacArgs<float> a{m->N, m->NT, m->inc_ptr, m->inc, m->inc_off, m->iau, m->d.sa, m->d.qa, m->d.vol, m->dt,
+                         m->d.rc, m->d.rd, m->af, m->b, p->gamma, p->entropy_fix, rhs_mode};
+        hipLaunchKernelGGL(k_jacobian<float>, dim3(nb), dim3(128), 0, m->stream, a);
+    } else {
+        JacArgs<double> a{m->N, m->NT, m->inc_ptr, m->inc, m->inc_off, m->iau, m->d.sa, m->d.qa, m->d.vol, m->dt,
+                          m->d.rc, m->d.rd, m->a, m->b, p->gamma, p->entropy_fix, rhs_mode};
+        hipLaunchKernelGGL(k_jacobian<double>, dim3(nb), dim3(128), 0, m->stream, a);
+    }
+    CFD_HIP(m, hipGetLastError());
+    m->jac_valid = true;
+    m->jac_mixed = mixed != 0;
+    m->prec_valid = false;
+    return CFD_OK;
+}
+
+namespace {
+
+template <class TM>
+int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
+    if (kind == CFD_PREC_ILU0) {
+        CFD_HIP(m, hipMemcpyAsync(lu, a, sizeof(TM) * 25 * (size_t)m->nnz, hipMemcpyDeviceToDevice, m->stream));
+        for (int l = 0; l < m->lower.nlevels; l++) {
+            const int b0 = m->lower.ptr[l], nr = m->lower.ptr[l + 1] - b0;
+            hipLaunchKernelGGL(k_ilu0_level<TM>, dim3((nr + 63) / 64), dim3(64), 0, m->stream, nr,
+                               m->lower.rows + b0, m->ia, m->ja, m->iau, lu, dinv, m->flag);
+        }
+    } else if (kind == CFD_PREC_BJACOBI) {
+        hipLaunchKernelGGL((k_block_inv<TM, TM>), dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream,
+                           m->N, m->iau, a, dinv, m->flag);
+    }
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+template <class TM, class TV>
+void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r, TV *z) {
+    const size_t L = 5 * (size_t)m->N;
+    if (kind == CFD_PREC_ILU0) {
+        for (int pass = 0; pass < 2; pass++) {
+            const LevelSchedule &ls = pass ? m->upper : m->lower;
+            for (int l = 0; l < ls.nlevels; l++) {
+                const int b0 = ls.ptr[l], nr = ls.ptr[l + 1] - b0;
+                const int nwaves = (nr + 11) / 12;
+                hipLaunchKernelGGL((k_trsv_level<TM, TV>), dim3((nwaves + 3) / 4), dim3(kBlock), 0, m->stream, nr,
+                                   ls.rows + b0, m->ia, m->ja, m->iau, lu, dinv, r, z, pass);
+            }
+        }
+    } else if (kind == CFD_PREC_BJACOBI) {
+        hipLaunchKernelGGL((k_bjac_apply<TM, TV>), dim3((int)((L + kBlock - 1) / kBlock)), dim3(kBlock), 0, m->stream,
+                           m->N, dinv, r, z);
+    } else {
+        hipMemcpyAsync(z, r, sizeof(TV) * L, hipMemcpyDeviceToDevice, m->stream);
+    }
+}
+
+// dots of w with V_0..V_{nv-1}; results in m->redout + off (device)
+template <class TV>
+void mdot(cfd_mesh *m, int nv, const TV *V, size_t ld, const TV *w, int off) {
+    const size_t L = 5 * (size_t)m->N;
+    const int np = grid_cap(L, kDotParts);
+    hipLaunchKernelGGL(k_mdot<TV>, dim3(np), dim3(kBlock), 0, m->stream, L, nv, V, ld, w, m->red);
+    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(kMaxKrylov + 1), 0, m->stream, np, nv, m->red, m->redout + off);
+}
+
+int ensure_krylov(cfd_mesh *m, int mk, bool mixed) {
+    if (m->kry_m >= mk && m->kry_mixed == mixed && (mixed ? m->Vf != nullptr : m->V != nullptr)) return CFD_OK;
+    const size_t L = 5 * (size_t)m->N;
+    if (mixed) {
+        m->Vf = (float *)cfd_dalloc(m, sizeof(float) * L * (mk + 1));
+        m->Zf = (float *)cfd_dalloc(m, sizeof(float) * L * mk);
+        m->wf = (float *)cfd_dalloc(m, sizeof(float) * L);
+        if (!m->Vf || !m->Zf || !m->wf) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (Krylov FP32)");
+    } else {
+        m->V = (double *)cfd_dalloc(m, sizeof(double) * L * (mk + 1));
+        m->Z = (double *)cfd_dalloc(m, sizeof(double) * L * mk);
+        m->w = (double *)cfd_dalloc(m, sizeof(double) * L);
+        if (!m->V || !m->Z || !m->w) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (Krylov)");
+    }
+    if (!m->r) m->r = (double *)cfd_dalloc(m, sizeof(double) * L);
+    m->kry_m = mk;
+    m->kry_mixed = mixed;
+    return CFD_OK;
+}
+
+// Restarted right-preconditioned FGMRES(m) with CGS2; same algorithm as
+// orc_fgmres.  TM: matrix and preconditioner storage, TV: Krylov vectors.
+template <class TM, class TV>
+int fgmres(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM *lu, const TM *dinv, TV *V, TV *Z, TV *w,
+           cfd_linsolve_stats *st) {
+    const int mk = o->krylov_m;
+    const size_t L = 5 * (size_t)m->N;
+    const int gN = grid_cap(L, 4096);
+    const int gS = (int)((L + kBlock - 1) / kBlock);
+    double *hr = m->hred;
+    // x0 = 0; r0 = b
+    hipLaunchKernelGGL(k_fill, dim3(gN), dim3(kBlock), 0, m->stream, L, 0.0, m->x);
+    CFD_HIP(m, hipMemcpyAsync(m->r, m->b, sizeof(double) * L, hipMemcpyDeviceToDevice, m->stream));
+    mdot<double>(m, 1, m->r, 0, m->r, 0);
+    CFD_HIP(m, hipMemcpyAsync(hr, m->redout, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    const double bnorm = sqrt(hr[0]);
+    const double tol = o->rtol * bnorm;
+    double beta = bnorm, res = bnorm;
+    std::vector<double> H((size_t)(mk + 1) * mk, 0.0), cs(mk), sn(mk), gv(mk + 1), hcol(mk + 2), y(mk);
+    int total = 0, cycles = 0;
+    st->b_norm = bnorm;
+    for (int cycle = 0; cycle <= o->max_restarts && beta > tol && beta > 0.0; cycle++) {
+        cycles++;
+        hipLaunchKernelGGL((k_scale<double, TV>), dim3(gN), dim3(kBlock), 0, m->stream, L, m->r, V, beta);
+        for (int i = 0; i <= mk; i++) gv[i] = 0.0;
+        gv[0] = beta;
+        int kdim = 0;
+        for (int j = 0; j < mk; j++) {
+            TV *vj = V + L * j, *zj = Z + L * j;
+            apply_prec<TM, TV>(m, m->prec_kind, lu, dinv, vj, zj);
+            hipLaunchKernelGGL((k_spmv<TM, TV, TV>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a, zj,
+                               w, nullptr);
+            // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
+            mdot<TV>(m, j + 1, V, L, w, 0);
+            hipLaunchKernelGGL(k_maxpy<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, V, L, m->redout, w);
+            mdot<TV>(m, j + 1, V, L, w, kMaxKrylov + 1);
+            hipLaunchKernelGGL(k_maxpy<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, V, L,
+                               m->redout + kMaxKrylov + 1, w);
+            mdot<TV>(m, 1, w, 0, w, 2 * (kMaxKrylov + 1));
+            CFD_HIP(m, hipMemcpyAsync(hr, m->redout, sizeof(double) * (2 * (kMaxKrylov + 1) + 1),
+                                      hipMemcpyDeviceToHost, m->stream));
+            CFD_HIP(m, hipStreamSynchronize(m->stream));
+            for (int k = 0; k <= j; k++) hcol[k] = 0.0 + hr[k];
+            for (int k = 0; k <= j; k++) hcol[k] += hr[kMaxKrylov + 1 + k];
+            const double hn = sqrt(hr[2 * (kMaxKrylov + 1)]);
+            hcol[j + 1] = hn;
+            for (int k = 0; k < j; k++) {
+                const double t = cs[k] * hcol[k] + sn[k] * hcol[k + 1];
+                hcol[k + 1] = -sn[k] * hcol[k] + cs[k] * hcol[k + 1];
+                hcol[k] = t;
+            }
+            const double den = sqrt(hcol[j] * hcol[j] + hcol[j + 1] * hcol[j + 1]);
+            cs[j] = hcol[j] / den;
+            sn[j] = hcol[j + 1] / den;
+            hcol[j] = den;
+            hcol[j + 1] = 0.0;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            for (int k = 0; k <= j; k++) H[(size_t)k * mk + j] = hcol[k];
+            total++;
+            kdim = j + 1;
+            res = fabs(gv[j + 1]);
+            if (!(res == res)) return cfd_set_error(m->ctx, CFD_E_NONFINITE, "FGMRES: non-finite residual");
+            if (hn != 0.0)
+                hipLaunchKernelGGL((k_scale<TV, TV>), dim3(gN), dim3(kBlock), 0, m->stream, L, w, V + L * (j + 1), hn);
+            if (res <= tol || hn == 0.0) break;
+        }
+        for (int k = kdim - 1; k >= 0; k--) {
+            double s = gv[k];
+            for (int t = k + 1; t < kdim; t++) s -= H[(size_t)k * mk + t] * y[t];
+            y[k] = s / H[(size_t)k * mk + k];
+        }
+        for (int k = 0; k < kdim; k++) hr[k] = y[k];
+        CFD_HIP(m, hipMemcpyAsync(m->redout + 3 * (kMaxKrylov + 1), hr, sizeof(double) * kdim, hipMemcpyHostToDevice,
+                                  m->stream));
+        hipLaunchKernelGGL(k_xupdate<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, kdim, Z, L,
+                           m->redout + 3 * (kMaxKrylov + 1), m->x);
+        if (res <= tol) break;
+        // true residual for the restart
+        hipLaunchKernelGGL((k_spmv<TM, double, double>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a,
+                           m->x, m->r, m->b);
+        mdot<double>(m, 1, m->r, 0, m->r, 0);
+        CFD_HIP(m, hipMemcpyAsync(hr, m->redout, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        CFD_HIP(m, hipStreamSynchronize(m->stream));
+        beta = sqrt(hr[0]);
+        res = beta;
+    }
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    st->iterations = total;
+    st->restarts = cycles > 0 ? cycles - 1 : 0;
+    st->final_res = res;
+    return CFD_OK;
+}
+
+}  // namespace
+
+int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st) {
+    if (!m->jac_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "cfd_linsolve: no Jacobian assembled");
+    const size_t L = 5 * (size_t)m->N;
+    *st = cfd_linsolve_stats{};
+    CFD_HIP(m, hipMemsetAsync(m->flag, 0, 4 * sizeof(int), m->stream));
+    if (o->solver == CFD_SOLVER_BLOCK_GS || o->solver == CFD_SOLVER_LUSGS) {
+        if (m->jac_mixed) return cfd_set_error(m->ctx, CFD_E_ARG, "reference sweeps need an FP64 Jacobian");
+        const int sweeps = o->sweeps > 0 ? o->sweeps : (o->solver == CFD_SOLVER_BLOCK_GS ? 10000 : 100);
+        if (o->solver == CFD_SOLVER_BLOCK_GS) {
+            if (!m->gslu) m->gslu = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->N);
+            if (!m->rowbuf) m->rowbuf = (double *)cfd_dalloc(m, sizeof(double) * (size_t)m->N);
+            if (!m->gslu || !m->rowbuf) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory");
+            hipLaunchKernelGGL(k_gs_lu, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, m->iau,
+                               m->a, m->gslu);
+            hipLaunchKernelGGL(k_fill, dim3(grid_cap(L, 4096)), dim3(kBlock), 0, m->stream, L, 0.0, m->x);
+            double rms = 0.0;
+            int it;
+            for (it = 0; it < sweeps; it++) {
+                for (int l = 0; l < m->lower.nlevels; l++) {
+                    const int b0 = m->lower.ptr[l], nr = m->lower.ptr[l + 1] - b0;
+                    hipLaunchKernelGGL(k_gs_level, dim3((nr + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, nr,
+                                       m->lower.rows + b0, m->ia, m->ja, m->iau, m->a, m->b, m->gslu, m->x, m->rowbuf);
+                }
+                // row RMS: ordered reduction of the per-row sums
+                const int np = grid_cap(m->N, kDotParts);
+                hipLaunchKernelGGL(k_vsum_partial, dim3(np), dim3(kBlock), 0, m->stream, (size_t)m->N, m->rowbuf,
+                                   m->red);
+                hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(kMaxKrylov + 1), 0, m->stream, np, 1, m->red,
+                                   m->redout);
+                CFD_HIP(m, hipMemcpyAsync(m->hred, m->redout, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+                CFD_HIP(m, hipStreamSynchronize(m->stream));
+                rms = sqrt(m->hred[0] / (double)m->N);
+                if (rms < 100.0 * DBL_EPSILON) {
+                    it++;
+                    break;
+                }
+            }
+            st->iterations = it;
+            st->final_res = rms;
+        } else {
+            hipLaunchKernelGGL(k_lusgs_init, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N,
+                               m->b, m->x);
+            for (int it = 0; it < sweeps; it++) {
+                for (int pass = 0; pass < 2; pass++) {
+                    const LevelSchedule &ls = pass ? m->upper : m->lower;
+                    for (int l = 0; l < ls.nlevels; l++) {
+                        const int b0 = ls.ptr[l], nr = ls.ptr[l + 1] - b0;
+                        hipLaunchKernelGGL(k_lusgs_level, dim3((nr + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream,
+                                           nr, ls.rows + b0, m->ia, m->ja, m->iau, m->a, m->b, m->x);
+                    }
+                }
+            }
+            st->iterations = sweeps;
+        }
+        CFD_HIP(m, hipGetLastError());
+        CFD_HIP(m, hipStreamSynchronize(m->stream));
+        return CFD_OK;
+    }
+    // FGMRES
+    if (o->krylov_m < 1 || o->krylov_m > kMaxKrylov)
+        return cfd_set_error(m->ctx, CFD_E_ARG, "krylov_m must be in [1, %d]", kMaxKrylov);
+    const bool mixed = o->mixed != 0;
+    if (mixed != m->jac_mixed)
+        return cfd_set_error(m->ctx, CFD_E_ARG, "Jacobian precision does not match linsolve precision");
+    int rc = ensure_krylov(m, o->krylov_m, mixed);
+    if (rc) return rc;
+    if (!m->prec_valid || m->prec_kind != o->precond || m->prec_mixed != mixed) {
+        m->prec_kind = o->precond;
+        m->prec_mixed = mixed;
+        if (mixed) {
+            if (o->precond == CFD_PREC_ILU0 && !m->luf) m->luf = (float *)cfd_dalloc(m, sizeof(float) * 25 * (size_t)m->nnz);
+            if (o->precond != CFD_PREC_NONE && !m->dinvf) m->dinvf = (float *)cfd_dalloc(m, sizeof(float) * 25 * (size_t)m->N);
+            rc = build_prec<float>(m, o->precond, m->af, m->luf, m->dinvf);
+        } else {
+            if (o->precond == CFD_PREC_ILU0 && !m->lu) m->lu = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->nnz);
+            if (o->precond != CFD_PREC_NONE && !m->dinv) m->dinv = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->N);
+            rc = build_prec<double>(m, o->precond, m->a, m->lu, m->dinv);
+        }
+        if (rc) return rc;
+        m->prec_valid = true;
+    }
+    if (mixed)
+        rc = fgmres<float, float>(m, o, m->af, m->luf, m->dinvf, m->Vf, m->Zf, m->wf, st);
+    else
+        rc = fgmres<double, double>(m, o, m->a, m->lu, m->dinv, m->V, m->Z, m->w, st);
+    if (rc) return rc;
+    CFD_HIP(m, hipMemcpyAsync(m->hflag, m->flag, 4 * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    if (m->hflag[2]) return cfd_set_error(m->ctx, CFD_E_SINGULAR, "zero pivot in %d diagonal blocks", m->hflag[2]);
+    return CFD_OK;
+}

@@ -1,0 +1,799 @@
+// Residual path kernels: characteristic BC, LSQ and Green-Gauss gradients,
+// limiters, the flux residual, local time step, RMS, update and forces.
+//
+// Every node-centred kernel owns one node per thread and accumulates its
+// contributions in the order the reference's sequential edge loops deliver
+// them (interior edges by index, then boundary edges), so the FP64 results
+// are bit-identical to the reference without atomics or colouring; each edge
+// flux is evaluated by both of its nodes.
+#include <cfloat>
+#include <cmath>
+
+#include "cfd_internal.h"
+#include "flux.h"
+
+using namespace cfd;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void flag_set(int *flag, int idx) { atomicAdd(flag + idx, 1); }
+
+// Non-negative doubles/floats order like their bit patterns.
+__device__ __forceinline__ void atomic_min_nonneg(unsigned long long *a, double v) {
+    atomicMin(a, (unsigned long long)__double_as_longlong(v));
+}
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long *a, double v) {
+    atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------
+// LSQ coefficients, Compute_Least_Square_Gradient_Coefficients,
+// Euler3D/src/Gradient.cpp:137-222 (unweighted: alpha = 1)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_lsq_coef(int N, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                     const double *__restrict__ qa, double *__restrict__ lsq) {
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const double xi = qa[kQA * (size_t)i + 5], yi = qa[kQA * (size_t)i + 6], zi = qa[kQA * (size_t)i + 7];
+    double r11 = 0.0, r12 = 0.0, r13 = 0.0, r22 = 0.0, r23 = 0.0, ro22 = 0.0, ro33 = 0.0;
+    const int b = ia[i], e = ia[i + 1];
+    for (int k = b; k < e; k++) {
+        const size_t j = ja[k];
+        double dx = 1.0 * (qa[kQA * j + 5] - xi), dy = 1.0 * (qa[kQA * j + 6] - yi), dz = 1.0 * (qa[kQA * j + 7] - zi);
+        r11 += dx * dx;
+        r12 += dx * dy;
+        r13 += dx * dz;
+    }
+    for (int k = b; k < e; k++) {
+        const size_t j = ja[k];
+        double dx = 1.0 * (qa[kQA * j + 5] - xi), dy = 1.0 * (qa[kQA * j + 6] - yi), dz = 1.0 * (qa[kQA * j + 7] - zi);
+        double t = dy - (r12 / r11) * dx;
+        r22 += t * t;
+        r23 += t * dz;
+        ro22 += t * dy;
+    }
+    for (int k = b; k < e; k++) {
+        const size_t j = ja[k];
+        double dx = 1.0 * (qa[kQA * j + 5] - xi), dy = 1.0 * (qa[kQA * j + 6] - yi), dz = 1.0 * (qa[kQA * j + 7] - zi);
+        double t = dz - (r13 / r11) * dx - (r23 / r22) * (dy - (r12 / r11) * dx);
+        ro33 += t * dz;
+    }
+    double *o = lsq + 8 * (size_t)i;
+    o[0] = r11;
+    o[1] = r12;
+    o[2] = r13;
+    o[3] = r22;
+    o[4] = r23;
+    o[5] = ro22;
+    o[6] = ro33;
+    o[7] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic boundary condition, Compute_Characteristic_BoundaryCondition_New,
+// Euler3D/src/BC.cpp:381-613, one thread per boundary edge.
+// ---------------------------------------------------------------------------
+template <class T>
+struct BcArgs {
+    int N, E, B, iter, zpg;
+    const int *bnode, *btype;
+    const T *sa;
+    T *qa;
+    T g, inf_rho, inf_u, inf_v, inf_w, inf_p, out_p;
+    int *flag;
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bc(BcArgs<T> a) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= a.B) return;
+    const size_t nl = a.bnode[e], nr = (size_t)a.N + e;
+    T QL[5], QR[5], Qa[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        QL[k] = a.qa[kQA * nl + k];
+        QR[k] = a.qa[kQA * nr + k];
+    }
+    const T *s = a.sa + 4 * ((size_t)a.E + e);
+    T nx, ny, nz;
+    area_normal(s[0], s[1], s[2], nx, ny, nz);
+    const T g = a.g;
+    Eos<T> I, Z;
+    eos_face(g, QL, nx, ny, nz, I);
+#pragma unroll
+    for (int k = 0; k < 5; k++) Qa[k] = T(0.5) * (QL[k] + QR[k]);
+    eos_face(g, Qa, nx, ny, nz, Z);
+    const T l1 = Z.ubar, l4 = l1 + Z.c, l5 = l1 - Z.c;
+    const T infP = a.inf_p;
+    T pb = infP, rb = a.inf_rho, ub = a.inf_u, vb = a.inf_v, wb = a.inf_w;
+    int type = a.btype[e], inst = type;
+    bool bad = false;
+    switch (type) {
+    case 0: break;  // Euler wall
+    case 1:         // far field
+        if ((l1 <= T(0.0)) && (l4 <= T(0.0)) && (l5 <= T(0.0))) inst = 10;
+        else if ((l1 >= T(0.0)) && (l4 >= T(0.0)) && (l5 >= T(0.0))) inst = 11;
+        else if ((l1 >= T(0.0)) && (l5 <= T(0.0)) && (l4 >= T(0.0))) inst = 9;
+        else if ((l1 <= T(0.0)) && (l4 >= T(0.0)) && (l5 <= T(0.0))) inst = 8;
+        else bad = true;
+        break;
+    case 2:  // inflow
+        if ((l1 <= T(0.0)) && (l4 >= T(0.0)) && (l5 <= T(0.0))) inst = 4;
+        else if ((l1 <= T(0.0)) && (l4 <= T(0.0)) && (l5 <= T(0.0))) inst = 6;
+        else inst = (fabs(l1) < Z.c) ? 4 : 6;
+        break;
+    case 3:  // outflow
+        if ((l1 >= T(0.0)) && (l5 <= T(0.0)) && (l4 >= T(0.0))) inst = 5;
+        else if ((l1 >= T(0.0)) && (l4 >= T(0.0)) && (l5 >= T(0.0))) inst = 7;
+        else inst = (fabs(l1) < Z.c) ? 5 : 7;
+        break;
+    case 4: case 5: case 6: case 7: break;
+    default: bad = true;
+    }
+    switch (inst) {
+    case 0:
+        pb = I.p + Z.rho * Z.c * I.ubar;
+        if (a.iter < a.zpg && pb < T(0.0)) pb = I.p;
+        rb = I.rho + (pb - I.p) / (Z.c * Z.c);
+        ub = I.u - nx * (pb - I.p) / (Z.rho * Z.c);
+        vb = I.v - ny * (pb - I.p) / (Z.rho * Z.c);
+        wb = I.w - nz * (pb - I.p) / (Z.rho * Z.c);
+        break;
+    case 5:
+    case 9:
+        pb = (inst == 5) ? a.out_p : infP;
+        rb = I.rho + (pb - I.p) / (Z.c * Z.c);
+        ub = I.u - nx * (pb - I.p) / (Z.rho * Z.c);
+        vb = I.v - ny * (pb - I.p) / (Z.rho * Z.c);
+        wb = I.w - nz * (pb - I.p) / (Z.rho * Z.c);
+        break;
+    case 7:
+    case 11:
+        pb = I.p;
+        rb = I.rho;
+        ub = I.u;
+        vb = I.v;
+        wb = I.w;
+        break;
+    case 8: {
+        const T temp = nx * (a.inf_u - I.u) + ny * (a.inf_v - I.v) + nz * (a.inf_w - I.w);
+        pb = T(0.5) * (infP + I.p - Z.rho * Z.c * temp);
+        rb = a.inf_rho + (pb - infP) / (Z.c * Z.c);
+        ub = a.inf_u + nx * (pb - infP) / (Z.rho * Z.c);
+        vb = a.inf_v + ny * (pb - infP) / (Z.rho * Z.c);
+        wb = a.inf_w + nz * (pb - infP) / (Z.rho * Z.c);
+        break;
+    }
+    default: break;  // 4, 6, 10: freestream
+    }
+    if (bad) {
+        flag_set(a.flag, 3);
+        return;
+    }
+    if (pb < T(0.0)) flag_set(a.flag, 0);
+    // Get_TotalEnergy, Euler3D/src/Material.cpp:1012-1040
+    T et = ub * ub + vb * vb + wb * wb;
+    et = pb / ((g - T(1.0)) * rb) + T(0.5) * et;
+    T *o = a.qa + kQA * nr;
+    o[0] = rb;
+    o[1] = rb * ub;
+    o[2] = rb * vb;
+    o[3] = rb * wb;
+    o[4] = rb * et;
+}
+
+// ---------------------------------------------------------------------------
+// Gradients
+// ---------------------------------------------------------------------------
+// Compute_Least_Square_Gradient, Euler3D/src/Gradient.cpp:227-295
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_lsq_grad(int N, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                     const T *__restrict__ lsq, const T *__restrict__ qa,
+                                                     T *__restrict__ qb) {
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const T *ra = qa + kQA * (size_t)i;
+    T qi[5], xi = ra[5], yi = ra[6], zi = ra[7];
+#pragma unroll
+    for (int k = 0; k < 5; k++) qi[k] = ra[k];
+    const T *c = lsq + 8 * (size_t)i;
+    const T r11 = c[0], r12 = c[1], r13 = c[2], r22 = c[3], r23 = c[4], ro22 = c[5], ro33 = c[6];
+    T gs[15];
+#pragma unroll
+    for (int f = 0; f < 15; f++) gs[f] = T(0.0);
+    const int b = ia[i], e = ia[i + 1];
+    for (int k = b; k < e; k++) {
+        const T *rj = qa + kQA * (size_t)ja[k];
+        const T alpha = T(1.0);
+        T dx = alpha * (rj[5] - xi), dy = alpha * (rj[6] - yi), dz = alpha * (rj[7] - zi);
+        T Wz = (dz - (r13 / r11) * dx - (r23 / r22) * (dy - (r12 / r11) * dx)) / ro33;
+        T Wy = (dy - (r12 / r11) * dx - r23 * Wz) / ro22;
+        T Wx = (dx - r12 * Wy - r13 * Wz) / r11;
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            T d = rj[f] - qi[f];
+            gs[3 * f + 0] += Wx * alpha * d;
+            gs[3 * f + 1] += Wy * alpha * d;
+            gs[3 * f + 2] += Wz * alpha * d;
+        }
+    }
+    T *o = qb + kQB * (size_t)i;
+#pragma unroll
+    for (int f = 0; f < 15; f++) o[f] = gs[f];
+}
+
+// Median-dual edge-based Green-Gauss:
+//   grad_i = (1/V_i) [ sum_edges +-1/2 (Q_L+Q_R) S_LR + sum_bnd Q_i S_b ]
+// accumulated in edge order (the reference has no 3D Green-Gauss).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict__ ip, const int2 *__restrict__ inc,
+                                                    const T *__restrict__ sa, const T *__restrict__ vol,
+                                                    const T *__restrict__ qa, T *__restrict__ qb) {
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const T *ra = qa + kQA * (size_t)i;
+    T qi[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) qi[k] = ra[k];
+    T gs[15];
+#pragma unroll
+    for (int f = 0; f < 15; f++) gs[f] = T(0.0);
+    for (int k = ip[i]; k < ip[i + 1]; k++) {
+        const int2 en = inc[k];
+        const T *s = sa + 4 * (size_t)en.y;
+        const T s0 = s[0], s1 = s[1], s2 = s[2];
+        if (en.x >= N) {
+#pragma unroll
+            for (int f = 0; f < 5; f++) {
+                gs[3 * f + 0] += qi[f] * s0;
+                gs[3 * f + 1] += qi[f] * s1;
+                gs[3 * f + 2] += qi[f] * s2;
+            }
+        } else {
+            const T *rj = qa + kQA * (size_t)en.x;
+            const bool iL = en.x > i;
+#pragma unroll
+            for (int f = 0; f < 5; f++) {
+                const T qf = T(0.5) * (qi[f] + rj[f]);
+                if (iL) {
+                    gs[3 * f + 0] += qf * s0;
+                    gs[3 * f + 1] += qf * s1;
+                    gs[3 * f + 2] += qf * s2;
+                } else {
+                    gs[3 * f + 0] -= qf * s0;
+                    gs[3 * f + 1] -= qf * s1;
+                    gs[3 * f + 2] -= qf * s2;
+                }
+            }
+        }
+    }
+    const T v = vol[i];
+    T *o = qb + kQB * (size_t)i;
+#pragma unroll
+    for (int f = 0; f < 15; f++) o[f] = gs[f] / v;
+}
+
+// ---------------------------------------------------------------------------
+// Limiters, Euler3D/src/Limiters.cpp:20-304
+// ---------------------------------------------------------------------------
+template <class T, int KIND>  // 1 Venkatakrishnan, 2 Barth-Jespersen
+__global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                    const T *__restrict__ qa, T *__restrict__ qb,
+                                                    const T *__restrict__ vol, T g, T frac, T k3) {
+    const int c = xcd_block() * kBlock + threadIdx.x;
+    if (c >= N) return;
+    const T *ra = qa + kQA * (size_t)c;
+    T qc[5], xc = ra[5], yc = ra[6], zc = ra[7];
+#pragma unroll
+    for (int k = 0; k < 5; k++) qc[k] = ra[k];
+    T *rb = qb + kQB * (size_t)c;
+    T gr[15];
+#pragma unroll
+    for (int f = 0; f < 15; f++) gr[f] = rb[f];
+    T ph[5], qmin[5], qmax[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        ph[k] = T(1.0);
+        qmin[k] = qmax[k] = qc[k];
+    }
+    const int b = ia[c], e = ia[c + 1];
+    for (int n = b; n < e; n++) {
+        const T *rj = qa + kQA * (size_t)ja[n];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            qmin[k] = tmin(qmin[k], rj[k]);
+            qmax[k] = tmax(qmax[k], rj[k]);
+        }
+    }
+    T eps2 = k3;
+    eps2 = T(6.0) * eps2 * vol[c] / T(M_PI);
+    for (int n = b; n < e; n++) {
+        const T *rj = qa + kQA * (size_t)ja[n];
+        const T Rx = T(0.5) * (rj[5] - xc), Ry = T(0.5) * (rj[6] - yc), Rz = T(0.5) * (rj[7] - zc);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const T qf = qc[k] + frac * (rj[k] - qc[k]) + (T(1.0) - frac) * (gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz);
+            const T dM = qf - qc[k];
+            if (KIND == 1) {
+                if (dM > T(0.0)) {
+                    const T dP = qmax[k] - qc[k];
+                    T t = (dP * dP + eps2) + T(2.0) * (dM * dP);
+                    t = t / (dP * dP + T(2.0) * dM * dM + dM * dP + eps2);
+                    ph[k] = tmin(ph[k], t);
+                } else if (dM < T(0.0)) {
+                    const T dP = qmin[k] - qc[k];
+                    T t = (dP * dP + eps2) + T(2.0) * (dM * dP);
+                    t = t / (dP * dP + T(2.0) * dM * dM + dM * dP + eps2);
+                    ph[k] = tmin(ph[k], t);
+                } else {
+                    ph[k] = tmin(ph[k], T(1.0));
+                }
+            } else {
+                if (dM > T(0.0))
+                    ph[k] = tmin(ph[k], tmin(T(1.0), (qmax[k] - qc[k]) / dM));
+                else if (dM < T(0.0))
+                    ph[k] = tmin(ph[k], tmin(T(1.0), (qmin[k] - qc[k]) / dM));
+                else
+                    ph[k] = tmin(ph[k], T(1.0));
+            }
+            ph[k] = tmax(T(0.0), ph[k]);
+        }
+    }
+    // Compute_Limiter_PressureCorrection, Limiters.cpp:273-304
+    for (int n = b; n < e; n++) {
+        const T *rj = qa + kQA * (size_t)ja[n];
+        const T Rx = T(0.5) * (rj[5] - xc), Ry = T(0.5) * (rj[6] - yc), Rz = T(0.5) * (rj[7] - zc);
+        T Q[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            Q[k] = qc[k] + ph[k] * (frac * (rj[k] - qc[k]) + (T(1.0) - frac) * (gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz));
+        const T P = (g - T(1.0)) * (Q[4] - (T(0.5) / Q[0]) * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]));
+        if (P < T(0.0)) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) ph[k] = T(0.0);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) rb[kPHI + k] = ph[k];
+}
+
+// Face state of node c towards neighbour n with limiter ph (Limiters.cpp:327-343).
+template <class T>
+__device__ __forceinline__ void face_state(const T *ra, const T *rbg, const T *ph, const T *rn, T frac, T *Q) {
+    const T Rx = T(0.5) * (rn[5] - ra[5]), Ry = T(0.5) * (rn[6] - ra[6]), Rz = T(0.5) * (rn[7] - ra[7]);
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        Q[k] = ra[k] + ph[k] * (frac * (rn[k] - ra[k]) + (T(1.0) - frac) * (rbg[3 * k] * Rx + rbg[3 * k + 1] * Ry + rbg[3 * k + 2] * Rz));
+}
+
+// Roe pressure correction (Limiters.cpp:71-110, 311-356).  The reference
+// sweeps nodes sequentially and zeroes the limiter of both ends of an edge
+// whose Roe-averaged reconstructed pressure is negative; here every edge is
+// tested with the limiter values of the previous kernel and the marked nodes
+// are zeroed afterwards (identical whenever no such edge exists).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_roe_mark(int N, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                     const T *__restrict__ qa, const T *__restrict__ qb, T g, T frac,
+                                                     int *mark, int epoch) {
+    const int c = xcd_block() * kBlock + threadIdx.x;
+    if (c >= N) return;
+    const T *ra = qa + kQA * (size_t)c;
+    const T *rb = qb + kQB * (size_t)c;
+    for (int n = ia[c]; n < ia[c + 1]; n++) {
+        const int j = ja[n];
+        const T *rj = qa + kQA * (size_t)j;
+        const T *rbj = qb + kQB * (size_t)j;
+        T QL[5], QR[5];
+        face_state(ra, rb, rb + kPHI, rj, frac, QL);
+        face_state(rj, rbj, rbj + kPHI, ra, frac, QR);
+        Eos<T> L, R;
+        eos_face(g, QL, T(0.0), T(0.0), T(0.0), L);
+        eos_face(g, QR, T(0.0), T(0.0), T(0.0), R);
+        const T rho = sqrt(R.rho * L.rho);
+        const T sigma = rho / (L.rho + rho);
+        const T u = L.u + sigma * (R.u - L.u);
+        const T v = L.v + sigma * (R.v - L.v);
+        const T w = L.w + sigma * (R.w - L.w);
+        const T ht = L.ht + sigma * (R.ht - L.ht);
+        const T q0 = rho, q1 = rho * u, q2 = rho * v, q3 = rho * w;
+        const T q4 = (rho / g) * (ht + T(0.5) * (g - T(1.0)) * (u * u + v * v + w * w));
+        const T P = (g - T(1.0)) * (q4 - (T(0.5) / q0) * (q1 * q1 + q2 * q2 + q3 * q3));
+        if (P < T(0.0)) {
+            mark[c] = epoch;
+            mark[j] = epoch;
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_roe_zero(int N, const int *__restrict__ mark, int epoch, T *qb) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= N || mark[c] != epoch) return;
+#pragma unroll
+    for (int k = 0; k < 5; k++) qb[kQB * (size_t)c + kPHI + k] = T(0.0);
+}
+
+// ---------------------------------------------------------------------------
+// Residual: Compute_Residual_Roe_New (Euler3D/src/FluxTest.cpp:7968-8151),
+// Compute_Residual_HLLC (Euler3D/src/HLLC_Fluxes.cpp:519-573), AUSM+.
+// ---------------------------------------------------------------------------
+template <class T>
+struct ResArgs {
+    int N, NT;
+    const int *ip;
+    const int2 *inc;
+    const T *sa, *qa, *qb;
+    T *rc, *rd, *dta;
+    T g, frac;
+    int efix;
+    unsigned long long *eig;
+    int *flag;
+};
+
+template <class T, int FLUX, int ORDER, bool LIM, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_residual(ResArgs<T> a) {
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= a.N) return;
+    const T *ra = a.qa + kQA * (size_t)i;
+    T qi[5], xi = T(0.0), yi = T(0.0), zi = T(0.0);
+#pragma unroll
+    for (int k = 0; k < 5; k++) qi[k] = ra[k];
+    T gi[15], pi[5];
+    if (ORDER == 2) {
+        xi = ra[5];
+        yi = ra[6];
+        zi = ra[7];
+        const T *rb = a.qb + kQB * (size_t)i;
+#pragma unroll
+        for (int f = 0; f < 15; f++) gi[f] = rb[f];
+#pragma unroll
+        for (int k = 0; k < 5; k++) pi[k] = LIM ? rb[kPHI + k] : T(1.0);
+    }
+    T ac[5], ad[5], dt = T(0.0);
+#pragma unroll
+    for (int k = 0; k < 5; k++) ac[k] = ad[k] = T(0.0);
+    T emin[3] = {T(DBL_MAX), T(DBL_MAX), T(DBL_MAX)}, emax[3] = {T(0.0), T(0.0), T(0.0)};
+    int bad = 0;
+    const int kb = a.ip[i], ke = a.ip[i + 1];
+    for (int k = kb; k < ke; k++) {
+        const int2 en = a.inc[k];
+        const int j = en.x;
+        const T *s = a.sa + 4 * (size_t)en.y;
+        const T s0 = s[0], s1 = s[1], s2 = s[2];
+        const T *rj = a.qa + kQA * (size_t)j;
+        const bool iL = j > i;  // ghosts (j >= N) always pair with a physical L
+        T QL[5], QR[5];
+        if (ORDER == 2 && j < a.N) {
+            // Compute_SecondOrderReconstructQ, Euler3D/src/HigherOrderReconstructQ.cpp:20-71
+            const T *rbj = a.qb + kQB * (size_t)j;
+            T xj = rj[5], yj = rj[6], zj = rj[7];
+            const T RxL = iL ? T(0.5) * (xj - xi) : T(0.5) * (xi - xj);
+            const T RyL = iL ? T(0.5) * (yj - yi) : T(0.5) * (yi - yj);
+            const T RzL = iL ? T(0.5) * (zj - zi) : T(0.5) * (zi - zj);
+            const T RxR = -RxL, RyR = -RyL, RzR = -RzL;
+#pragma unroll
+            for (int k2 = 0; k2 < 5; k2++) {
+                const T qj = rj[k2];
+                const T pj = LIM ? rbj[kPHI + k2] : T(1.0);
+                const T gjx = rbj[3 * k2], gjy = rbj[3 * k2 + 1], gjz = rbj[3 * k2 + 2];
+                const T qL = iL ? qi[k2] : qj, qR = iL ? qj : qi[k2];
+                const T pL = iL ? pi[k2] : pj, pR = iL ? pj : pi[k2];
+                const T gLx = iL ? gi[3 * k2] : gjx, gLy = iL ? gi[3 * k2 + 1] : gjy, gLz = iL ? gi[3 * k2 + 2] : gjz;
+                const T gRx = iL ? gjx : gi[3 * k2], gRy = iL ? gjy : gi[3 * k2 + 1], gRz = iL ? gjz : gi[3 * k2 + 2];
+                QL[k2] = qL + pL * (a.frac * (qR - qL) + (T(1.0) - a.frac) * (gLx * RxL + gLy * RyL + gLz * RzL));
+                QR[k2] = qR + pR * (a.frac * (qL - qR) + (T(1.0) - a.frac) * (gRx * RxR + gRy * RyR + gRz * RzR));
+            }
+        } else {
+#pragma unroll
+            for (int k2 = 0; k2 < 5; k2++) {
+                const T qj = rj[k2];
+                QL[k2] = iL ? qi[k2] : qj;
+                QR[k2] = iL ? qj : qi[k2];
+            }
+        }
+        T cf[5], df[5], lam;
+        if (FLUX == 0) {
+            T E0, E1, E2;
+            lam = roe_flux(a.g, a.efix, s0, s1, s2, QL, QR, cf, df, E0, E1, E2);
+            if (STATS) {
+                emin[0] = tmin(emin[0], E0);
+                emax[0] = tmax(emax[0], E0);
+                emin[1] = tmin(emin[1], E1);
+                emax[1] = tmax(emax[1], E1);
+                emin[2] = tmin(emin[2], E2);
+                emax[2] = tmax(emax[2], E2);
+            }
+        } else if (FLUX == 1) {
+            lam = hllc_flux(a.g, s0, s1, s2, QL, QR, cf, bad);
+        } else {
+            lam = ausmp_flux(a.g, s0, s1, s2, QL, QR, cf);
+        }
+        if (iL) {
+#pragma unroll
+            for (int k2 = 0; k2 < 5; k2++) {
+                ac[k2] += cf[k2];
+                if (FLUX == 0) ad[k2] += df[k2];
+            }
+        } else {
+#pragma unroll
+            for (int k2 = 0; k2 < 5; k2++) {
+                ac[k2] -= cf[k2];
+                if (FLUX == 0) ad[k2] -= df[k2];
+            }
+        }
+        dt += lam;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        a.rc[(size_t)k * a.N + i] = ac[k];
+        a.rd[(size_t)k * a.N + i] = ad[k];
+    }
+    a.dta[i] = dt;
+    if (bad) flag_set(a.flag, 1);
+    if (STATS && kb < ke) {
+        for (int t = 0; t < 3; t++) {
+            atomic_min_nonneg(a.eig + 2 * t, (double)emin[t]);
+            atomic_max_nonneg(a.eig + 2 * t + 1, (double)emax[t]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Local time step, RMS, update, forces
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_delta_t(int N, const double *vol, const double *dta, double cfl,
+                                                    double *dt) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) dt[i] = vol[i] * cfl / dta[i];
+}
+
+// Deterministic two-stage reductions: fixed per-thread strides, fixed tree.
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out) {
+    __shared__ double sh[NV][kBlock];
+#pragma unroll
+    for (int k = 0; k < NV; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) out[k] = sh[k][0];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rms_partial(int N, const double *rc, const double *rd, double *part) {
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const double r = rc[(size_t)k * N + i] + rd[(size_t)k * N + i];
+            v[k] += r * r;
+        }
+    }
+    block_reduce_store<5>(v, part + 5 * blockIdx.x);
+}
+
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_reduce_final(int nparts, const double *part, double *out) {
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += kBlock)
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] += part[NV * p + k];
+    block_reduce_store<NV>(v, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_update(int N, double relax, const double *x, double *qa) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+#pragma unroll
+    for (int k = 0; k < 5; k++) qa[kQA * (size_t)i + k] += relax * x[5 * (size_t)i + k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_forces_partial(int N, int E, int B, const int *bnode, const int *btype,
+                                                           const int *btag, int wall_tag, const double *sa,
+                                                           const double *qa, double g, double *part) {
+    double v[4] = {0, 0, 0, 0};
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < B; e += gridDim.x * kBlock) {
+        const bool use = wall_tag > 0 ? (btag && btag[e] == wall_tag) : (btype[e] == 0);
+        if (!use) continue;
+        const double *r = qa + kQA * (size_t)bnode[e];
+        Eos<double> s;
+        eos_face(g, r, 0.0, 0.0, 0.0, s);
+        const double *S = sa + 4 * ((size_t)E + e);
+        v[0] += s.p * S[0];
+        v[1] += s.p * S[1];
+        v[2] += s.p * S[2];
+        v[3] += s.p * sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+    }
+    block_reduce_store<4>(v, part + 4 * blockIdx.x);
+}
+
+template <class TD, class TS>
+__global__ void k_convert(size_t n, const TS *s, TD *d) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) d[i] = (TD)s[i];
+}
+
+int grid_for(size_t n) {
+    size_t g = (n + kBlock - 1) / kBlock;
+    return (int)(g < 4096 ? (g ? g : 1) : 4096);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int launch_lsq_coef(cfd_mesh *m) {
+    const int nb = (m->N + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_lsq_coef, dim3(xcd_grid(nb)), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, m->d.qa,
+                       m->d.lsq);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+template <class T>
+int launch_bc(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, int iter) {
+    if (m->B == 0) return CFD_OK;
+    BcArgs<T> a{m->N, m->E, m->B, iter, p->zpg_iter, m->bnode, m->btype, b.sa, b.qa, (T)p->gamma, (T)p->inf_rho,
+                (T)p->inf_u, (T)p->inf_v, (T)p->inf_w, (T)p->inf_p, (T)p->outflow_p, m->flag};
+    hipLaunchKernelGGL(k_bc<T>, dim3((m->B + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, a);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+template <class T>
+int launch_gradient(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p) {
+    const int nb = xcd_grid((m->N + kBlock - 1) / kBlock);
+    if (p->gradient == CFD_GRAD_GG)
+        hipLaunchKernelGGL(k_gg_grad<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->inc, b.sa, b.vol,
+                           b.qa, b.qb);
+    else
+        hipLaunchKernelGGL(k_lsq_grad<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, b.lsq, b.qa,
+                           b.qb);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+template <class T>
+int launch_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p) {
+    const int nb = xcd_grid((m->N + kBlock - 1) / kBlock);
+    const T frac = (p->limiter_order == 2) ? T(0.5) : T(0.0);
+    const T k3 = (T)pow(p->venkat_k, 3.0);  // host pow, as the reference's glibc pow
+    const T g = (T)p->gamma;
+    if (p->limiter == CFD_LIM_VENKAT)
+        hipLaunchKernelGGL((k_limiter<T, 1>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, b.qa,
+                           b.qb, b.vol, g, frac, k3);
+    else if (p->limiter == CFD_LIM_BJ)
+        hipLaunchKernelGGL((k_limiter<T, 2>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, b.qa,
+                           b.qb, b.vol, g, frac, k3);
+    else
+        return CFD_OK;
+    CFD_HIP(m, hipGetLastError());
+    if (p->flux == CFD_FLUX_ROE) {
+        const int epoch = ++m->mark_epoch;
+        hipLaunchKernelGGL(k_roe_mark<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, b.qa, b.qb,
+                           g, frac, m->mark, epoch);
+        hipLaunchKernelGGL(k_roe_zero<T>, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N,
+                           m->mark, epoch, b.qb);
+        CFD_HIP(m, hipGetLastError());
+    }
+    return CFD_OK;
+}
+
+template <class T, int FLUX>
+static void launch_res_flux(cfd_mesh *m, const ResArgs<T> &a, int order, bool lim, bool stats, int nb) {
+    dim3 g(nb), bl(kBlock);
+    if (order == 2) {
+        if (lim) {
+            if (stats) hipLaunchKernelGGL((k_residual<T, FLUX, 2, true, true>), g, bl, 0, m->stream, a);
+            else hipLaunchKernelGGL((k_residual<T, FLUX, 2, true, false>), g, bl, 0, m->stream, a);
+        } else {
+            if (stats) hipLaunchKernelGGL((k_residual<T, FLUX, 2, false, true>), g, bl, 0, m->stream, a);
+            else hipLaunchKernelGGL((k_residual<T, FLUX, 2, false, false>), g, bl, 0, m->stream, a);
+        }
+    } else {
+        if (stats) hipLaunchKernelGGL((k_residual<T, FLUX, 1, false, true>), g, bl, 0, m->stream, a);
+        else hipLaunchKernelGGL((k_residual<T, FLUX, 1, false, false>), g, bl, 0, m->stream, a);
+    }
+}
+
+template <class T>
+int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats) {
+    ResArgs<T> a{m->N, m->NT, m->inc_ptr, m->inc, b.sa, b.qa, b.qb, b.rc, b.rd, b.dta, (T)p->gamma,
+                 (p->limiter_order == 2) ? T(0.5) : T(0.0), p->entropy_fix, m->eigbits, m->flag};
+    const int nb = xcd_grid((m->N + kBlock - 1) / kBlock);
+    const bool lim = p->limiter != CFD_LIM_NONE;
+    const bool st = stats && p->flux == CFD_FLUX_ROE;
+    if (p->flux == CFD_FLUX_ROE) launch_res_flux<T, 0>(m, a, p->order, lim, st, nb);
+    else if (p->flux == CFD_FLUX_HLLC) launch_res_flux<T, 1>(m, a, p->order, lim, false, nb);
+    else launch_res_flux<T, 2>(m, a, p->order, lim, false, nb);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+int launch_delta_t(cfd_mesh *m, double cfl) {
+    hipLaunchKernelGGL(k_delta_t, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, m->d.vol,
+                       m->d.dta, cfl, m->dt);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+int launch_rms(cfd_mesh *m, double *sums5) {
+    const int nparts = grid_for(m->N) > 1024 ? 1024 : grid_for(m->N);
+    hipLaunchKernelGGL(k_rms_partial, dim3(nparts), dim3(kBlock), 0, m->stream, m->N, m->d.rc, m->d.rd, m->red);
+    hipLaunchKernelGGL(k_reduce_final<5>, dim3(1), dim3(kBlock), 0, m->stream, nparts, m->red, m->redout);
+    CFD_HIP(m, hipGetLastError());
+    CFD_HIP(m, hipMemcpyAsync(m->hred, m->redout, 5 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    for (int k = 0; k < 5; k++) sums5[k] = m->hred[k];
+    return CFD_OK;
+}
+
+int launch_update(cfd_mesh *m, double relax) {
+    hipLaunchKernelGGL(k_update, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, relax, m->x,
+                       m->d.qa);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+int launch_forces(cfd_mesh *m, const cfd_params *p, int wall_tag, double *out4) {
+    const int nparts = m->B ? (grid_for(m->B) > 256 ? 256 : grid_for(m->B)) : 1;
+    hipLaunchKernelGGL(k_forces_partial, dim3(nparts), dim3(kBlock), 0, m->stream, m->N, m->E, m->B, m->bnode,
+                       m->btype, m->btag, wall_tag, m->d.sa, m->d.qa, p->gamma, m->red);
+    hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), dim3(kBlock), 0, m->stream, nparts, m->red, m->redout);
+    CFD_HIP(m, hipGetLastError());
+    CFD_HIP(m, hipMemcpyAsync(m->hred, m->redout, 4 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    for (int k = 0; k < 4; k++) out4[k] = m->hred[k];
+    return CFD_OK;
+}
+
+int launch_convert_state_to_fp32(cfd_mesh *m) {
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for((size_t)kQA * m->NT)), dim3(kBlock), 0, m->stream,
+                       (size_t)kQA * m->NT, m->d.qa, m->f.qa);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for((size_t)kQB * m->N)), dim3(kBlock), 0, m->stream,
+                       (size_t)kQB * m->N, m->d.qb, m->f.qb);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+int launch_residual_to_fp64(cfd_mesh *m) {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((size_t)5 * m->N)), dim3(kBlock), 0, m->stream,
+                       (size_t)5 * m->N, m->f.rc, m->d.rc);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((size_t)5 * m->N)), dim3(kBlock), 0, m->stream,
+                       (size_t)5 * m->N, m->f.rd, m->d.rd);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((size_t)m->N)), dim3(kBlock), 0, m->stream,
+                       (size_t)m->N, m->f.dta, m->d.dta);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+int launch_convert_geometry_fp32(cfd_mesh *m) {
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for((size_t)4 * (m->E + m->B))), dim3(kBlock), 0,
+                       m->stream, (size_t)4 * (m->E + m->B), m->d.sa, m->f.sa);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for((size_t)m->N)), dim3(kBlock), 0, m->stream,
+                       (size_t)m->N, m->d.vol, m->f.vol);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for((size_t)8 * m->N)), dim3(kBlock), 0, m->stream,
+                       (size_t)8 * m->N, m->d.lsq, m->f.lsq);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+template int launch_bc<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *, int);
+template int launch_bc<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *, int);
+template int launch_gradient<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *);
+template int launch_gradient<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *);
+template int launch_limiter<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *);
+template int launch_limiter<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *);
+template int launch_residual<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *, bool);
+template int launch_residual<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *, bool);

@@ -1,0 +1,231 @@
+/*
+ * cfd_hip.h - C ABI of the MI355X (gfx950) implicit unstructured
+ * finite-volume Euler hot path (libcfd_hip.so).
+ *
+ * The reference solver (ashishgupta02/cfdsolutions, Euler3D/MPFLOW3D) keeps
+ * its state in globals and exposes argument-less C++ functions; this ABI
+ * takes the same arrays explicitly, keeps them resident in HBM behind an
+ * opaque handle, and returns status codes instead of calling exit().  Each
+ * entry point names the reference function it replaces (paths relative to
+ * Trunk/ of the reference).  A C++ shim with the reference's exact
+ * signatures is described in INTEGRATION.md.
+ *
+ * Conventions
+ *   - Arrays use the reference layouts: conserved state and residuals are
+ *     structure-of-arrays [5][n] (Q1..Q5, Res1..Res5), ghost nodes follow the
+ *     physical nodes (index n_node + k for boundary edge k), area vectors and
+ *     coordinates are interleaved x,y,z, BSR blocks are 25 doubles row-major
+ *     (A[blk][row][col]), the right-hand side and solution are [N][5].
+ *   - All pointers passed in are host pointers unless the name says device.
+ *   - Every call is ordered on the handle's own HIP stream; calls on one
+ *     handle are not thread-safe, different handles are independent.
+ *   - Return value: CFD_OK or a negative CFD_E_* code; cfd_last_error()
+ *     describes the last failure on the context.
+ */
+#ifndef CFD_HIP_H
+#define CFD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFD_OK 0
+#define CFD_E_ARG -1
+#define CFD_E_HIP -2
+#define CFD_E_OOM -3
+#define CFD_E_NONFINITE -4          /* NaN/Inf in residual or linear solve */
+#define CFD_E_NEGATIVE_PRESSURE -5  /* characteristic BC produced p < 0 */
+#define CFD_E_SINGULAR -6           /* zero pivot in a 5x5 diagonal block */
+#define CFD_E_HLLC -7               /* no HLLC wave configuration applies */
+#define CFD_E_STATE -8              /* call order: required data not computed */
+
+/* flux schemes (values of FLUX_SCHEME, Euler3D/include/SolverParameters.h:109-119) */
+#define CFD_FLUX_ROE 0
+#define CFD_FLUX_HLLC 1
+#define CFD_FLUX_AUSMP 2
+/* limiters (LIMITER_METHOD + 1) */
+#define CFD_LIM_NONE 0
+#define CFD_LIM_VENKAT 1
+#define CFD_LIM_BJ 2
+/* gradients */
+#define CFD_GRAD_LSQ 0
+#define CFD_GRAD_GG 1
+/* precision of residual-path arithmetic and storage */
+#define CFD_FP64 0
+#define CFD_FP32 1
+/* preconditioners */
+#define CFD_PREC_NONE 0
+#define CFD_PREC_BJACOBI 1
+#define CFD_PREC_ILU0 2
+/* linear solvers */
+#define CFD_SOLVER_FGMRES 0
+#define CFD_SOLVER_BLOCK_GS 1  /* MC_Iterative_Block_LU_Jacobi_CRS, forward */
+#define CFD_SOLVER_LUSGS 2     /* MC_Iterative_Block_LUSGS_CRS */
+
+/* fields for cfd_field_set / cfd_field_get */
+#define CFD_FIELD_Q 0          /* [5][N+B] */
+#define CFD_FIELD_GRAD 1       /* [15][N] Q1x,Q1y,Q1z,...,Q5z */
+#define CFD_FIELD_PHI 2        /* [5][N] */
+#define CFD_FIELD_RES_CONV 3   /* [5][N] */
+#define CFD_FIELD_RES_DISS 4   /* [5][N] */
+#define CFD_FIELD_DT_ACC 5     /* [N] sum of area*max|lambda| */
+#define CFD_FIELD_DT 6         /* [N] local time step */
+#define CFD_FIELD_JAC_B 7      /* [N][5] */
+#define CFD_FIELD_X 8          /* [N][5] linear-solve solution */
+
+typedef struct cfd_ctx cfd_ctx;
+typedef struct cfd_mesh cfd_mesh;
+
+/* Arrays of the preprocessed mesh, as produced by the reference's
+ * Create_Connectivity_Maps + Calculate_Area_Volume
+ * (Euler3D/src/Connectivity_Maps.cpp:991-1059, Euler3D/src/Area_Volume.cpp:903-917). */
+typedef struct cfd_mesh_desc {
+    int32_t n_node, n_edge, n_bedge;
+    const int32_t *edge;        /* [2E] intEdge[].node, L < R */
+    const double *edge_area;    /* [3E] intEdge[].areav */
+    const int32_t *bedge;       /* [2B] bndEdge[].node (node, n_node + k) */
+    const double *bedge_area;   /* [3B] bndEdge[].areav, outward */
+    const int32_t *bedge_type;  /* [B]  bndEdge[].type (BC_TYPE values) */
+    const int32_t *bedge_tag;   /* [B]  surface tag, may be NULL */
+    const double *xyz;          /* [3N] coordXYZ */
+    const double *vol;          /* [N]  cVolume */
+    const int32_t *n2n_ia;      /* [N+1] crs_IA_Node2Node */
+    const int32_t *n2n_ja;      /*       crs_JA_Node2Node */
+} cfd_mesh_desc;
+
+/* Solver parameters (globals of Euler3D/src/SolverParameters.cpp:14-95 and
+ * Euler3D/src/Material.cpp:16-58 for an ideal gas, GENERIC non-dimensionalisation). */
+typedef struct cfd_params {
+    double gamma;
+    double inf_rho, inf_u, inf_v, inf_w, inf_p;
+    double outflow_p;
+    int32_t flux;           /* CFD_FLUX_* */
+    int32_t order;          /* 1 or 2 */
+    int32_t limiter;        /* CFD_LIM_* */
+    int32_t limiter_order;  /* 1: reconstruction fraction 0, 2: fraction 1/2 */
+    int32_t entropy_fix;    /* 0 or 1 (Kermani) */
+    int32_t gradient;       /* CFD_GRAD_* */
+    int32_t zpg_iter;
+    int32_t precision;      /* CFD_FP64 / CFD_FP32 for gradient, limiter, residual */
+    double venkat_k;
+    double cfl_min, cfl_max;
+    int32_t cfl_ramp;
+    int32_t pad_;
+} cfd_params;
+
+typedef struct cfd_linsolve_opts {
+    int32_t solver;         /* CFD_SOLVER_* */
+    int32_t precond;        /* CFD_PREC_* (FGMRES only) */
+    int32_t krylov_m;       /* restart length, <= 64 */
+    int32_t max_restarts;
+    double rtol;            /* FGMRES: stop at ||r|| <= rtol*||b|| */
+    int32_t sweeps;         /* block GS / LU-SGS sweeps */
+    int32_t mixed;          /* 1: FP32 matrix, preconditioner and Krylov basis */
+} cfd_linsolve_opts;
+
+typedef struct cfd_linsolve_stats {
+    int32_t iterations;
+    int32_t restarts;
+    double final_res;       /* FGMRES residual estimate, GS sweep RMS */
+    double b_norm;
+} cfd_linsolve_stats;
+
+typedef struct cfd_solve_opts {
+    int32_t max_iter;
+    double tol;             /* absolute RMS of Conv+Diss, RMS_Res of Solver_Steady_Implicit */
+    int32_t first_order_iters;
+    double relaxation;
+    int32_t wall_tag;       /* surface tag for forces (<= 0: every Euler wall) */
+    double alpha_deg;       /* for lift/drag axes */
+    cfd_linsolve_opts lin;
+} cfd_solve_opts;
+
+typedef struct cfd_solve_result {
+    int32_t iterations;     /* pseudo-time (Newton) iterations */
+    int32_t krylov_total;
+    int32_t converged;
+    int32_t status;
+    double first_rms, final_rms;
+    double force[3];
+    double lift, drag, pressure_integral;
+    double seconds;
+} cfd_solve_result;
+
+/* ---- context and mesh ------------------------------------------------- */
+int cfd_ctx_create(int device, cfd_ctx **out);
+void cfd_ctx_destroy(cfd_ctx *ctx);
+const char *cfd_last_error(const cfd_ctx *ctx);
+
+/* Uploads a mesh and builds the device structures: node incidence lists in
+ * the reference's accumulation order, LSQ coefficients
+ * (Euler3D/src/Gradient.cpp:137-222), BSR structure
+ * (Create_CRS_SolverBlockMatrix, Euler3D/src/Solver.cpp:612-704) and the
+ * level schedules of the triangular sweeps.  The state is set to the
+ * freestream of params (Solver_Set_Initial_Conditions,
+ * Euler3D/src/Solver.cpp:340-362). */
+int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *desc, const cfd_params *params, cfd_mesh **out);
+void cfd_mesh_destroy(cfd_mesh *mesh);
+/* HBM bytes held by the handle (device allocations) */
+int64_t cfd_mesh_device_bytes(const cfd_mesh *mesh);
+/* Bytes a mesh of this size would hold with a Jacobian/Krylov workspace
+ * (admission control); mixed = 1 for FP32 solver storage. */
+int64_t cfd_footprint_bytes(int32_t n_node, int32_t n_edge, int32_t n_bedge, int32_t krylov_m, int32_t mixed);
+/* The handle's HIP stream (hipStream_t) for interop and timing. */
+void *cfd_mesh_stream(cfd_mesh *mesh);
+/* Blocks until all work on the handle's stream is done. */
+int cfd_mesh_sync(cfd_mesh *mesh);
+
+int cfd_field_set(cfd_mesh *mesh, int field, const double *host);
+int cfd_field_get(cfd_mesh *mesh, int field, double *host);
+
+/* ---- hot path ---------------------------------------------------------- */
+/* Apply_Boundary_Condition (Euler3D/src/BC.cpp:1440-1487, 1847-1860):
+ * characteristic ghost states for every boundary edge. */
+int cfd_bc(cfd_mesh *mesh, const cfd_params *p, int iter);
+/* Compute_Least_Square_Gradient(0) (Euler3D/src/Gradient.cpp:227-295) or
+ * the median-dual Green-Gauss gradient (p->gradient). */
+int cfd_gradient(cfd_mesh *mesh, const cfd_params *p);
+/* Compute_Limiter (Euler3D/src/Limiters.cpp:20-110). */
+int cfd_limiter(cfd_mesh *mesh, const cfd_params *p);
+/* Compute_Residual(AddTime) (Euler3D/src/Residual.cpp:18-49): convective and
+ * dissipative residual and the time-step accumulator, left on the device.
+ * eig6 (host, may be NULL) receives the eigenvalue extrema of the Roe flux. */
+int cfd_residual(cfd_mesh *mesh, const cfd_params *p, int add_time, double *eig6);
+/* Compute_DeltaT (Euler3D/src/Time_Step.cpp:23-50); returns CFL in *cfl. */
+int cfd_delta_t(cfd_mesh *mesh, const cfd_params *p, int iter, double *cfl);
+/* RMS of Conv+Diss as in Solver_Steady_Implicit.cpp:112-126. */
+int cfd_residual_rms(cfd_mesh *mesh, double *rms);
+/* Compute_Jacobian (Euler3D/src/Jacobian.cpp:18-47 ->
+ * Compute_Jacobian_Approximate_Roe, Euler3D/src/Roe_Jacobian.cpp:29-228):
+ * first-order approximate Roe Jacobian with V/dt on the diagonal.
+ * rhs_mode 0: B = -Conv (Euler3D), 1: B = -(Conv+Diss). */
+int cfd_jacobian(cfd_mesh *mesh, const cfd_params *p, int rhs_mode, int mixed);
+/* Copies the BSR structure (ia[N+1], ja[nnz], iau[N]; any may be NULL) and
+ * returns nnz. */
+int32_t cfd_bsr_structure(cfd_mesh *mesh, int32_t *ia, int32_t *ja, int32_t *iau);
+int cfd_jacobian_get(cfd_mesh *mesh, double *a, double *b);
+int cfd_jacobian_set(cfd_mesh *mesh, const double *a, const double *b);
+/* Linear solve A X = B: FGMRES + ILU(0)/block-Jacobi, or the reference's
+ * MC_Iterative_Block_LU_Jacobi_CRS / MC_Iterative_Block_LUSGS_CRS
+ * (MATH/src/MC_Iterative_Jacobi.c:346-717). X stays on the device
+ * (CFD_FIELD_X). */
+int cfd_linsolve(cfd_mesh *mesh, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);
+/* Q += relaxation * X (Solver_Steady_Implicit.cpp:150-156). */
+int cfd_update(cfd_mesh *mesh, double relaxation);
+/* Pressure force on wall boundary edges: F = sum p_node S_out. */
+int cfd_forces(cfd_mesh *mesh, const cfd_params *p, int wall_tag, double alpha_deg, double *force3, double *lift,
+               double *drag, double *pint);
+/* Solver_Steady_Implicit (Euler3D/src/Solver_Steady_Implicit.cpp:33-168)
+ * with B = -(Conv+Diss): BC, gradient, limiter, residual, time step, RMS,
+ * Jacobian, linear solve, update, until RMS < tol.  rms_hist (host,
+ * max_iter entries) may be NULL. */
+int cfd_solve_steady(cfd_mesh *mesh, const cfd_params *p, const cfd_solve_opts *o, cfd_solve_result *res,
+                     double *rms_hist);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFD_HIP_H */
