@@ -175,23 +175,61 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
             delete m;
             return cfd_set_error(ctx, CFD_E_HIP, "hipStreamCreate failed");
         }
-        // node incidence lists in edge order (interior edges, then boundary)
+        // Internal numbering of the node data: Morton order of the coordinates,
+        // so the nodes a kernel has in flight on one XCD, and their neighbours,
+        // occupy a compact range of memory that stays in that XCD's L2.
+        // order[ii] = reference (API) index of internal node ii; ghosts keep
+        // their index N + b.  The BSR system, its solvers and the [N][5]
+        // vectors stay in the reference ordering.
+        std::vector<int> order(N), iperm(N);
+        {
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int i = 0; i < N; i++)
+                for (int k = 0; k < 3; k++) {
+                    lo[k] = std::min(lo[k], d->xyz[3 * (size_t)i + k]);
+                    hi[k] = std::max(hi[k], d->xyz[3 * (size_t)i + k]);
+                }
+            std::vector<std::pair<uint64_t, int>> key(N);
+            for (int i = 0; i < N; i++) {
+                uint64_t c = 0;
+                uint32_t q3[3];
+                for (int k = 0; k < 3; k++) {
+                    const double span = hi[k] - lo[k];
+                    const double f = span > 0 ? (d->xyz[3 * (size_t)i + k] - lo[k]) / span : 0.0;
+                    q3[k] = (uint32_t)std::min(2097151.0, std::max(0.0, f * 2097151.0));
+                }
+                for (int bit = 20; bit >= 0; bit--)
+                    for (int k = 0; k < 3; k++) c = (c << 1) | ((q3[k] >> bit) & 1u);
+                key[i] = {c, i};
+            }
+            std::sort(key.begin(), key.end());
+            for (int t = 0; t < N; t++) {
+                order[t] = key[t].second;
+                iperm[key[t].second] = t;
+            }
+        }
+        m->h_order = order;
+        // node incidence lists (internal numbering) in edge order: interior
+        // edges by index, then boundary edges -- the order in which the
+        // reference's sequential loops deliver each node's contributions
         std::vector<int> ip(N + 1, 0);
         for (int e = 0; e < E; e++) {
-            ip[d->edge[2 * e] + 1]++;
-            ip[d->edge[2 * e + 1] + 1]++;
+            ip[iperm[d->edge[2 * e]] + 1]++;
+            ip[iperm[d->edge[2 * e + 1]] + 1]++;
         }
-        for (int b = 0; b < B; b++) ip[d->bedge[2 * b] + 1]++;
+        for (int b = 0; b < B; b++) ip[iperm[d->bedge[2 * b]] + 1]++;
         for (int i = 0; i < N; i++) ip[i + 1] += ip[i];
         std::vector<int2> inc(ip[N]);
         {
             std::vector<int> pos(ip.begin(), ip.end() - 1);
             for (int e = 0; e < E; e++) {
-                const int l = d->edge[2 * e], r = d->edge[2 * e + 1];
+                // .y = e for the L end of the edge, ~e for the R end (roles follow
+                // the reference numbering, L < R, not the internal one)
+                const int l = iperm[d->edge[2 * e]], r = iperm[d->edge[2 * e + 1]];
                 inc[pos[l]++] = make_int2(r, e);
-                inc[pos[r]++] = make_int2(l, e);
+                inc[pos[r]++] = make_int2(l, ~e);
             }
-            for (int b = 0; b < B; b++) inc[pos[d->bedge[2 * b]]++] = make_int2(N + b, E + b);
+            for (int b = 0; b < B; b++) inc[pos[iperm[d->bedge[2 * b]]]++] = make_int2(N + b, E + b);
         }
         // BSR pattern, Create_CRS_SolverBlockMatrix (Euler3D/src/Solver.cpp:612-675):
         // row i holds i and its neighbours in ascending column order.
@@ -208,11 +246,13 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
             for (int q = m->h_ia[i]; q < m->h_ia[i + 1]; q++)
                 if (m->h_ja[q] == i) m->h_iau[i] = q;
         }
+        // BSR block of the off-diagonal (row of the node, column of the neighbour)
         std::vector<int> off(inc.size(), -1);
-        for (int i = 0; i < N; i++)
-            for (int k = ip[i]; k < ip[i + 1]; k++) {
-                const int j = inc[k].x;
-                if (j >= N) continue;
+        for (int ii = 0; ii < N; ii++) {
+            const int i = order[ii];
+            for (int k = ip[ii]; k < ip[ii + 1]; k++) {
+                if (inc[k].x >= N) continue;
+                const int j = order[inc[k].x];
                 auto b0 = m->h_ja.begin() + m->h_ia[i], b1 = m->h_ja.begin() + m->h_ia[i + 1];
                 auto it = std::lower_bound(b0, b1, j);
                 if (it == b1 || *it != j) {
@@ -221,20 +261,31 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
                 }
                 off[k] = (int)(it - m->h_ja.begin());
             }
+        }
         std::vector<int> lrows, urows;
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, false, m->lower.ptr, lrows, m->lower.maxwidth);
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, true, m->upper.ptr, urows, m->upper.maxwidth);
         m->lower.nlevels = (int)m->lower.ptr.size() - 1;
         m->upper.nlevels = (int)m->upper.ptr.size() - 1;
+        // neighbour lists (reference order within each list), internal numbering
+        std::vector<int> nia(N + 1, 0), nja((size_t)d->n2n_ia[N]);
+        for (int ii = 0; ii < N; ii++) {
+            const int i = order[ii];
+            nia[ii + 1] = nia[ii] + (d->n2n_ia[i + 1] - d->n2n_ia[i]);
+            int k = nia[ii];
+            for (int q = d->n2n_ia[i]; q < d->n2n_ia[i + 1]; q++) nja[k++] = iperm[d->n2n_ja[q]];
+        }
 
         // packed node records: q (freestream), coordinates
-        std::vector<double> qa((size_t)kQA * m->NT, 0.0), sa((size_t)4 * (E + B), 0.0);
+        std::vector<double> qa((size_t)kQA * m->NT, 0.0), sa((size_t)4 * (E + B), 0.0), vol(N);
         double q5[5];
         freestream_q(p, q5);
         for (int i = 0; i < m->NT; i++) {
             for (int k = 0; k < 5; k++) qa[(size_t)kQA * i + k] = q5[k];
-            if (i < N)
-                for (int k = 0; k < 3; k++) qa[(size_t)kQA * i + 5 + k] = d->xyz[3 * (size_t)i + k];
+            if (i < N) {
+                for (int k = 0; k < 3; k++) qa[(size_t)kQA * i + 5 + k] = d->xyz[3 * (size_t)order[i] + k];
+                vol[i] = d->vol[order[i]];
+            }
         }
         for (int e = 0; e < E; e++)
             for (int k = 0; k < 3; k++) sa[4 * (size_t)e + k] = d->edge_area[3 * (size_t)e + k];
@@ -242,21 +293,22 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
             for (int k = 0; k < 3; k++) sa[4 * ((size_t)E + b) + k] = d->bedge_area[3 * (size_t)b + k];
         std::vector<int> bnode(B), btag(B, 0);
         for (int b = 0; b < B; b++) {
-            bnode[b] = d->bedge[2 * b];
+            bnode[b] = iperm[d->bedge[2 * b]];
             if (d->bedge_tag) btag[b] = d->bedge_tag[b];
         }
 
-        m->inc_ptr = upload(m, ip.data(), ip.size(), &rc);
+        m->order = upload(m, order.data(), order.size(), &rc);
+        if (!rc) m->inc_ptr = upload(m, ip.data(), ip.size(), &rc);
         if (!rc) m->inc = upload(m, inc.data(), inc.size(), &rc);
         if (!rc) m->inc_off = upload(m, off.data(), off.size(), &rc);
-        if (!rc) m->n2n_ia = upload(m, d->n2n_ia, (size_t)N + 1, &rc);
-        if (!rc) m->n2n_ja = upload(m, d->n2n_ja, (size_t)d->n2n_ia[N], &rc);
+        if (!rc) m->n2n_ia = upload(m, nia.data(), nia.size(), &rc);
+        if (!rc) m->n2n_ja = upload(m, nja.data(), nja.size(), &rc);
         if (!rc) m->bnode = upload(m, bnode.data(), bnode.size(), &rc);
         if (!rc) m->btype = upload(m, d->bedge_type, (size_t)B, &rc);
         if (!rc) m->btag = upload(m, btag.data(), btag.size(), &rc);
         if (!rc) m->d.qa = upload(m, qa.data(), qa.size(), &rc);
         if (!rc) m->d.sa = upload(m, sa.data(), sa.size(), &rc);
-        if (!rc) m->d.vol = upload(m, d->vol, (size_t)N, &rc);
+        if (!rc) m->d.vol = upload(m, vol.data(), vol.size(), &rc);
         if (!rc) m->ia = upload(m, m->h_ia.data(), m->h_ia.size(), &rc);
         if (!rc) m->ja = upload(m, m->h_ja.data(), m->h_ja.size(), &rc);
         if (!rc) m->iau = upload(m, m->h_iau.data(), m->h_iau.size(), &rc);
@@ -268,18 +320,17 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
         }
         m->d.qb = (double *)cfd_dalloc(m, sizeof(double) * kQB * (size_t)N);
         m->d.lsq = (double *)cfd_dalloc(m, sizeof(double) * 8 * (size_t)N);
-        m->d.rc = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
-        m->d.rd = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
-        m->d.dta = (double *)cfd_dalloc(m, sizeof(double) * (size_t)N);
+        m->d.res = (double *)cfd_dalloc(m, sizeof(double) * kRES * (size_t)N);
         m->dt = (double *)cfd_dalloc(m, sizeof(double) * (size_t)N);
         m->b = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
         m->x = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
         m->mark = (int *)cfd_dalloc(m, sizeof(int) * (size_t)N);
+        m->risk = (int *)cfd_dalloc(m, sizeof(int));
         m->red = (double *)cfd_dalloc(m, sizeof(double) * 1024 * 65);
         m->redout = (double *)cfd_dalloc(m, sizeof(double) * 4 * 65);
         m->flag = (int *)cfd_dalloc(m, sizeof(int) * 4);
         m->eigbits = (unsigned long long *)cfd_dalloc(m, sizeof(unsigned long long) * 6);
-        if (!m->d.qb || !m->d.lsq || !m->d.rc || !m->d.rd || !m->d.dta || !m->dt || !m->b || !m->x || !m->mark ||
+        if (!m->d.qb || !m->d.lsq || !m->d.res || !m->dt || !m->b || !m->x || !m->mark || !m->risk ||
             !m->red || !m->redout || !m->flag || !m->eigbits) {
             cfd_mesh_destroy(m);
             return cfd_set_error(ctx, CFD_E_OOM, "cfd_mesh_create: out of device memory");
@@ -292,8 +343,7 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
         hipMemsetAsync(m->d.qb, 0, sizeof(double) * kQB * (size_t)N, m->stream);
         hipMemsetAsync(m->mark, 0, sizeof(int) * (size_t)N, m->stream);
         hipMemsetAsync(m->flag, 0, sizeof(int) * 4, m->stream);
-        hipMemsetAsync(m->d.rc, 0, sizeof(double) * 5 * (size_t)N, m->stream);
-        hipMemsetAsync(m->d.rd, 0, sizeof(double) * 5 * (size_t)N, m->stream);
+        hipMemsetAsync(m->d.res, 0, sizeof(double) * kRES * (size_t)N, m->stream);
         rc = launch_lsq_coef(m);
         if (!rc && hipStreamSynchronize(m->stream) != hipSuccess) rc = cfd_set_error(ctx, CFD_E_HIP, "LSQ setup failed");
         if (rc) {
@@ -338,10 +388,8 @@ int cfd_ensure_fp32(cfd_mesh *m) {
         m->f.sa = (float *)cfd_dalloc(m, sizeof(float) * 4 * EB);
         m->f.vol = (float *)cfd_dalloc(m, sizeof(float) * N);
         m->f.lsq = (float *)cfd_dalloc(m, sizeof(float) * 8 * N);
-        m->f.rc = (float *)cfd_dalloc(m, sizeof(float) * 5 * N);
-        m->f.rd = (float *)cfd_dalloc(m, sizeof(float) * 5 * N);
-        m->f.dta = (float *)cfd_dalloc(m, sizeof(float) * N);
-        if (!m->f.qa || !m->f.qb || !m->f.sa || !m->f.vol || !m->f.lsq || !m->f.rc || !m->f.rd || !m->f.dta)
+        m->f.res = (float *)cfd_dalloc(m, sizeof(float) * kRES * N);
+        if (!m->f.qa || !m->f.qb || !m->f.sa || !m->f.vol || !m->f.lsq || !m->f.res)
             return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (FP32 buffers)");
         int rc = launch_convert_geometry_fp32(m);
         if (rc) return rc;
@@ -359,6 +407,9 @@ int cfd_ensure_fp32(cfd_mesh *m) {
 // ---------------------------------------------------------------------------
 // fields
 // ---------------------------------------------------------------------------
+// reference (API) index of internal node record i (ghosts are not renumbered)
+static inline size_t api_index(const cfd_mesh *m, size_t i) { return i < (size_t)m->N ? (size_t)m->h_order[i] : i; }
+
 extern "C" int cfd_field_set(cfd_mesh *m, int field, const double *h) {
     if (!m || !h) return CFD_E_ARG;
     hipSetDevice(m->ctx->device);
@@ -372,31 +423,49 @@ extern "C" int cfd_field_set(cfd_mesh *m, int field, const double *h) {
         CFD_HIP(m, hipMemcpy(buf.data(), dev, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
         const int nf = field == CFD_FIELD_GRAD ? 15 : 5, o = field == CFD_FIELD_PHI ? kPHI : 0;
         for (int f = 0; f < nf; f++)
-            for (size_t i = 0; i < n; i++) buf[rec * i + o + f] = h[f * n + i];
+            for (size_t i = 0; i < n; i++) buf[rec * i + o + f] = h[f * n + api_index(m, i)];
         CFD_HIP(m, hipMemcpy(dev, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
         m->q_version++;
+        return CFD_OK;
+    }
+    if (field == CFD_FIELD_RES_CONV || field == CFD_FIELD_RES_DISS || field == CFD_FIELD_DT_ACC) {
+        std::vector<double> buf((size_t)kRES * N);
+        CFD_HIP(m, hipMemcpy(buf.data(), m->d.res, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        const int nf = field == CFD_FIELD_DT_ACC ? 1 : 5;
+        const int o = field == CFD_FIELD_RES_CONV ? 0 : (field == CFD_FIELD_RES_DISS ? 5 : 10);
+        for (int f = 0; f < nf; f++)
+            for (size_t i = 0; i < N; i++) buf[kRES * i + o + f] = h[f * N + api_index(m, i)];
+        CFD_HIP(m, hipMemcpy(m->d.res, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+        m->res_prec = CFD_FP64;
+        return CFD_OK;
+    }
+    if (field == CFD_FIELD_DT) {
+        std::vector<double> buf(N);
+        for (size_t i = 0; i < N; i++) buf[i] = h[api_index(m, i)];
+        CFD_HIP(m, hipMemcpy(m->dt, buf.data(), sizeof(double) * N, hipMemcpyHostToDevice));
         return CFD_OK;
     }
     double *dev = nullptr;
     size_t n = 0;
     switch (field) {
-    case CFD_FIELD_RES_CONV: dev = m->d.rc; n = 5 * N; break;
-    case CFD_FIELD_RES_DISS: dev = m->d.rd; n = 5 * N; break;
-    case CFD_FIELD_DT_ACC: dev = m->d.dta; n = N; break;
-    case CFD_FIELD_DT: dev = m->dt; n = N; break;
     case CFD_FIELD_JAC_B: dev = m->b; n = 5 * N; break;
     case CFD_FIELD_X: dev = m->x; n = 5 * N; break;
     default: return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_field_set: unknown field %d", field);
     }
     CFD_HIP(m, hipMemcpy(dev, h, sizeof(double) * n, hipMemcpyHostToDevice));
-    if (field == CFD_FIELD_RES_CONV || field == CFD_FIELD_RES_DISS) m->res_prec = CFD_FP64;
     return CFD_OK;
 }
+
+static int check_residual_flags(cfd_mesh *m);
 
 extern "C" int cfd_field_get(cfd_mesh *m, int field, double *h) {
     if (!m || !h) return CFD_E_ARG;
     hipSetDevice(m->ctx->device);
     const size_t N = m->N, NT = m->NT;
+    if (field == CFD_FIELD_RES_CONV || field == CFD_FIELD_RES_DISS || field == CFD_FIELD_DT_ACC) {
+        int rc = check_residual_flags(m);
+        if (rc) return rc;
+    }
     if ((field == CFD_FIELD_RES_CONV || field == CFD_FIELD_RES_DISS || field == CFD_FIELD_DT_ACC) &&
         m->res_prec == CFD_FP32) {
         int rc = launch_residual_to_fp64(m);
@@ -414,16 +483,27 @@ extern "C" int cfd_field_get(cfd_mesh *m, int field, double *h) {
         CFD_HIP(m, hipMemcpy(buf.data(), isq ? m->d.qa : m->d.qb, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
         const int nf = field == CFD_FIELD_GRAD ? 15 : 5, o = field == CFD_FIELD_PHI ? kPHI : 0;
         for (int f = 0; f < nf; f++)
-            for (size_t i = 0; i < n; i++) h[f * n + i] = buf[rec * i + o + f];
+            for (size_t i = 0; i < n; i++) h[f * n + api_index(m, i)] = buf[rec * i + o + f];
+        return CFD_OK;
+    }
+    if (field == CFD_FIELD_RES_CONV || field == CFD_FIELD_RES_DISS || field == CFD_FIELD_DT_ACC) {
+        std::vector<double> buf((size_t)kRES * N);
+        CFD_HIP(m, hipMemcpy(buf.data(), m->d.res, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        const int nf = field == CFD_FIELD_DT_ACC ? 1 : 5;
+        const int o = field == CFD_FIELD_RES_CONV ? 0 : (field == CFD_FIELD_RES_DISS ? 5 : 10);
+        for (int f = 0; f < nf; f++)
+            for (size_t i = 0; i < N; i++) h[f * N + api_index(m, i)] = buf[kRES * i + o + f];
+        return CFD_OK;
+    }
+    if (field == CFD_FIELD_DT) {
+        std::vector<double> buf(N);
+        CFD_HIP(m, hipMemcpy(buf.data(), m->dt, sizeof(double) * N, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) h[api_index(m, i)] = buf[i];
         return CFD_OK;
     }
     const double *dev = nullptr;
     size_t n = 0;
     switch (field) {
-    case CFD_FIELD_RES_CONV: dev = m->d.rc; n = 5 * N; break;
-    case CFD_FIELD_RES_DISS: dev = m->d.rd; n = 5 * N; break;
-    case CFD_FIELD_DT_ACC: dev = m->d.dta; n = N; break;
-    case CFD_FIELD_DT: dev = m->dt; n = N; break;
     case CFD_FIELD_JAC_B: dev = m->b; n = 5 * N; break;
     case CFD_FIELD_X: dev = m->x; n = 5 * N; break;
     default: return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_field_get: unknown field %d", field);
@@ -435,6 +515,8 @@ extern "C" int cfd_field_get(cfd_mesh *m, int field, double *h) {
 // ---------------------------------------------------------------------------
 // hot-path entry points
 // ---------------------------------------------------------------------------
+static int check_residual_flags(cfd_mesh *m);
+
 static int check_params(cfd_mesh *m, const cfd_params *p) {
     if (!m || !p) return CFD_E_ARG;
     if (p->flux < 0 || p->flux > 2 || (p->order != 1 && p->order != 2) || p->limiter < 0 || p->limiter > 2 ||
@@ -520,17 +602,24 @@ extern "C" int cfd_residual(cfd_mesh *m, const cfd_params *p, int add_time, doub
     }
     if (rc) return rc;
     m->residual_valid = true;
+    m->hllc_check = p->flux == CFD_FLUX_HLLC;
     if (stats) {
         unsigned long long bits[6];
         CFD_HIP(m, hipMemcpyAsync(bits, m->eigbits, sizeof(bits), hipMemcpyDeviceToHost, m->stream));
         CFD_HIP(m, hipStreamSynchronize(m->stream));
         for (int t = 0; t < 6; t++) memcpy(&eig6[t], &bits[t], 8);
     }
-    if (p->flux == CFD_FLUX_HLLC) {
-        rc = read_flags(m);
-        if (rc) return rc;
-        if (m->hflag[1]) return cfd_set_error(m->ctx, CFD_E_HLLC, "HLLC: no wave configuration on %d nodes", m->hflag[1]);
-    }
+    return CFD_OK;
+}
+
+// Reports an HLLC wave-configuration failure of the last residual (read back
+// lazily so the residual launch itself never waits on the host).
+static int check_residual_flags(cfd_mesh *m) {
+    if (!m->hllc_check) return CFD_OK;
+    m->hllc_check = false;
+    int rc = read_flags(m);
+    if (rc) return rc;
+    if (m->hflag[1]) return cfd_set_error(m->ctx, CFD_E_HLLC, "HLLC: no wave configuration on %d nodes", m->hflag[1]);
     return CFD_OK;
 }
 
@@ -566,7 +655,9 @@ extern "C" int cfd_residual_rms(cfd_mesh *m, double *rms) {
         m->res_prec = CFD_FP64;
     }
     double s[5];
-    int rc = launch_rms(m, s);
+    int rc = check_residual_flags(m);
+    if (rc) return rc;
+    rc = launch_rms(m, s);
     if (rc) return rc;
     double t = s[0] + s[1] + s[2] + s[3] + s[4];
     *rms = sqrt(t / (5.0 * (double)m->N));
@@ -588,6 +679,8 @@ extern "C" int cfd_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int 
     int rc = check_params(m, p);
     if (rc) return rc;
     if (!m->residual_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "cfd_jacobian: call cfd_residual first");
+    rc = check_residual_flags(m);
+    if (rc) return rc;
     if (m->res_prec != CFD_FP64) {
         rc = launch_residual_to_fp64(m);
         if (rc) return rc;

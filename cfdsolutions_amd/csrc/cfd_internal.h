@@ -20,7 +20,9 @@ struct cfd_ctx {
 //   qb [N][24]  = grad (Q1x,Q1y,Q1z,...,Q5z), phi1..phi5, 4 pad
 //   sa [E+B][4] = area vector x, y, z, pad (interior edges, then boundary)
 //   lsq [N][8]  = r11 r12 r13 r22 r23 ro22 ro33 pad
-// Outputs keep the reference layout: rc, rd [5][N], dta [N].
+//   res [N][12] = convective residual (5), dissipative residual (5), time-step
+//                 accumulator, pad; one contiguous record per node because
+//                 nodes are processed in a locality order, not storage order.
 template <class T>
 struct ResidualBufs {
     T *qa = nullptr;
@@ -28,12 +30,12 @@ struct ResidualBufs {
     T *sa = nullptr;
     T *vol = nullptr;
     T *lsq = nullptr;
-    T *rc = nullptr, *rd = nullptr;
-    T *dta = nullptr;
+    T *res = nullptr;
 };
 constexpr int kQA = 8;
 constexpr int kQB = 24;
 constexpr int kPHI = 15;  // offset of phi in a qb record
+constexpr int kRES = 12;  // residual record: conv 0-4, diss 5-9, dt accumulator 10
 
 struct LevelSchedule {
     std::vector<int> ptr;   // host: level l holds rows[ptr[l]..ptr[l+1])
@@ -50,12 +52,16 @@ struct cfd_mesh {
     std::vector<void *> allocs;
 
     // topology
+    int *order = nullptr;     // [N] reference index of each internal node (internal = Morton order)
+    std::vector<int> h_order;
     int *inc_ptr = nullptr;   // [N+1]
-    int2 *inc = nullptr;      // [2E+B] {neighbour (ghost >= N), area index}, in edge order
+    int2 *inc = nullptr;      // [2E+B] {neighbour (ghost >= N), area index e if the node is the edge's L
+                              //  end, ~e if it is the R end}, in edge order
     int *inc_off = nullptr;   // [2E+B] BSR block of the off-diagonal (i, nbr), -1 for ghosts
     int *n2n_ia = nullptr, *n2n_ja = nullptr;
     int *bnode = nullptr, *btype = nullptr, *btag = nullptr;  // [B]
     int *mark = nullptr;      // [N] limiter correction marks
+    int *risk = nullptr;      // [1] nodes with a near-vacuum face state
     int mark_epoch = 0;
 
     ResidualBufs<double> d;   // FP64
@@ -93,6 +99,7 @@ struct cfd_mesh {
     unsigned long long *eigbits = nullptr;  // [6]
     bool residual_valid = false;
     int res_prec = CFD_FP64;   // precision of the last residual
+    bool hllc_check = false;   // HLLC anomaly flag not yet read back
     int64_t q_version = 0, f_version = -1;  // FP32 mirror of the state is current when equal
 };
 

@@ -119,10 +119,11 @@ __device__ __forceinline__ void roe_ajac(double g, int efix, const double *QL, c
 template <class TA>
 struct JacArgs {
     int N, NT;
+    const int *order;
     const int *ip;
     const int2 *inc;
     const int *off, *iau;
-    const double *sa, *qa, *vol, *dt, *rc, *rd;
+    const double *sa, *qa, *vol, *dt, *res;
     TA *a;
     double *b;
     double g;
@@ -134,8 +135,9 @@ struct JacArgs {
 // order) and the off-diagonal block of every neighbour.
 template <class TA>
 __global__ __launch_bounds__(128) void k_jacobian(JacArgs<TA> a) {
-    const int i = xcd_block() * 128 + threadIdx.x;
+    const int i = xcd_block() * 128 + threadIdx.x;  // internal node
     if (i >= a.N) return;
+    const int ai = a.order[i];                      // its row of the BSR system
     double qi[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) qi[k] = a.qa[kQA * (size_t)i + k];
@@ -148,11 +150,11 @@ __global__ __launch_bounds__(128) void k_jacobian(JacArgs<TA> a) {
     for (int k = a.ip[i]; k < a.ip[i + 1]; k++) {
         const int2 en = a.inc[k];
         const int j = en.x;
-        const double *s = a.sa + 4 * (size_t)en.y;
+        const bool iL = en.y >= 0;
+        const double *s = a.sa + 4 * (size_t)(iL ? en.y : ~en.y);
         double nx, ny, nz;
         const double area = area_normal(s[0], s[1], s[2], nx, ny, nz);
         const double *rj = a.qa + kQA * (size_t)j;
-        const bool iL = j > i;
         double QL[5], QR[5];
 #pragma unroll
         for (int t = 0; t < 5; t++) {
@@ -184,14 +186,14 @@ __global__ __launch_bounds__(128) void k_jacobian(JacArgs<TA> a) {
             for (int t = 0; t < 25; t++) O[t] = (TA)(0.0 - 0.5 * (J[t] + AR[t]) * area);
         }
     }
-    TA *Dd = a.a + 25 * (size_t)a.iau[i];
+    TA *Dd = a.a + 25 * (size_t)a.iau[ai];
 #pragma unroll
     for (int t = 0; t < 25; t++) Dd[t] = (TA)D[t];
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-        double r = a.rc[(size_t)k * a.N + i];
-        if (a.rhs_mode == 1) r = r + a.rd[(size_t)k * a.N + i];
-        a.b[5 * (size_t)i + k] = -r;
+        double r = a.res[kRES * (size_t)i + k];
+        if (a.rhs_mode == 1) r = r + a.res[kRES * (size_t)i + 5 + k];
+        a.b[5 * (size_t)ai + k] = -r;
     }
 }
 
@@ -611,12 +613,12 @@ int grid_cap(size_t n, int cap) {
 int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed) {
     const int nb = xcd_grid((m->N + 127) / 128);
     if (mixed) {
-        JacArgs<float> a{m->N, m->NT, m->inc_ptr, m->inc, m->inc_off, m->iau, m->d.sa, m->d.qa, m->d.vol, m->dt,
-                         m->d.rc, m->d.rd, m->af, m->b, p->gamma, p->entropy_fix, rhs_mode};
+        JacArgs<float> a{m->N, m->NT, m->order, m->inc_ptr, m->inc, m->inc_off, m->iau, m->d.sa, m->d.qa, m->d.vol,
+                         m->dt, m->d.res, m->af, m->b, p->gamma, p->entropy_fix, rhs_mode};
         hipLaunchKernelGGL(k_jacobian<float>, dim3(nb), dim3(128), 0, m->stream, a);
     } else {
-        JacArgs<double> a{m->N, m->NT, m->inc_ptr, m->inc, m->inc_off, m->iau, m->d.sa, m->d.qa, m->d.vol, m->dt,
-                          m->d.rc, m->d.rd, m->a, m->b, p->gamma, p->entropy_fix, rhs_mode};
+        JacArgs<double> a{m->N, m->NT, m->order, m->inc_ptr, m->inc, m->inc_off, m->iau, m->d.sa, m->d.qa, m->d.vol,
+                          m->dt, m->d.res, m->a, m->b, p->gamma, p->entropy_fix, rhs_mode};
         hipLaunchKernelGGL(k_jacobian<double>, dim3(nb), dim3(128), 0, m->stream, a);
     }
     CFD_HIP(m, hipGetLastError());

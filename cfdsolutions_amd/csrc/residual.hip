@@ -189,9 +189,9 @@ __global__ __launch_bounds__(kBlock) void k_bc(BcArgs<T> a) {
 // ---------------------------------------------------------------------------
 // Compute_Least_Square_Gradient, Euler3D/src/Gradient.cpp:227-295
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_lsq_grad(int N, const int *__restrict__ ia, const int *__restrict__ ja,
-                                                     const T *__restrict__ lsq, const T *__restrict__ qa,
-                                                     T *__restrict__ qb) {
+__global__ __launch_bounds__(kBlock) void k_lsq_grad(int N, const int *__restrict__ order, const int *__restrict__ ia,
+                                                     const int *__restrict__ ja, const T *__restrict__ lsq,
+                                                     const T *__restrict__ qa, T *__restrict__ qb) {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= N) return;
     const T *ra = qa + kQA * (size_t)i;
@@ -228,9 +228,10 @@ __global__ __launch_bounds__(kBlock) void k_lsq_grad(int N, const int *__restric
 //   grad_i = (1/V_i) [ sum_edges +-1/2 (Q_L+Q_R) S_LR + sum_bnd Q_i S_b ]
 // accumulated in edge order (the reference has no 3D Green-Gauss).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict__ ip, const int2 *__restrict__ inc,
-                                                    const T *__restrict__ sa, const T *__restrict__ vol,
-                                                    const T *__restrict__ qa, T *__restrict__ qb) {
+__global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict__ order, const int *__restrict__ ip,
+                                                    const int2 *__restrict__ inc, const T *__restrict__ sa,
+                                                    const T *__restrict__ vol, const T *__restrict__ qa,
+                                                    T *__restrict__ qb) {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= N) return;
     const T *ra = qa + kQA * (size_t)i;
@@ -242,7 +243,8 @@ __global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict
     for (int f = 0; f < 15; f++) gs[f] = T(0.0);
     for (int k = ip[i]; k < ip[i + 1]; k++) {
         const int2 en = inc[k];
-        const T *s = sa + 4 * (size_t)en.y;
+        const bool iL = en.y >= 0;
+        const T *s = sa + 4 * (size_t)(iL ? en.y : ~en.y);
         const T s0 = s[0], s1 = s[1], s2 = s[2];
         if (en.x >= N) {
 #pragma unroll
@@ -253,7 +255,6 @@ __global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict
             }
         } else {
             const T *rj = qa + kQA * (size_t)en.x;
-            const bool iL = en.x > i;
 #pragma unroll
             for (int f = 0; f < 5; f++) {
                 const T qf = T(0.5) * (qi[f] + rj[f]);
@@ -278,10 +279,28 @@ __global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict
 // ---------------------------------------------------------------------------
 // Limiters, Euler3D/src/Limiters.cpp:20-304
 // ---------------------------------------------------------------------------
+// Relative pressure margin below which a face state counts as near-vacuum.
+// When every face state of both ends of an edge has rho > 0 and p above this
+// margin, the Roe-averaged pressure of the edge is positive: h - q^2/2 of the
+// Roe average is at least the sqrt(rho)-weighted mean of c_k^2/(g-1) > 0, a
+// margin orders of magnitude above the rounding of its evaluation.
+template <class T>
+__device__ __forceinline__ T vacuum_margin();
+template <>
+__device__ __forceinline__ double vacuum_margin<double>() { return 1.0e-6; }
+template <>
+__device__ __forceinline__ float vacuum_margin<float>() { return 1.0e-3f; }
+
+template <class T>
+__device__ __forceinline__ bool near_vacuum(const T *Q, T P) {
+    return !(Q[0] > T(0.0)) || !(P > vacuum_margin<T>() * fabs(Q[4]));
+}
+
 template <class T, int KIND>  // 1 Venkatakrishnan, 2 Barth-Jespersen
-__global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict__ ia, const int *__restrict__ ja,
-                                                    const T *__restrict__ qa, T *__restrict__ qb,
-                                                    const T *__restrict__ vol, T g, T frac, T k3) {
+__global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict__ order, const int *__restrict__ ia,
+                                                    const int *__restrict__ ja, const T *__restrict__ qa,
+                                                    T *__restrict__ qb, const T *__restrict__ vol, T g, T frac, T k3,
+                                                    int *risk) {
     const int c = xcd_block() * kBlock + threadIdx.x;
     if (c >= N) return;
     const T *ra = qa + kQA * (size_t)c;
@@ -341,7 +360,10 @@ __global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict
             ph[k] = tmax(T(0.0), ph[k]);
         }
     }
-    // Compute_Limiter_PressureCorrection, Limiters.cpp:273-304
+    // Compute_Limiter_PressureCorrection, Limiters.cpp:273-304.  Also notes
+    // whether a final face state is near vacuum (the only case in which the
+    // Roe pressure correction can act).
+    bool zeroed = false, risky = false;
     for (int n = b; n < e; n++) {
         const T *rj = qa + kQA * (size_t)ja[n];
         const T Rx = T(0.5) * (rj[5] - xc), Ry = T(0.5) * (rj[6] - yc), Rz = T(0.5) * (rj[7] - zc);
@@ -353,8 +375,16 @@ __global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict
         if (P < T(0.0)) {
 #pragma unroll
             for (int k = 0; k < 5; k++) ph[k] = T(0.0);
+            zeroed = true;
+        } else if (near_vacuum(Q, P)) {
+            risky = true;
         }
     }
+    if (zeroed) {  // every face state is now the node state
+        const T P = (g - T(1.0)) * (qc[4] - (T(0.5) / qc[0]) * (qc[1] * qc[1] + qc[2] * qc[2] + qc[3] * qc[3]));
+        risky = near_vacuum(qc, P);
+    }
+    if (risky) atomicAdd(risk, 1);
 #pragma unroll
     for (int k = 0; k < 5; k++) rb[kPHI + k] = ph[k];
 }
@@ -374,9 +404,11 @@ __device__ __forceinline__ void face_state(const T *ra, const T *rbg, const T *p
 // tested with the limiter values of the previous kernel and the marked nodes
 // are zeroed afterwards (identical whenever no such edge exists).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_roe_mark(int N, const int *__restrict__ ia, const int *__restrict__ ja,
-                                                     const T *__restrict__ qa, const T *__restrict__ qb, T g, T frac,
-                                                     int *mark, int epoch) {
+__global__ __launch_bounds__(kBlock) void k_roe_mark(int N, const int *__restrict__ order, const int *__restrict__ ia,
+                                                     const int *__restrict__ ja, const T *__restrict__ qa,
+                                                     const T *__restrict__ qb, T g, T frac, int *mark, int epoch,
+                                                     const int *risk) {
+    if (*risk == 0) return;  // no near-vacuum face state: no edge can qualify
     const int c = xcd_block() * kBlock + threadIdx.x;
     if (c >= N) return;
     const T *ra = qa + kQA * (size_t)c;
@@ -422,10 +454,11 @@ __global__ __launch_bounds__(kBlock) void k_roe_zero(int N, const int *__restric
 template <class T>
 struct ResArgs {
     int N, NT;
+    const int *order;
     const int *ip;
     const int2 *inc;
     const T *sa, *qa, *qb;
-    T *rc, *rd, *dta;
+    T *res;
     T g, frac;
     int efix;
     unsigned long long *eig;
@@ -460,10 +493,10 @@ __global__ __launch_bounds__(kBlock) void k_residual(ResArgs<T> a) {
     for (int k = kb; k < ke; k++) {
         const int2 en = a.inc[k];
         const int j = en.x;
-        const T *s = a.sa + 4 * (size_t)en.y;
+        const bool iL = en.y >= 0;  // ghosts (j >= N) always pair with a physical L
+        const T *s = a.sa + 4 * (size_t)(iL ? en.y : ~en.y);
         const T s0 = s[0], s1 = s[1], s2 = s[2];
         const T *rj = a.qa + kQA * (size_t)j;
-        const bool iL = j > i;  // ghosts (j >= N) always pair with a physical L
         T QL[5], QR[5];
         if (ORDER == 2 && j < a.N) {
             // Compute_SecondOrderReconstructQ, Euler3D/src/HigherOrderReconstructQ.cpp:20-71
@@ -525,12 +558,14 @@ __global__ __launch_bounds__(kBlock) void k_residual(ResArgs<T> a) {
         }
         dt += lam;
     }
+    T *o = a.res + kRES * (size_t)i;
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-        a.rc[(size_t)k * a.N + i] = ac[k];
-        a.rd[(size_t)k * a.N + i] = ad[k];
+        o[k] = ac[k];
+        o[5 + k] = ad[k];
     }
-    a.dta[i] = dt;
+    o[10] = dt;
+    o[11] = T(0.0);
     if (bad) flag_set(a.flag, 1);
     if (STATS && kb < ke) {
         for (int t = 0; t < 3; t++) {
@@ -543,10 +578,10 @@ __global__ __launch_bounds__(kBlock) void k_residual(ResArgs<T> a) {
 // ---------------------------------------------------------------------------
 // Local time step, RMS, update, forces
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_delta_t(int N, const double *vol, const double *dta, double cfl,
+__global__ __launch_bounds__(kBlock) void k_delta_t(int N, const double *vol, const double *res, double cfl,
                                                     double *dt) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < N) dt[i] = vol[i] * cfl / dta[i];
+    if (i < N) dt[i] = vol[i] * cfl / res[kRES * (size_t)i + 10];
 }
 
 // Deterministic two-stage reductions: fixed per-thread strides, fixed tree.
@@ -569,12 +604,12 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_rms_partial(int N, const double *rc, const double *rd, double *part) {
+__global__ __launch_bounds__(kBlock) void k_rms_partial(int N, const double *res, double *part) {
     double v[5] = {0, 0, 0, 0, 0};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            const double r = rc[(size_t)k * N + i] + rd[(size_t)k * N + i];
+            const double r = res[kRES * (size_t)i + k] + res[kRES * (size_t)i + 5 + k];
             v[k] += r * r;
         }
     }
@@ -592,11 +627,12 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(int nparts, const doubl
     block_reduce_store<NV>(v, out);
 }
 
-__global__ __launch_bounds__(kBlock) void k_update(int N, double relax, const double *x, double *qa) {
+__global__ __launch_bounds__(kBlock) void k_update(int N, double relax, const int *order, const double *x, double *qa) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= N) return;
+    const size_t r = order[i];  // X is in the reference ordering
 #pragma unroll
-    for (int k = 0; k < 5; k++) qa[kQA * (size_t)i + k] += relax * x[5 * (size_t)i + k];
+    for (int k = 0; k < 5; k++) qa[kQA * (size_t)i + k] += relax * x[5 * r + k];
 }
 
 __global__ __launch_bounds__(kBlock) void k_forces_partial(int N, int E, int B, const int *bnode, const int *btype,
@@ -655,11 +691,11 @@ template <class T>
 int launch_gradient(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p) {
     const int nb = xcd_grid((m->N + kBlock - 1) / kBlock);
     if (p->gradient == CFD_GRAD_GG)
-        hipLaunchKernelGGL(k_gg_grad<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->inc, b.sa, b.vol,
-                           b.qa, b.qb);
+        hipLaunchKernelGGL(k_gg_grad<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->inc_ptr, m->inc,
+                           b.sa, b.vol, b.qa, b.qb);
     else
-        hipLaunchKernelGGL(k_lsq_grad<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, b.lsq, b.qa,
-                           b.qb);
+        hipLaunchKernelGGL(k_lsq_grad<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->n2n_ia, m->n2n_ja,
+                           b.lsq, b.qa, b.qb);
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;
 }
@@ -670,19 +706,19 @@ int launch_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p) {
     const T frac = (p->limiter_order == 2) ? T(0.5) : T(0.0);
     const T k3 = (T)pow(p->venkat_k, 3.0);  // host pow, as the reference's glibc pow
     const T g = (T)p->gamma;
+    if (p->limiter != CFD_LIM_VENKAT && p->limiter != CFD_LIM_BJ) return CFD_OK;
+    CFD_HIP(m, hipMemsetAsync(m->risk, 0, sizeof(int), m->stream));
     if (p->limiter == CFD_LIM_VENKAT)
-        hipLaunchKernelGGL((k_limiter<T, 1>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, b.qa,
-                           b.qb, b.vol, g, frac, k3);
-    else if (p->limiter == CFD_LIM_BJ)
-        hipLaunchKernelGGL((k_limiter<T, 2>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, b.qa,
-                           b.qb, b.vol, g, frac, k3);
+        hipLaunchKernelGGL((k_limiter<T, 1>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->n2n_ia,
+                           m->n2n_ja, b.qa, b.qb, b.vol, g, frac, k3, m->risk);
     else
-        return CFD_OK;
+        hipLaunchKernelGGL((k_limiter<T, 2>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->n2n_ia,
+                           m->n2n_ja, b.qa, b.qb, b.vol, g, frac, k3, m->risk);
     CFD_HIP(m, hipGetLastError());
     if (p->flux == CFD_FLUX_ROE) {
         const int epoch = ++m->mark_epoch;
-        hipLaunchKernelGGL(k_roe_mark<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, b.qa, b.qb,
-                           g, frac, m->mark, epoch);
+        hipLaunchKernelGGL(k_roe_mark<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->n2n_ia, m->n2n_ja,
+                           b.qa, b.qb, g, frac, m->mark, epoch, m->risk);
         hipLaunchKernelGGL(k_roe_zero<T>, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N,
                            m->mark, epoch, b.qb);
         CFD_HIP(m, hipGetLastError());
@@ -709,7 +745,7 @@ static void launch_res_flux(cfd_mesh *m, const ResArgs<T> &a, int order, bool li
 
 template <class T>
 int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats) {
-    ResArgs<T> a{m->N, m->NT, m->inc_ptr, m->inc, b.sa, b.qa, b.qb, b.rc, b.rd, b.dta, (T)p->gamma,
+    ResArgs<T> a{m->N, m->NT, m->order, m->inc_ptr, m->inc, b.sa, b.qa, b.qb, b.res, (T)p->gamma,
                  (p->limiter_order == 2) ? T(0.5) : T(0.0), p->entropy_fix, m->eigbits, m->flag};
     const int nb = xcd_grid((m->N + kBlock - 1) / kBlock);
     const bool lim = p->limiter != CFD_LIM_NONE;
@@ -723,14 +759,14 @@ int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool s
 
 int launch_delta_t(cfd_mesh *m, double cfl) {
     hipLaunchKernelGGL(k_delta_t, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, m->d.vol,
-                       m->d.dta, cfl, m->dt);
+                       m->d.res, cfl, m->dt);
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;
 }
 
 int launch_rms(cfd_mesh *m, double *sums5) {
     const int nparts = grid_for(m->N) > 1024 ? 1024 : grid_for(m->N);
-    hipLaunchKernelGGL(k_rms_partial, dim3(nparts), dim3(kBlock), 0, m->stream, m->N, m->d.rc, m->d.rd, m->red);
+    hipLaunchKernelGGL(k_rms_partial, dim3(nparts), dim3(kBlock), 0, m->stream, m->N, m->d.res, m->red);
     hipLaunchKernelGGL(k_reduce_final<5>, dim3(1), dim3(kBlock), 0, m->stream, nparts, m->red, m->redout);
     CFD_HIP(m, hipGetLastError());
     CFD_HIP(m, hipMemcpyAsync(m->hred, m->redout, 5 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
@@ -740,8 +776,8 @@ int launch_rms(cfd_mesh *m, double *sums5) {
 }
 
 int launch_update(cfd_mesh *m, double relax) {
-    hipLaunchKernelGGL(k_update, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, relax, m->x,
-                       m->d.qa);
+    hipLaunchKernelGGL(k_update, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, relax, m->order,
+                       m->x, m->d.qa);
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;
 }
@@ -768,12 +804,8 @@ int launch_convert_state_to_fp32(cfd_mesh *m) {
 }
 
 int launch_residual_to_fp64(cfd_mesh *m) {
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((size_t)5 * m->N)), dim3(kBlock), 0, m->stream,
-                       (size_t)5 * m->N, m->f.rc, m->d.rc);
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((size_t)5 * m->N)), dim3(kBlock), 0, m->stream,
-                       (size_t)5 * m->N, m->f.rd, m->d.rd);
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((size_t)m->N)), dim3(kBlock), 0, m->stream,
-                       (size_t)m->N, m->f.dta, m->d.dta);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((size_t)kRES * m->N)), dim3(kBlock), 0, m->stream,
+                       (size_t)kRES * m->N, m->f.res, m->d.res);
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;
 }
