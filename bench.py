@@ -157,7 +157,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build measures one GPU; the residual path shards by node partition with one halo "
-              "exchange per stage (see DESIGN.md), which is not part of this build. Run with --gpus 1.",
+              "exchange per stage, which is not built here. Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

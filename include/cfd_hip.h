@@ -7,8 +7,8 @@
  * takes the same arrays explicitly, keeps them resident in HBM behind an
  * opaque handle, and returns status codes instead of calling exit().  Each
  * entry point names the reference function it replaces (paths relative to
- * Trunk/ of the reference).  A C++ shim with the reference's exact
- * signatures is described in INTEGRATION.md.
+ * Trunk/ of the reference), so a C++ shim with the reference's exact
+ * signatures maps each reference call onto one call here.
  *
  * Conventions
  *   - Arrays use the reference layouts: conserved state and residuals are
@@ -195,7 +195,8 @@ int cfd_limiter(cfd_mesh *mesh, const cfd_params *p);
 int cfd_residual(cfd_mesh *mesh, const cfd_params *p, int add_time, double *eig6);
 /* Compute_DeltaT (Euler3D/src/Time_Step.cpp:23-50); returns CFL in *cfl. */
 int cfd_delta_t(cfd_mesh *mesh, const cfd_params *p, int iter, double *cfl);
-/* RMS of Conv+Diss as in Solver_Steady_Implicit.cpp:112-126. */
+/* RMS of Conv+Diss: RMS_Writer (MPFLOW3D/src/RMSIO.cpp:98-153); Euler3D squares Conv only
+ * (Euler3D/src/Solver_Steady_Implicit.cpp:112-126), which equals this when Diss = 0. */
 int cfd_residual_rms(cfd_mesh *mesh, double *rms);
 /* Compute_Jacobian (Euler3D/src/Jacobian.cpp:18-47 ->
  * Compute_Jacobian_Approximate_Roe, Euler3D/src/Roe_Jacobian.cpp:29-228):
