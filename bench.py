@@ -142,6 +142,132 @@ def cpu_baseline(m, p, q):
     return time.perf_counter() - t0, rc, rd
 
 
+def bump_dims(n_nodes):
+    """Bump-channel hex counts (3n, n, n) whose node count is closest to n_nodes."""
+    n = max(4, int(round((n_nodes / 3.0) ** (1.0 / 3.0))))
+    return 3 * n, n, n
+
+
+SOLVE_KW = dict(max_iter=400, tol=1e-10, krylov_m=30, max_restarts=4, rtol=1e-3)
+
+
+def solve_leg(ctx):
+    """Bump channel 48x16x16 (14,161 nodes), first-order Roe, M 0.5, AoA 0, backward Euler, CFL 1 -> 1000,
+    FGMRES(30) + ILU(0), to RMS 1e-10: plain FP64 vs mixed precision, each alone on the GPU."""
+    m = mesh.bump_channel(48, 16, 16, bump_h=0.1)
+    p = cfd.make_params(mach=0.5, alpha_deg=0.0, flux="roe", order=1)
+    out = {}
+    for mixed in (False, True):
+        dm = cfd.DeviceMesh(ctx, m, p)
+        r, _ = dm.solve_steady(p, precond=cfd.PREC_ILU0, mixed=mixed, **SOLVE_KW)  # warm-up (JIT, allocation)
+        dm.reset(p)
+        r, hist = dm.solve_steady(p, precond=cfd.PREC_ILU0, mixed=mixed, **SOLVE_KW)
+        out[mixed] = (r, dm.get(cfd.FIELD_Q))
+        dm.close()
+    (r0, q0), (r1, q1) = out[False], out[True]
+    rel_f = float(np.max(np.abs(np.array(r1.force) - np.array(r0.force))) / np.max(np.abs(r0.force)))
+    rel_q = float(np.max(np.abs(q1 - q0)) / np.max(np.abs(q0)))
+    res = {"workload": "bump channel 48x16x16 (%d nodes), 1st-order Roe, M 0.5, FGMRES(30)+ILU(0), RMS < 1e-10"
+                       % m.n_node,
+           "fp64": {"solves_per_s": 1.0 / r0.seconds, "seconds": r0.seconds, "newton": r0.iterations,
+                    "krylov": r0.krylov_total, "converged": bool(r0.converged)},
+           "mixed": {"solves_per_s": 1.0 / r1.seconds, "seconds": r1.seconds, "newton": r1.iterations,
+                     "krylov": r1.krylov_total, "converged": bool(r1.converged),
+                     "force_rel_err_vs_fp64": rel_f, "state_rel_err_vs_fp64": rel_q},
+           "speedup_mixed": r0.seconds / r1.seconds}
+    print("bench: solve 48x16x16 fp64 %.3f s (%d/%d), mixed %.3f s (%d/%d), force err %.2e" % (
+        r0.seconds, r0.iterations, r0.krylov_total, r1.seconds, r1.iterations, r1.krylov_total, rel_f),
+        file=sys.stderr, flush=True)
+    return res
+
+
+def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202):
+    """Steady-solve requests on bump channels of log-uniform size in [nmin, nmax] nodes, bump height U(0.05,
+    0.15), Mach U(0.3, 0.85), AoA U(-2, 6) deg, Roe or HLLC (first order), to RMS 1e-10.  The plain FP64 GPU
+    path solves them one at a time (meshes resident); the server solves them in mixed precision with
+    continuous batching, first as a burst (capacity) and then as a Poisson stream at 80% of that capacity
+    (latency)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_meshes))
+    heights = rng.uniform(0.05, 0.15, n_meshes)
+    pool = [mesh.bump_channel(*bump_dims(n), bump_h=h) for n, h in zip(sizes, heights)]
+    reqs = []
+    for _ in range(n_req):
+        reqs.append((int(rng.integers(n_meshes)), float(rng.uniform(0.3, 0.85)), float(rng.uniform(-2.0, 6.0)),
+                     "roe" if rng.random() < 0.5 else "hllc"))
+
+    def params(r):
+        return cfd.make_params(mach=r[1], alpha_deg=r[2], flux=r[3], order=1)
+
+    # plain FP64 GPU path, one request at a time, meshes already resident
+    dms = [cfd.DeviceMesh(ctx, m, params(reqs[0])) for m in pool]
+    t64, res64 = 0.0, []
+    for r in reqs:
+        p = params(r)
+        dms[r[0]].reset(p)
+        out, _ = dms[r[0]].solve_steady(p, alpha_deg=r[2], precond=cfd.PREC_ILU0, mixed=False, check=False,
+                                        **SOLVE_KW)
+        t64 += out.seconds
+        res64.append(out)
+    for d in dms:
+        d.close()
+    print("bench: serve baseline fp64 %d solves in %.2f s" % (n_req, t64), file=sys.stderr, flush=True)
+
+    srv = cfd.Server(ctx, hbm_budget=200 << 30, max_inflight=inflight)
+    for k, m in enumerate(pool):  # structure resident before timing (one solve per mesh)
+        srv.submit(-1 - k, k, m, params(reqs[0]), alpha_deg=reqs[0][2], precond=cfd.PREC_ILU0, mixed=True,
+                   **SOLVE_KW)
+    got = 0
+    while got < n_meshes:
+        got += len(srv.poll(wait=30.0))
+
+    def run(arrivals):
+        t_start = srv.stats()["clock"]
+        done, k, t0 = [], 0, time.perf_counter()
+        while len(done) < n_req:
+            now = time.perf_counter() - t0
+            while k < n_req and arrivals[k] <= now:
+                r = reqs[k]
+                srv.submit(k, r[0], pool[r[0]], params(r), alpha_deg=r[2], precond=cfd.PREC_ILU0, mixed=True,
+                           **SOLVE_KW)
+                k += 1
+            wait = 0.0 if k >= n_req else max(0.0, arrivals[k] - (time.perf_counter() - t0))
+            done += srv.poll(wait=min(wait, 0.05) if k < n_req else 1.0)
+        t_end = max(c["t_done"] for c in done)
+        lat = np.array([c["t_done"] - c["t_submit"] for c in done])
+        return done, t_end - t_start, lat
+
+    burst, span_b, lat_b = run(np.zeros(n_req))
+    cap = n_req / span_b
+    gaps = rng.exponential(1.0 / (0.8 * cap), n_req)
+    stream, span_s, lat_s = run(np.cumsum(gaps) - gaps[0])
+    st = srv.stats()
+    srv.close()
+    bad = [c for c in burst + stream if c["status"] != 0 or not c["result"].converged]
+    ferr = 0.0
+    for c in stream:
+        f64 = np.array(res64[c["id"]].force)
+        ferr = max(ferr, float(np.max(np.abs(np.array(c["result"].force) - f64)) / np.max(np.abs(f64))))
+    base = n_req / t64
+    res = {"workload": "%d requests on %d bump channels of %d-%d nodes (log-uniform), 1st-order Roe/HLLC, "
+                       "M 0.3-0.85, AoA -2..6, RMS < 1e-10, %d in flight" % (n_req, n_meshes, min(m.n_node for m in pool),
+                                                                         max(m.n_node for m in pool), inflight),
+           "fp64_sequential_solves_per_s": base,
+           "serving_capacity_solves_per_s": cap,
+           "serving_vs_fp64": cap / base,
+           "stream_rate_per_s": 0.8 * cap,
+           "stream_solves_per_s": n_req / span_s,
+           "latency_p50_s": float(np.percentile(lat_s, 50)), "latency_p99_s": float(np.percentile(lat_s, 99)),
+           "burst_latency_p50_s": float(np.percentile(lat_b, 50)),
+           "burst_latency_p99_s": float(np.percentile(lat_b, 99)),
+           "force_rel_err_mixed_vs_fp64": ferr, "failed": len(bad), "peak_hbm_bytes": int(st["peak_bytes"]),
+           "mesh_reuse": int(st["reused"])}
+    print("bench: serve fp64 %.2f solves/s, serving %.2f solves/s (%.2fx), stream p50 %.3f s p99 %.3f s, "
+          "force err %.2e, failed %d" % (base, cap, cap / base, res["latency_p50_s"], res["latency_p99_s"], ferr,
+                                          len(bad)), file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +277,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--only", action="store_true", help="run only the headline variant (for profiling)")
+    ap.add_argument("--no-solves", action="store_true", help="skip the steady-solve and serving legs")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-meshes", type=int, default=6)
+    ap.add_argument("--serve-nodes", type=str, default="20000,150000", help="min,max nodes of the serving meshes")
+    ap.add_argument("--serve-inflight", type=int, default=6)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (scale factor for HBM counters)")
     args = ap.parse_args()
@@ -228,6 +359,12 @@ def main():
                    parity_max_rel_diff=float(rel))
         print("bench: cpu oracle %.2f s, GPU-vs-oracle max rel diff %.3e" % (secs, rel), file=sys.stderr, flush=True)
     dm.close()
+    solves = serving = None
+    if not args.no_solves and not args.only:
+        del dm
+        solves = solve_leg(ctx)
+        nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
+        serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight)
     ctx.close()
     r_achieved = variants["roe-fp64-G+L+R2"]["r_gbs"]
     out = {
@@ -253,6 +390,8 @@ def main():
                      "algorithmic_bytes_per_launch": bytes64["R2split"],
                      "avg_launch_ms": round(variants["roe-fp64-G+L+R2"]["r_ms"], 4)},
         "cpu_baseline": cpu,
+        "solves": solves,
+        "serving": serving,
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},
     }

@@ -71,6 +71,53 @@ class SolveResult(ctypes.Structure):
                 ("pressure_integral", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("max_inflight", ctypes.c_int32), ("lookahead", ctypes.c_int32), ("max_bypass", ctypes.c_int32),
+                ("keep_resident", ctypes.c_int32), ("hbm_budget", ctypes.c_int64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("mesh_key", ctypes.c_int64), ("mesh", ctypes.POINTER(MeshDesc)),
+                ("params", Params), ("opts", SolveOpts), ("q_out", ctypes.c_void_p), ("rms_hist", ctypes.c_void_p)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("status", ctypes.c_int32), ("reused", ctypes.c_int32),
+                ("result", SolveResult), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("footprint", ctypes.c_int64), ("device_bytes", ctypes.c_int64),
+                ("message", ctypes.c_char * 160)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("bytes_in_use", ctypes.c_int64), ("peak_bytes", ctypes.c_int64), ("queued", ctypes.c_int32),
+                ("running", ctypes.c_int32), ("resident", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("admitted", ctypes.c_int64), ("reused", ctypes.c_int64), ("evicted", ctypes.c_int64),
+                ("clock", ctypes.c_double)]
+
+
+def solve_opts(max_iter=200, tol=1e-10, first_order_iters=0, relaxation=1.0, wall_tag=3, alpha_deg=0.0,
+               precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, mixed=False, solver=SOLVER_FGMRES,
+               sweeps=0):
+    lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
+    return SolveOpts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, lin)
+
+
+def mesh_desc(mesh, bedge_type=None):
+    """Builds a MeshDesc for a mesh; returns (desc, arrays) -- keep the arrays alive while desc is in use."""
+    c = np.ascontiguousarray
+    keep = dict(
+        edge=c(mesh.edge, np.int32), edge_area=c(mesh.edge_area, np.float64), bedge=c(mesh.bedge, np.int32),
+        bedge_area=c(mesh.bedge_area, np.float64),
+        bedge_type=c(mesh.bedge_type if bedge_type is None else bedge_type, np.int32),
+        bedge_tag=c(mesh.bedge_tag, np.int32), xyz=c(mesh.xyz, np.float64), vol=c(mesh.vol, np.float64),
+        n2n_ia=c(mesh.n2n_ia, np.int32), n2n_ja=c(mesh.n2n_ja, np.int32))
+    k = keep
+    d = MeshDesc(k["vol"].size, k["edge"].size // 2, k["bedge"].size // 2, _p(k["edge"]), _p(k["edge_area"]),
+                 _p(k["bedge"]), _p(k["bedge_area"]), _p(k["bedge_type"]), _p(k["bedge_tag"]), _p(k["xyz"]),
+                 _p(k["vol"]), _p(k["n2n_ia"]), _p(k["n2n_ja"]))
+    return d, keep
+
+
 _LIB = None
 
 
@@ -87,6 +134,7 @@ def lib():
         L.cfd_footprint_bytes.argtypes = [ctypes.c_int32] * 5
         L.cfd_mesh_stream.restype = ctypes.c_void_p
         L.cfd_bsr_structure.restype = ctypes.c_int32
+        L.cfd_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_double]
         _LIB = L
     return _LIB
 
@@ -146,20 +194,8 @@ class DeviceMesh:
 
     def __init__(self, ctx, mesh, params, bedge_type=None):
         self.ctx = ctx
-        c = np.ascontiguousarray
-        self._keep = dict(
-            edge=c(mesh.edge, np.int32), edge_area=c(mesh.edge_area, np.float64), bedge=c(mesh.bedge, np.int32),
-            bedge_area=c(mesh.bedge_area, np.float64),
-            bedge_type=c(mesh.bedge_type if bedge_type is None else bedge_type, np.int32),
-            bedge_tag=c(mesh.bedge_tag, np.int32), xyz=c(mesh.xyz, np.float64), vol=c(mesh.vol, np.float64),
-            n2n_ia=c(mesh.n2n_ia, np.int32), n2n_ja=c(mesh.n2n_ja, np.int32))
-        k = self._keep
-        self.N = k["vol"].size
-        self.E = k["edge"].size // 2
-        self.B = k["bedge"].size // 2
-        d = MeshDesc(self.N, self.E, self.B, _p(k["edge"]), _p(k["edge_area"]), _p(k["bedge"]), _p(k["bedge_area"]),
-                     _p(k["bedge_type"]), _p(k["bedge_tag"]), _p(k["xyz"]), _p(k["vol"]), _p(k["n2n_ia"]),
-                     _p(k["n2n_ja"]))
+        d, self._keep = mesh_desc(mesh, bedge_type)
+        self.N, self.E, self.B = d.n_node, d.n_edge, d.n_bedge
         self.h = ctypes.c_void_p()
         ctx.check(lib().cfd_mesh_create(ctx.h, ctypes.byref(d), ctypes.byref(params), ctypes.byref(self.h)))
 
@@ -210,6 +246,9 @@ class DeviceMesh:
 
     def limiter(self, p):
         self.ctx.check(lib().cfd_limiter(self.h, ctypes.byref(p)))
+
+    def gradient_limiter(self, p):
+        self.ctx.check(lib().cfd_gradient_limiter(self.h, ctypes.byref(p)))
 
     def residual(self, p, stats=False):
         eig = np.zeros(6) if stats else None
@@ -266,17 +305,79 @@ class DeviceMesh:
                                         ctypes.byref(lift), ctypes.byref(drag), ctypes.byref(pint)))
         return np.array(f[:]), lift.value, drag.value, pint.value
 
+    def reset(self, p):
+        self.ctx.check(lib().cfd_mesh_reset(self.h, ctypes.byref(p)))
+
     def solve_steady(self, p, max_iter=200, tol=1e-10, first_order_iters=0, relaxation=1.0, wall_tag=3,
                      alpha_deg=0.0, precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, mixed=False,
                      solver=SOLVER_FGMRES, sweeps=0, check=True):
-        lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
-        o = SolveOpts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, lin)
+        o = solve_opts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, precond, krylov_m,
+                       max_restarts, rtol, mixed, solver, sweeps)
         r = SolveResult()
         hist = np.zeros(max_iter)
         rc = lib().cfd_solve_steady(self.h, ctypes.byref(p), ctypes.byref(o), ctypes.byref(r), _p(hist))
         if check:
             self.ctx.check(rc)
         return r, hist[: r.iterations + 1]
+
+
+class Server:
+    """Continuous-batching solve server (cfd_server_* in the C ABI).
+
+    submit() queues a steady solve; completions arrive through poll().  The
+    arrays of a request are kept alive here until its completion is polled."""
+
+    def __init__(self, ctx, hbm_budget, max_inflight=4, lookahead=16, max_bypass=8, keep_resident=True):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        o = ServerOpts(int(max_inflight), int(lookahead), int(max_bypass), int(bool(keep_resident)), int(hbm_budget))
+        ctx.check(lib().cfd_server_create(ctx.h, ctypes.byref(o), ctypes.byref(self.h)))
+        self._live = {}
+        self._descs = {}
+
+    def submit(self, rid, mesh_key, mesh, params, want_state=False, **solve_kw):
+        if mesh_key not in self._descs:
+            self._descs[mesh_key] = mesh_desc(mesh)
+        d, _ = self._descs[mesh_key]
+        o = solve_opts(**solve_kw)
+        q = np.zeros((5, d.n_node + d.n_bedge)) if want_state else None
+        hist = np.zeros(o.max_iter)
+        req = Request(int(rid), int(mesh_key), ctypes.pointer(d), params, o, _p(q), _p(hist))
+        self.ctx.check(lib().cfd_server_submit(self.h, ctypes.byref(req)))
+        self._live[int(rid)] = (req, q, hist)
+
+    def poll(self, max_n=64, wait=0.0):
+        buf = (Completion * max_n)()
+        n = lib().cfd_server_poll(self.h, buf, int(max_n), float(wait))
+        out = []
+        for k in range(n):
+            c = buf[k]
+            _, q, hist = self._live.pop(int(c.id))
+            out.append(dict(id=int(c.id), status=int(c.status), reused=bool(c.reused), result=c.result,
+                            t_submit=c.t_submit, t_admit=c.t_admit, t_done=c.t_done, footprint=int(c.footprint),
+                            device_bytes=int(c.device_bytes), message=c.message.decode(errors="replace"), q=q,
+                            hist=hist[: c.result.iterations + 1]))
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        self.ctx.check(lib().cfd_server_get_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_ if f != "pad_"}
+
+    def pending(self):
+        return len(self._live)
+
+    def close(self):
+        if self.h:
+            lib().cfd_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def footprint_bytes(n_node, n_edge, n_bedge, krylov_m=30, mixed=True):

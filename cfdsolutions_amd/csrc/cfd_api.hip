@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -16,13 +18,20 @@
 // ---------------------------------------------------------------------------
 // errors and allocation
 // ---------------------------------------------------------------------------
+// The message of the last failure is kept per calling thread (meshes of one
+// context may be driven from several threads) and, under the context's lock,
+// on the context.
+static thread_local std::string t_last_error;
+
 int cfd_set_error(cfd_ctx *ctx, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    t_last_error = buf;
     if (ctx) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof(buf), fmt, ap);
-        va_end(ap);
+        std::lock_guard<std::mutex> g(ctx->lock);
         ctx->err = buf;
     }
     return code;
@@ -57,11 +66,26 @@ extern "C" int cfd_ctx_create(int device, cfd_ctx **out) {
 
 extern "C" void cfd_ctx_destroy(cfd_ctx *ctx) { delete ctx; }
 
-extern "C" const char *cfd_last_error(const cfd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+extern "C" const char *cfd_last_error(const cfd_ctx *ctx) {
+    if (!ctx) return "null context";
+    if (!t_last_error.empty()) return t_last_error.c_str();
+    std::lock_guard<std::mutex> g(const_cast<cfd_ctx *>(ctx)->lock);
+    t_last_error = ctx->err;
+    return t_last_error.c_str();
+}
 
 // ---------------------------------------------------------------------------
 // mesh creation
 // ---------------------------------------------------------------------------
+// Rows of each level followed by -1 up to a multiple of w.
+static void pad_levels(const std::vector<int> &ptr, const std::vector<int> &rows, int w, std::vector<int> &out) {
+    out.clear();
+    for (size_t l = 0; l + 1 < ptr.size(); l++) {
+        for (int k = ptr[l]; k < ptr[l + 1]; k++) out.push_back(rows[k]);
+        while (out.size() % w) out.push_back(-1);
+    }
+}
+
 static void build_levels(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau, int N,
                          bool upper, std::vector<int> &ptr, std::vector<int> &rows, int &maxw) {
     std::vector<int> lev(N, 0);
@@ -231,6 +255,46 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
             }
             for (int b = 0; b < B; b++) inc[pos[iperm[d->bedge[2 * b]]]++] = make_int2(N + b, E + b);
         }
+        // SELL-64 copy of the incidence lists with the area vector of each
+        // incidence (see cfd_mesh::nbr); padding entries are never read.
+        const int nsl = (N + kSlice - 1) / kSlice;
+        std::vector<int> sptr(nsl + 1, 0);
+        {
+            int64_t tot = 0;
+            for (int sl = 0; sl < nsl; sl++) {
+                int w = 0;
+                for (int i = sl * kSlice; i < std::min(N, (sl + 1) * kSlice); i++) w = std::max(w, ip[i + 1] - ip[i]);
+                tot += (int64_t)kSlice * w;
+                if (tot > INT32_MAX) {
+                    delete m;
+                    return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: mesh too large for 32-bit incidence offsets");
+                }
+                sptr[sl + 1] = (int)tot;
+            }
+        }
+        m->nsell = (size_t)sptr[nsl];
+        std::vector<int> nbr(m->nsell, 0);
+        std::vector<double> isx(m->nsell, 0.0), isy(m->nsell, 0.0), isz(m->nsell, 0.0);
+        for (int i = 0; i < N; i++) {
+            size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
+            for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) {
+                const int2 en = inc[k];
+                const double *S;
+                if (en.x >= N) {
+                    nbr[idx] = en.x;
+                    S = d->bedge_area + 3 * (size_t)(en.x - N);
+                } else if (en.y >= 0) {
+                    nbr[idx] = en.x;
+                    S = d->edge_area + 3 * (size_t)en.y;
+                } else {
+                    nbr[idx] = ~en.x;
+                    S = d->edge_area + 3 * (size_t)(~en.y);
+                }
+                isx[idx] = S[0];
+                isy[idx] = S[1];
+                isz[idx] = S[2];
+            }
+        }
         // BSR pattern, Create_CRS_SolverBlockMatrix (Euler3D/src/Solver.cpp:612-675):
         // row i holds i and its neighbours in ascending column order.
         m->h_ia.assign(N + 1, 0);
@@ -262,6 +326,11 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
                 off[k] = (int)(it - m->h_ja.begin());
             }
         }
+        std::vector<int> soff(m->nsell, -1);
+        for (int i = 0; i < N; i++) {
+            size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
+            for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) soff[idx] = off[k];
+        }
         std::vector<int> lrows, urows;
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, false, m->lower.ptr, lrows, m->lower.maxwidth);
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, true, m->upper.ptr, urows, m->upper.maxwidth);
@@ -277,7 +346,7 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
         }
 
         // packed node records: q (freestream), coordinates
-        std::vector<double> qa((size_t)kQA * m->NT, 0.0), sa((size_t)4 * (E + B), 0.0), vol(N);
+        std::vector<double> qa((size_t)kQA * m->NT, 0.0), sa((size_t)4 * B, 0.0), vol(N);
         double q5[5];
         freestream_q(p, q5);
         for (int i = 0; i < m->NT; i++) {
@@ -287,10 +356,8 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
                 vol[i] = d->vol[order[i]];
             }
         }
-        for (int e = 0; e < E; e++)
-            for (int k = 0; k < 3; k++) sa[4 * (size_t)e + k] = d->edge_area[3 * (size_t)e + k];
         for (int b = 0; b < B; b++)
-            for (int k = 0; k < 3; k++) sa[4 * ((size_t)E + b) + k] = d->bedge_area[3 * (size_t)b + k];
+            for (int k = 0; k < 3; k++) sa[4 * (size_t)b + k] = d->bedge_area[3 * (size_t)b + k];
         std::vector<int> bnode(B), btag(B, 0);
         for (int b = 0; b < B; b++) {
             bnode[b] = iperm[d->bedge[2 * b]];
@@ -299,8 +366,12 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
 
         m->order = upload(m, order.data(), order.size(), &rc);
         if (!rc) m->inc_ptr = upload(m, ip.data(), ip.size(), &rc);
-        if (!rc) m->inc = upload(m, inc.data(), inc.size(), &rc);
-        if (!rc) m->inc_off = upload(m, off.data(), off.size(), &rc);
+        if (!rc) m->sptr = upload(m, sptr.data(), sptr.size(), &rc);
+        if (!rc) m->nbr = upload(m, nbr.data(), nbr.size(), &rc);
+        if (!rc) m->soff = upload(m, soff.data(), soff.size(), &rc);
+        if (!rc) m->d.isx = upload(m, isx.data(), isx.size(), &rc);
+        if (!rc) m->d.isy = upload(m, isy.data(), isy.size(), &rc);
+        if (!rc) m->d.isz = upload(m, isz.data(), isz.size(), &rc);
         if (!rc) m->n2n_ia = upload(m, nia.data(), nia.size(), &rc);
         if (!rc) m->n2n_ja = upload(m, nja.data(), nja.size(), &rc);
         if (!rc) m->bnode = upload(m, bnode.data(), bnode.size(), &rc);
@@ -314,6 +385,23 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
         if (!rc) m->iau = upload(m, m->h_iau.data(), m->h_iau.size(), &rc);
         if (!rc) m->lower.rows = upload(m, lrows.data(), lrows.size(), &rc);
         if (!rc) m->upper.rows = upload(m, urows.data(), urows.size(), &rc);
+        {
+            std::vector<int> s12, s64;
+            pad_levels(m->lower.ptr, lrows, 12, s12);
+            pad_levels(m->lower.ptr, lrows, 64, s64);
+            m->lower.n12 = (int)s12.size();
+            m->lower.n64 = (int)s64.size();
+            if (!rc) m->lower.slots12 = upload(m, s12.data(), s12.size(), &rc);
+            if (!rc) m->lower.slots64 = upload(m, s64.data(), s64.size(), &rc);
+            pad_levels(m->upper.ptr, urows, 12, s12);
+            m->upper.n12 = (int)s12.size();
+            if (!rc) m->upper.slots12 = upload(m, s12.data(), s12.size(), &rc);
+            std::vector<int> zeros((size_t)N + 1, 0);
+            if (!rc) m->ready = upload(m, zeros.data(), (size_t)N, &rc);
+            if (!rc) m->epoch = upload(m, zeros.data(), 1, &rc);
+            const char *sf = getenv("CFD_SYNC_FREE");
+            m->sync_free = !(sf && sf[0] == '0');
+        }
         if (rc) {
             cfd_mesh_destroy(m);
             return rc;
@@ -364,8 +452,10 @@ extern "C" int64_t cfd_footprint_bytes(int32_t N, int32_t E, int32_t B, int32_t 
     const int64_t n = N, e = E, b = B, nnz = n + 2 * e;
     const int64_t s = mixed ? 4 : 8;
     int64_t bytes = 0;
-    bytes += 8 * (kQA * (n + b) + kQB * n + 4 * (e + b) + n + 8 * n + 12 * n + 5 * n);  // state, geometry, residuals
-    bytes += 12 * (2 * e + b) + 4 * (n + 1) + 8 * e + 4 * (n + 1 + nnz + n) + 12 * b + 4 * n;  // topology
+    bytes += 8 * (kQA * (n + b) + kQB * n + 4 * b + n + 8 * n + 12 * n + 5 * n);  // state, geometry, residuals
+    bytes += 4 * (n + 1) + 8 * e + 4 * (n + 1 + nnz + n) + 12 * b + 4 * n;       // topology, BSR pattern
+    bytes += 4 * 7 * n + 4 * 4 * n + 8 * n;                                      // level schedules, flags
+    bytes += (32 * (2 * e + b) * 5) / 4 + 4 * (n / kSlice + 2);  // SELL incidences (<= 25% slice padding)
     bytes += 25 * s * nnz * 2 + 25 * s * n;        // Jacobian, ILU(0) factors, inverse pivots
     if (mixed) bytes += 8 * 25 * 0;                // FP64 values are not kept in mixed mode
     bytes += s * 5 * n * (2 * (int64_t)km + 2) + 8 * 5 * n * 3;  // Krylov basis, work, b, x, r
@@ -385,11 +475,15 @@ int cfd_ensure_fp32(cfd_mesh *m) {
         const size_t N = m->N, NT = m->NT, EB = (size_t)m->E + m->B;
         m->f.qa = (float *)cfd_dalloc(m, sizeof(float) * kQA * NT);
         m->f.qb = (float *)cfd_dalloc(m, sizeof(float) * kQB * N);
-        m->f.sa = (float *)cfd_dalloc(m, sizeof(float) * 4 * EB);
+        m->f.sa = (float *)cfd_dalloc(m, sizeof(float) * 4 * (size_t)m->B);
         m->f.vol = (float *)cfd_dalloc(m, sizeof(float) * N);
         m->f.lsq = (float *)cfd_dalloc(m, sizeof(float) * 8 * N);
         m->f.res = (float *)cfd_dalloc(m, sizeof(float) * kRES * N);
-        if (!m->f.qa || !m->f.qb || !m->f.sa || !m->f.vol || !m->f.lsq || !m->f.res)
+        m->f.isx = (float *)cfd_dalloc(m, sizeof(float) * m->nsell);
+        m->f.isy = (float *)cfd_dalloc(m, sizeof(float) * m->nsell);
+        m->f.isz = (float *)cfd_dalloc(m, sizeof(float) * m->nsell);
+        if (!m->f.qa || !m->f.qb || !m->f.sa || !m->f.vol || !m->f.lsq || !m->f.res || !m->f.isx || !m->f.isy ||
+            !m->f.isz)
             return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (FP32 buffers)");
         int rc = launch_convert_geometry_fp32(m);
         if (rc) return rc;
@@ -401,6 +495,38 @@ int cfd_ensure_fp32(cfd_mesh *m) {
         if (rc) return rc;
         m->f_version = m->q_version;
     }
+    return CFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// state reset (for reuse of a resident mesh by another solve)
+// ---------------------------------------------------------------------------
+static int check_params(cfd_mesh *m, const cfd_params *p);
+
+__global__ void k_set_state(int NT, double q0, double q1, double q2, double q3, double q4, double *qa) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= NT) return;
+    double *r = qa + kQA * (size_t)i;
+    r[0] = q0;
+    r[1] = q1;
+    r[2] = q2;
+    r[3] = q3;
+    r[4] = q4;
+}
+
+extern "C" int cfd_mesh_reset(cfd_mesh *m, const cfd_params *p) {
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    hipSetDevice(m->ctx->device);
+    double q5[5];
+    freestream_q(p, q5);
+    hipLaunchKernelGGL(k_set_state, dim3((m->NT + 255) / 256), dim3(256), 0, m->stream, m->NT, q5[0], q5[1], q5[2],
+                       q5[3], q5[4], m->d.qa);
+    CFD_HIP(m, hipGetLastError());
+    m->q_version++;
+    m->jac_valid = false;
+    m->prec_valid = false;
+    m->residual_valid = false;
     return CFD_OK;
 }
 
@@ -575,6 +701,16 @@ extern "C" int cfd_limiter(cfd_mesh *m, const cfd_params *p) {
         return rc ? rc : launch_limiter<float>(m, m->f, p);
     }
     return launch_limiter<double>(m, m->d, p);
+}
+
+extern "C" int cfd_gradient_limiter(cfd_mesh *m, const cfd_params *p) {
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    if (p->precision == CFD_FP32) {
+        rc = cfd_ensure_fp32(m);
+        return rc ? rc : launch_gradient_limiter<float>(m, m->f, p);
+    }
+    return launch_gradient_limiter<double>(m, m->d, p);
 }
 
 extern "C" int cfd_residual(cfd_mesh *m, const cfd_params *p, int add_time, double *eig6) {
@@ -779,8 +915,7 @@ extern "C" int cfd_solve_steady(cfd_mesh *m, const cfd_params *p0, const cfd_sol
         rc = cfd_bc(m, &pi, it);
         if (rc) break;
         if (pi.order == 2) {
-            rc = cfd_gradient(m, &pi);
-            if (!rc && pi.limiter != CFD_LIM_NONE) rc = cfd_limiter(m, &pi);
+            rc = pi.limiter != CFD_LIM_NONE ? cfd_gradient_limiter(m, &pi) : cfd_gradient(m, &pi);
             if (rc) break;
         }
         rc = cfd_residual(m, &pi, 1, nullptr);

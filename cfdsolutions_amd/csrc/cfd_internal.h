@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -11,6 +12,7 @@
 
 struct cfd_ctx {
     int device = 0;
+    std::mutex lock;  // guards err
     std::string err;
 };
 
@@ -18,16 +20,21 @@ struct cfd_ctx {
 // neighbour gathers is packed per node so one gather touches whole sectors:
 //   qa [NT][8]  = q1..q5, x, y, z        (ghost records hold q only)
 //   qb [N][24]  = grad (Q1x,Q1y,Q1z,...,Q5z), phi1..phi5, 4 pad
-//   sa [E+B][4] = area vector x, y, z, pad (interior edges, then boundary)
+//   sa [B][4]   = outward area vector x, y, z, pad of each boundary edge
 //   lsq [N][8]  = r11 r12 r13 r22 r23 ro22 ro33 pad
 //   res [N][12] = convective residual (5), dissipative residual (5), time-step
 //                 accumulator, pad; one contiguous record per node because
 //                 nodes are processed in a locality order, not storage order.
+//   isx, isy, isz = area vector of every node incidence (edge orientation,
+//                 L -> R, boundary outward) in the sliced layout of
+//                 cfd_mesh::nbr, so each step of a node walk is one coalesced
+//                 load per wavefront.
 template <class T>
 struct ResidualBufs {
     T *qa = nullptr;
     T *qb = nullptr;
     T *sa = nullptr;
+    T *isx = nullptr, *isy = nullptr, *isz = nullptr;
     T *vol = nullptr;
     T *lsq = nullptr;
     T *res = nullptr;
@@ -42,6 +49,11 @@ struct LevelSchedule {
     int *rows = nullptr;    // device: rows sorted by level
     int nlevels = 0;
     int maxwidth = 0;
+    // rows in level order with every level padded (-1) to a multiple of 12
+    // (sync-free sweeps, 12 rows per wavefront) and of 64 (sync-free ILU(0),
+    // one row per lane)
+    int *slots12 = nullptr, *slots64 = nullptr;
+    int n12 = 0, n64 = 0;
 };
 
 struct cfd_mesh {
@@ -55,9 +67,18 @@ struct cfd_mesh {
     int *order = nullptr;     // [N] reference index of each internal node (internal = Morton order)
     std::vector<int> h_order;
     int *inc_ptr = nullptr;   // [N+1]
-    int2 *inc = nullptr;      // [2E+B] {neighbour (ghost >= N), area index e if the node is the edge's L
-                              //  end, ~e if it is the R end}, in edge order
-    int *inc_off = nullptr;   // [2E+B] BSR block of the off-diagonal (i, nbr), -1 for ghosts
+    // node incidence lists, in the order the reference's sequential loops
+    // deliver each node's contributions: interior edges by edge index, then
+    // boundary edges (inc_ptr gives each node's count)
+    // The same incidence lists in a sliced ELL layout (SELL-64): slice s holds
+    // internal nodes 64s..64s+63; entry k of the node in lane l sits at
+    // sptr[s] + 64 k + l, so the 64 lanes of a wavefront read consecutive
+    // words at every step.  nbr = j if the node is the L end of the edge,
+    // ~j if it is the R end, N + b for boundary edge b (ghost record).
+    int *sptr = nullptr;      // [ceil(N/64)+1]
+    int *nbr = nullptr;       // [sptr[last]]
+    int *soff = nullptr;      // [sptr[last]] BSR block (row of the node, column of the neighbour), -1 for ghosts
+    size_t nsell = 0;
     int *n2n_ia = nullptr, *n2n_ja = nullptr;
     int *bnode = nullptr, *btype = nullptr, *btag = nullptr;  // [B]
     int *mark = nullptr;      // [N] limiter correction marks
@@ -80,6 +101,9 @@ struct cfd_mesh {
     double *gslu = nullptr;   // [N][25] no-pivot LU of the diagonal blocks (block GS)
     double *rowbuf = nullptr; // [N] per-row sums of the GS sweep
     LevelSchedule lower, upper;
+    bool sync_free = true;    // single-launch triangular sweeps (else one launch per level)
+    int *ready = nullptr;     // [N] sweep completion epochs
+    int *epoch = nullptr;     // [1] current sweep epoch
     bool jac_valid = false, jac_mixed = false;
     bool prec_valid = false;
     int prec_kind = -1;
@@ -126,6 +150,13 @@ __device__ __forceinline__ int xcd_block() {
     const int per = gridDim.x / kXcd;
     return (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
 }
+constexpr int kSlice = 64;  // nodes per SELL slice (one wavefront)
+// Streaming load: read once per kernel, so it is not kept in L2 at the
+// expense of the node records that neighbour gathers reuse.
+template <class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
 
 // ---------------------------------------------------------------------------
 // kernel launchers implemented in residual.hip / linsolve.hip
@@ -137,6 +168,8 @@ template <class T>
 int launch_gradient(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);
 template <class T>
 int launch_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);
+template <class T>
+int launch_gradient_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);
 template <class T>
 int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats);
 int launch_delta_t(cfd_mesh *m, double cfl);

@@ -120,10 +120,8 @@ template <class TA>
 struct JacArgs {
     int N, NT;
     const int *order;
-    const int *ip;
-    const int2 *inc;
-    const int *off, *iau;
-    const double *sa, *qa, *vol, *dt, *res;
+    const int *ip, *sptr, *nbr, *soff, *iau;
+    const double *isx, *isy, *isz, *qa, *vol, *dt, *res;
     TA *a;
     double *b;
     double g;
@@ -147,13 +145,15 @@ __global__ __launch_bounds__(128) void k_jacobian(JacArgs<TA> a) {
     const double vd = a.vol[i] / a.dt[i];
 #pragma unroll
     for (int t = 0; t < 5; t++) D[6 * t] = vd;
-    for (int k = a.ip[i]; k < a.ip[i + 1]; k++) {
-        const int2 en = a.inc[k];
-        const int j = en.x;
-        const bool iL = en.y >= 0;
-        const double *s = a.sa + 4 * (size_t)(iL ? en.y : ~en.y);
+    const int deg = a.ip[i + 1] - a.ip[i];
+    size_t idx = (size_t)a.sptr[i / kSlice] + (i % kSlice);
+    for (int k = 0; k < deg; k++, idx += kSlice) {
+        const int jr = ld_stream(a.nbr + idx);
+        const bool iL = jr >= 0;  // ghosts (jr >= N) always pair with a physical L
+        const int j = iL ? jr : ~jr;
         double nx, ny, nz;
-        const double area = area_normal(s[0], s[1], s[2], nx, ny, nz);
+        const double area = area_normal(ld_stream(a.isx + idx), ld_stream(a.isy + idx), ld_stream(a.isz + idx), nx,
+                                        ny, nz);
         const double *rj = a.qa + kQA * (size_t)j;
         double QL[5], QR[5];
 #pragma unroll
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(128) void k_jacobian(JacArgs<TA> a) {
             for (int t = 0; t < 25; t++) D[t] += 0.5 * (J[t] + AR[t]) * area;
             continue;
         }
-        TA *O = a.a + 25 * (size_t)a.off[k];
+        TA *O = a.a + 25 * (size_t)ld_stream(a.soff + idx);
         if (iL) {
             flux_jac(a.g, QL, nx, ny, nz, J);
 #pragma unroll
@@ -364,6 +364,141 @@ __global__ __launch_bounds__(kBlock) void k_trsv_level(int nrows, const int *__r
         for (int c = 0; c < 5; c++) s += (TV)D[c] * yy[c];
         z[5 * (size_t)i + r] = s;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Sync-free triangular sweeps: one launch per sweep instead of one per level.
+// Rows are taken in level order and every wavefront holds rows of a single
+// level (levels are padded to whole wavefronts, pad = -1), so a row waits
+// only on rows of earlier wavefronts.  Workgroups are dispatched in index
+// order, so the earliest unfinished row always has a slot and the sweep
+// always progresses.  A finished row publishes its values, then sets
+// ready[row] = epoch with release semantics; a dependent row spins on the
+// flag (acquire, which also invalidates this CU's L1) before reading them.
+// The epoch is read from device memory (bumped by k_epoch_bump before each
+// sweep), so ready[] never needs clearing.  A wait that exceeds kSpinMax
+// polls raises flag[3]; every waiter also watches flag[3] and gives up once
+// it is set, so a sweep always ends, after at most one bounded wait.
+// ---------------------------------------------------------------------------
+constexpr int kSpinMax = 1 << 18;
+
+__global__ void k_epoch_bump(int *epoch) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) epoch[0] = epoch[0] + 1;
+}
+
+__device__ __forceinline__ bool wait_ready(const int *ready, int j, int ep, int *flag) {
+    for (int it = 0; it < kSpinMax; it++) {
+        if (__hip_atomic_load(ready + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == ep) return true;
+        if ((it & 255) == 255 && __hip_atomic_load(flag + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    atomicOr(flag + 3, 1);
+    return false;
+}
+
+__device__ __forceinline__ void publish_ready(int *ready, int i, int ep) {
+    __hip_atomic_store(ready + i, ep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Same arithmetic as k_trsv_level; 12 rows of 5 lanes per wavefront.
+template <class TM, class TV>
+__global__ __launch_bounds__(kBlock) void k_trsv_sf(int nslots, const int *__restrict__ slots,
+                                                    const int *__restrict__ ia, const int *__restrict__ ja,
+                                                    const int *__restrict__ iau, const TM *__restrict__ lu,
+                                                    const TM *__restrict__ dinv, const TV *__restrict__ rhs, TV *z,
+                                                    int backward, int *ready, const int *epoch, int *flag) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / 5, r = lane - 5 * slot;
+    const int t = (blockIdx.x * (kBlock / 64) + wave) * 12 + slot;
+    const int i = (slot < 12 && t < nslots) ? slots[t] : -1;
+    const int ep = epoch[0];
+    TV y = TV(0.0);
+    bool ok = true;
+    if (i >= 0) {
+        const int kb = backward ? iau[i] + 1 : ia[i];
+        const int ke = backward ? ia[i + 1] : iau[i];
+        y = backward ? z[5 * (size_t)i + r] : rhs[5 * (size_t)i + r];
+        for (int kk = kb; kk < ke; kk++) {
+            const int j = ja[kk];
+            ok = wait_ready(ready, j, ep, flag) && ok;
+            const TM *A = lu + 25 * (size_t)kk + 5 * r;
+            const TV *zj = z + 5 * (size_t)j;
+            TV s = TV(0.0);
+#pragma unroll
+            for (int c = 0; c < 5; c++) s += (TV)A[c] * zj[c];
+            y = y - s;
+        }
+    }
+    if (backward) {
+        // gather the row's 5 components from its lanes, then apply D_i^-1
+        const int base = lane - r;
+        TV yy[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) yy[c] = __shfl(y, base + c, 64);
+        if (i >= 0) {
+            const TM *D = dinv + 25 * (size_t)i + 5 * r;
+            TV s = TV(0.0);
+#pragma unroll
+            for (int c = 0; c < 5; c++) s += (TV)D[c] * yy[c];
+            y = s;
+        }
+    }
+    if (i >= 0) z[5 * (size_t)i + r] = y;
+    __threadfence();
+    if (i >= 0 && r == 0) publish_ready(ready, i, ep);
+    if (!ok) atomicOr(flag + 3, 1);
+}
+
+// Sync-free block ILU(0) factorization, same arithmetic as k_ilu0_level; one
+// row per thread, wavefronts padded to 64 rows of one level.
+template <class TM>
+__global__ __launch_bounds__(64) void k_ilu0_sf(int nslots, const int *__restrict__ slots, const int *__restrict__ ia,
+                                                const int *__restrict__ ja, const int *__restrict__ iau, TM *lu,
+                                                TM *dinv, int *flag, int *ready, const int *epoch) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const int i = t < nslots ? slots[t] : -1;
+    if (i < 0) return;
+    const int ep = epoch[0];
+    bool ok = true;
+    TM L[25], U[25], P[25];
+    for (int kk = ia[i]; kk < iau[i]; kk++) {
+        const int k = ja[kk];
+        ok = wait_ready(ready, k, ep, flag) && ok;
+        TM *Lk = lu + 25 * (size_t)kk;
+        for (int q = 0; q < 25; q++) U[q] = Lk[q];
+        bmul(U, dinv + 25 * (size_t)k, L);
+        for (int q = 0; q < 25; q++) Lk[q] = L[q];
+        int pp = iau[k] + 1, qq = kk + 1;
+        const int pe = ia[k + 1], qe = ia[i + 1];
+        while (pp < pe && qq < qe) {
+            const int cp = ja[pp], cq = ja[qq];
+            if (cp == cq) {
+                const TM *Up = lu + 25 * (size_t)pp;
+                for (int q = 0; q < 25; q++) U[q] = Up[q];
+                bmul(L, U, P);
+                TM *T = lu + 25 * (size_t)qq;
+                for (int q = 0; q < 25; q++) T[q] = T[q] - P[q];
+                pp++;
+                qq++;
+            } else if (cp < cq) {
+                pp++;
+            } else {
+                qq++;
+            }
+        }
+    }
+    double Dg[25], Di[25];
+    const TM *Dp = lu + 25 * (size_t)iau[i];
+    for (int q = 0; q < 25; q++) Dg[q] = (double)Dp[q];
+    if (!inv5(Dg, Di)) {
+        atomicAdd(flag + 2, 1);
+        for (int q = 0; q < 25; q++) Di[q] = 0.0;
+    }
+    for (int q = 0; q < 25; q++) dinv[25 * (size_t)i + q] = (TM)Di[q];
+    __threadfence();
+    publish_ready(ready, i, ep);
+    if (!ok) atomicOr(flag + 3, 1);
 }
 
 // z = D^-1 r (block-Jacobi apply)
@@ -613,12 +748,14 @@ int grid_cap(size_t n, int cap) {
 int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed) {
     const int nb = xcd_grid((m->N + 127) / 128);
     if (mixed) {
-        JacArgs<float> a{m->N, m->NT, m->order, m->inc_ptr, m->inc, m->inc_off, m->iau, m->d.sa, m->d.qa, m->d.vol,
-                         m->dt, m->d.res, m->af, m->b, p->gamma, p->entropy_fix, rhs_mode};
+        JacArgs<float> a{m->N, m->NT, m->order, m->inc_ptr, m->sptr, m->nbr, m->soff, m->iau, m->d.isx, m->d.isy,
+                         m->d.isz, m->d.qa, m->d.vol, m->dt, m->d.res, m->af, m->b, p->gamma, p->entropy_fix,
+                         rhs_mode};
         hipLaunchKernelGGL(k_jacobian<float>, dim3(nb), dim3(128), 0, m->stream, a);
     } else {
-        JacArgs<double> a{m->N, m->NT, m->order, m->inc_ptr, m->inc, m->inc_off, m->iau, m->d.sa, m->d.qa, m->d.vol,
-                          m->dt, m->d.res, m->a, m->b, p->gamma, p->entropy_fix, rhs_mode};
+        JacArgs<double> a{m->N, m->NT, m->order, m->inc_ptr, m->sptr, m->nbr, m->soff, m->iau, m->d.isx, m->d.isy,
+                          m->d.isz, m->d.qa, m->d.vol, m->dt, m->d.res, m->a, m->b, p->gamma, p->entropy_fix,
+                          rhs_mode};
         hipLaunchKernelGGL(k_jacobian<double>, dim3(nb), dim3(128), 0, m->stream, a);
     }
     CFD_HIP(m, hipGetLastError());
@@ -632,7 +769,12 @@ namespace {
 
 template <class TM>
 int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
-    if (kind == CFD_PREC_ILU0) {
+    if (kind == CFD_PREC_ILU0 && m->sync_free) {
+        CFD_HIP(m, hipMemcpyAsync(lu, a, sizeof(TM) * 25 * (size_t)m->nnz, hipMemcpyDeviceToDevice, m->stream));
+        hipLaunchKernelGGL(k_epoch_bump, dim3(1), dim3(64), 0, m->stream, m->epoch);
+        hipLaunchKernelGGL(k_ilu0_sf<TM>, dim3((m->lower.n64 + 63) / 64), dim3(64), 0, m->stream, m->lower.n64,
+                           m->lower.slots64, m->ia, m->ja, m->iau, lu, dinv, m->flag, m->ready, m->epoch);
+    } else if (kind == CFD_PREC_ILU0) {
         CFD_HIP(m, hipMemcpyAsync(lu, a, sizeof(TM) * 25 * (size_t)m->nnz, hipMemcpyDeviceToDevice, m->stream));
         for (int l = 0; l < m->lower.nlevels; l++) {
             const int b0 = m->lower.ptr[l], nr = m->lower.ptr[l + 1] - b0;
@@ -650,7 +792,14 @@ int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
 template <class TM, class TV>
 void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r, TV *z) {
     const size_t L = 5 * (size_t)m->N;
-    if (kind == CFD_PREC_ILU0) {
+    if (kind == CFD_PREC_ILU0 && m->sync_free) {
+        for (int pass = 0; pass < 2; pass++) {
+            const LevelSchedule &ls = pass ? m->upper : m->lower;
+            hipLaunchKernelGGL(k_epoch_bump, dim3(1), dim3(64), 0, m->stream, m->epoch);
+            hipLaunchKernelGGL((k_trsv_sf<TM, TV>), dim3((ls.n12 + 47) / 48), dim3(kBlock), 0, m->stream, ls.n12,
+                               ls.slots12, m->ia, m->ja, m->iau, lu, dinv, r, z, pass, m->ready, m->epoch, m->flag);
+        }
+    } else if (kind == CFD_PREC_ILU0) {
         for (int pass = 0; pass < 2; pass++) {
             const LevelSchedule &ls = pass ? m->upper : m->lower;
             for (int l = 0; l < ls.nlevels; l++) {
@@ -883,5 +1032,6 @@ int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st
     CFD_HIP(m, hipMemcpyAsync(m->hflag, m->flag, 4 * sizeof(int), hipMemcpyDeviceToHost, m->stream));
     CFD_HIP(m, hipStreamSynchronize(m->stream));
     if (m->hflag[2]) return cfd_set_error(m->ctx, CFD_E_SINGULAR, "zero pivot in %d diagonal blocks", m->hflag[2]);
+    if (m->hflag[3]) return cfd_set_error(m->ctx, CFD_E_HIP, "sync-free triangular sweep exceeded its wait bound");
     return CFD_OK;
 }

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_bc(BcArgs<T> a) {
         QL[k] = a.qa[kQA * nl + k];
         QR[k] = a.qa[kQA * nr + k];
     }
-    const T *s = a.sa + 4 * ((size_t)a.E + e);
+    const T *s = a.sa + 4 * (size_t)e;
     T nx, ny, nz;
     area_normal(s[0], s[1], s[2], nx, ny, nz);
     const T g = a.g;
@@ -226,10 +226,12 @@ __global__ __launch_bounds__(kBlock) void k_lsq_grad(int N, const int *__restric
 
 // Median-dual edge-based Green-Gauss:
 //   grad_i = (1/V_i) [ sum_edges +-1/2 (Q_L+Q_R) S_LR + sum_bnd Q_i S_b ]
-// accumulated in edge order (the reference has no 3D Green-Gauss).
+// accumulated in edge order (the reference has no 3D Green-Gauss).  One
+// thread per node walks the node's SELL incidence list.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict__ order, const int *__restrict__ ip,
-                                                    const int2 *__restrict__ inc, const T *__restrict__ sa,
+__global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict__ ip, const int *__restrict__ sptr,
+                                                    const int *__restrict__ nbr, const T *__restrict__ isx,
+                                                    const T *__restrict__ isy, const T *__restrict__ isz,
                                                     const T *__restrict__ vol, const T *__restrict__ qa,
                                                     T *__restrict__ qb) {
     const int i = xcd_block() * kBlock + threadIdx.x;
@@ -241,12 +243,13 @@ __global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict
     T gs[15];
 #pragma unroll
     for (int f = 0; f < 15; f++) gs[f] = T(0.0);
-    for (int k = ip[i]; k < ip[i + 1]; k++) {
-        const int2 en = inc[k];
-        const bool iL = en.y >= 0;
-        const T *s = sa + 4 * (size_t)(iL ? en.y : ~en.y);
-        const T s0 = s[0], s1 = s[1], s2 = s[2];
-        if (en.x >= N) {
+    const int deg = ip[i + 1] - ip[i];
+    size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
+#pragma unroll 2
+    for (int k = 0; k < deg; k++, idx += kSlice) {
+        const int jr = ld_stream(nbr + idx);
+        const T s0 = ld_stream(isx + idx), s1 = ld_stream(isy + idx), s2 = ld_stream(isz + idx);
+        if (jr >= N) {
 #pragma unroll
             for (int f = 0; f < 5; f++) {
                 gs[3 * f + 0] += qi[f] * s0;
@@ -254,7 +257,8 @@ __global__ __launch_bounds__(kBlock) void k_gg_grad(int N, const int *__restrict
                 gs[3 * f + 2] += qi[f] * s2;
             }
         } else {
-            const T *rj = qa + kQA * (size_t)en.x;
+            const bool iL = jr >= 0;
+            const T *rj = qa + kQA * (size_t)(iL ? jr : ~jr);
 #pragma unroll
             for (int f = 0; f < 5; f++) {
                 const T qf = T(0.5) * (qi[f] + rj[f]);
@@ -296,40 +300,25 @@ __device__ __forceinline__ bool near_vacuum(const T *Q, T P) {
     return !(Q[0] > T(0.0)) || !(P > vacuum_margin<T>() * fabs(Q[4]));
 }
 
+// Limiter value of every variable of node c from its state qc, coordinates,
+// gradient gr and the neighbour extrema, followed by the pressure correction
+// (Compute_Limiter_Venkatakrishnan / _BarthJespersen and
+// Compute_Limiter_PressureCorrection, Limiters.cpp:130-304).  Returns whether
+// a final face state of the node is near vacuum (the only case in which the
+// Roe pressure correction can act).  Neither pass depends on the order of the
+// neighbours.
 template <class T, int KIND>  // 1 Venkatakrishnan, 2 Barth-Jespersen
-__global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict__ order, const int *__restrict__ ia,
-                                                    const int *__restrict__ ja, const T *__restrict__ qa,
-                                                    T *__restrict__ qb, const T *__restrict__ vol, T g, T frac, T k3,
-                                                    int *risk) {
-    const int c = xcd_block() * kBlock + threadIdx.x;
-    if (c >= N) return;
-    const T *ra = qa + kQA * (size_t)c;
-    T qc[5], xc = ra[5], yc = ra[6], zc = ra[7];
+__device__ __forceinline__ bool limiter_passes(int N, int deg, size_t base, const int *__restrict__ nbr,
+                                               const T *__restrict__ qa, const T *qc, T xc, T yc, T zc, const T *gr,
+                                               const T *qmin, const T *qmax, T v, T g, T frac, T k3, T *ph) {
 #pragma unroll
-    for (int k = 0; k < 5; k++) qc[k] = ra[k];
-    T *rb = qb + kQB * (size_t)c;
-    T gr[15];
-#pragma unroll
-    for (int f = 0; f < 15; f++) gr[f] = rb[f];
-    T ph[5], qmin[5], qmax[5];
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-        ph[k] = T(1.0);
-        qmin[k] = qmax[k] = qc[k];
-    }
-    const int b = ia[c], e = ia[c + 1];
-    for (int n = b; n < e; n++) {
-        const T *rj = qa + kQA * (size_t)ja[n];
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            qmin[k] = tmin(qmin[k], rj[k]);
-            qmax[k] = tmax(qmax[k], rj[k]);
-        }
-    }
+    for (int k = 0; k < 5; k++) ph[k] = T(1.0);
     T eps2 = k3;
-    eps2 = T(6.0) * eps2 * vol[c] / T(M_PI);
-    for (int n = b; n < e; n++) {
-        const T *rj = qa + kQA * (size_t)ja[n];
+    eps2 = T(6.0) * eps2 * v / T(M_PI);
+    for (int n = 0; n < deg; n++) {
+        const int jr = nbr[base + (size_t)n * kSlice];
+        if (jr >= N) continue;
+        const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
         const T Rx = T(0.5) * (rj[5] - xc), Ry = T(0.5) * (rj[6] - yc), Rz = T(0.5) * (rj[7] - zc);
 #pragma unroll
         for (int k = 0; k < 5; k++) {
@@ -360,12 +349,12 @@ __global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict
             ph[k] = tmax(T(0.0), ph[k]);
         }
     }
-    // Compute_Limiter_PressureCorrection, Limiters.cpp:273-304.  Also notes
-    // whether a final face state is near vacuum (the only case in which the
-    // Roe pressure correction can act).
+    // Compute_Limiter_PressureCorrection, Limiters.cpp:273-304
     bool zeroed = false, risky = false;
-    for (int n = b; n < e; n++) {
-        const T *rj = qa + kQA * (size_t)ja[n];
+    for (int n = 0; n < deg; n++) {
+        const int jr = nbr[base + (size_t)n * kSlice];
+        if (jr >= N) continue;
+        const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
         const T Rx = T(0.5) * (rj[5] - xc), Ry = T(0.5) * (rj[6] - yc), Rz = T(0.5) * (rj[7] - zc);
         T Q[5];
 #pragma unroll
@@ -384,6 +373,114 @@ __global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict
         const T P = (g - T(1.0)) * (qc[4] - (T(0.5) / qc[0]) * (qc[1] * qc[1] + qc[2] * qc[2] + qc[3] * qc[3]));
         risky = near_vacuum(qc, P);
     }
+    return risky;
+}
+
+template <class T, int KIND>
+__global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict__ ip, const int *__restrict__ sptr,
+                                                    const int *__restrict__ nbr, const T *__restrict__ qa,
+                                                    T *__restrict__ qb, const T *__restrict__ vol, T g, T frac, T k3,
+                                                    int *risk) {
+    const int c = xcd_block() * kBlock + threadIdx.x;
+    if (c >= N) return;
+    const T *ra = qa + kQA * (size_t)c;
+    T qc[5], xc = ra[5], yc = ra[6], zc = ra[7];
+#pragma unroll
+    for (int k = 0; k < 5; k++) qc[k] = ra[k];
+    T *rb = qb + kQB * (size_t)c;
+    T gr[15];
+#pragma unroll
+    for (int f = 0; f < 15; f++) gr[f] = rb[f];
+    T qmin[5], qmax[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) qmin[k] = qmax[k] = qc[k];
+    // The neighbours are the interior entries of the node's incidence list;
+    // min/max and the limiter minimum do not depend on their order.
+    const int deg = ip[c + 1] - ip[c];
+    const size_t base = (size_t)sptr[c / kSlice] + (c % kSlice);
+    for (int n = 0; n < deg; n++) {
+        const int jr = nbr[base + (size_t)n * kSlice];
+        if (jr >= N) continue;
+        const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            qmin[k] = tmin(qmin[k], rj[k]);
+            qmax[k] = tmax(qmax[k], rj[k]);
+        }
+    }
+    T ph[5];
+    const bool risky = limiter_passes<T, KIND>(N, deg, base, nbr, qa, qc, xc, yc, zc, gr, qmin, qmax, vol[c], g, frac,
+                                               k3, ph);
+    if (risky) atomicAdd(risk, 1);
+#pragma unroll
+    for (int k = 0; k < 5; k++) rb[kPHI + k] = ph[k];
+}
+
+// Green-Gauss gradient and limiter of one node in one pass over its
+// incidences: the edge-ordered Green-Gauss sums and the neighbour extrema are
+// gathered together, the gradient stays in registers for the limiter passes.
+// Bit-identical to k_gg_grad followed by k_limiter.
+template <class T, int KIND>
+__global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restrict__ ip, const int *__restrict__ sptr,
+                                                       const int *__restrict__ nbr, const T *__restrict__ isx,
+                                                       const T *__restrict__ isy, const T *__restrict__ isz,
+                                                       const T *__restrict__ vol, const T *__restrict__ qa,
+                                                       T *__restrict__ qb, T g, T frac, T k3, int *risk) {
+    const int c = xcd_block() * kBlock + threadIdx.x;
+    if (c >= N) return;
+    const T *ra = qa + kQA * (size_t)c;
+    T qc[5], xc = ra[5], yc = ra[6], zc = ra[7];
+#pragma unroll
+    for (int k = 0; k < 5; k++) qc[k] = ra[k];
+    T gs[15], qmin[5], qmax[5];
+#pragma unroll
+    for (int f = 0; f < 15; f++) gs[f] = T(0.0);
+#pragma unroll
+    for (int k = 0; k < 5; k++) qmin[k] = qmax[k] = qc[k];
+    const int deg = ip[c + 1] - ip[c];
+    const size_t base = (size_t)sptr[c / kSlice] + (c % kSlice);
+    size_t idx = base;
+    for (int k = 0; k < deg; k++, idx += kSlice) {
+        const int jr = nbr[idx];
+        const T s0 = ld_stream(isx + idx), s1 = ld_stream(isy + idx), s2 = ld_stream(isz + idx);
+        if (jr >= N) {
+#pragma unroll
+            for (int f = 0; f < 5; f++) {
+                gs[3 * f + 0] += qc[f] * s0;
+                gs[3 * f + 1] += qc[f] * s1;
+                gs[3 * f + 2] += qc[f] * s2;
+            }
+        } else {
+            const bool iL = jr >= 0;
+            const T *rj = qa + kQA * (size_t)(iL ? jr : ~jr);
+#pragma unroll
+            for (int f = 0; f < 5; f++) {
+                const T qj = rj[f];
+                qmin[f] = tmin(qmin[f], qj);
+                qmax[f] = tmax(qmax[f], qj);
+                const T qf = T(0.5) * (qc[f] + qj);
+                if (iL) {
+                    gs[3 * f + 0] += qf * s0;
+                    gs[3 * f + 1] += qf * s1;
+                    gs[3 * f + 2] += qf * s2;
+                } else {
+                    gs[3 * f + 0] -= qf * s0;
+                    gs[3 * f + 1] -= qf * s1;
+                    gs[3 * f + 2] -= qf * s2;
+                }
+            }
+        }
+    }
+    const T v = vol[c];
+    T gr[15];
+    T *rb = qb + kQB * (size_t)c;
+#pragma unroll
+    for (int f = 0; f < 15; f++) {
+        gr[f] = gs[f] / v;
+        rb[f] = gr[f];
+    }
+    T ph[5];
+    const bool risky = limiter_passes<T, KIND>(N, deg, base, nbr, qa, qc, xc, yc, zc, gr, qmin, qmax, v, g, frac, k3, ph);
     if (risky) atomicAdd(risk, 1);
 #pragma unroll
     for (int k = 0; k < 5; k++) rb[kPHI + k] = ph[k];
@@ -404,8 +501,8 @@ __device__ __forceinline__ void face_state(const T *ra, const T *rbg, const T *p
 // tested with the limiter values of the previous kernel and the marked nodes
 // are zeroed afterwards (identical whenever no such edge exists).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_roe_mark(int N, const int *__restrict__ order, const int *__restrict__ ia,
-                                                     const int *__restrict__ ja, const T *__restrict__ qa,
+__global__ __launch_bounds__(kBlock) void k_roe_mark(int N, const int *__restrict__ ip, const int *__restrict__ sptr,
+                                                     const int *__restrict__ nbr, const T *__restrict__ qa,
                                                      const T *__restrict__ qb, T g, T frac, int *mark, int epoch,
                                                      const int *risk) {
     if (*risk == 0) return;  // no near-vacuum face state: no edge can qualify
@@ -413,8 +510,12 @@ __global__ __launch_bounds__(kBlock) void k_roe_mark(int N, const int *__restric
     if (c >= N) return;
     const T *ra = qa + kQA * (size_t)c;
     const T *rb = qb + kQB * (size_t)c;
-    for (int n = ia[c]; n < ia[c + 1]; n++) {
-        const int j = ja[n];
+    const int deg = ip[c + 1] - ip[c];
+    const size_t base = (size_t)sptr[c / kSlice] + (c % kSlice);
+    for (int n = 0; n < deg; n++) {
+        const int jr = nbr[base + (size_t)n * kSlice];
+        if (jr >= N) continue;
+        const int j = jr >= 0 ? jr : ~jr;
         const T *rj = qa + kQA * (size_t)j;
         const T *rbj = qb + kQB * (size_t)j;
         T QL[5], QR[5];
@@ -454,10 +555,9 @@ __global__ __launch_bounds__(kBlock) void k_roe_zero(int N, const int *__restric
 template <class T>
 struct ResArgs {
     int N, NT;
-    const int *order;
-    const int *ip;
-    const int2 *inc;
-    const T *sa, *qa, *qb;
+    const int *ip, *sptr, *nbr;
+    const T *isx, *isy, *isz;
+    const T *qa, *qb;
     T *res;
     T g, frac;
     int efix;
@@ -489,13 +589,13 @@ __global__ __launch_bounds__(kBlock) void k_residual(ResArgs<T> a) {
     for (int k = 0; k < 5; k++) ac[k] = ad[k] = T(0.0);
     T emin[3] = {T(DBL_MAX), T(DBL_MAX), T(DBL_MAX)}, emax[3] = {T(0.0), T(0.0), T(0.0)};
     int bad = 0;
-    const int kb = a.ip[i], ke = a.ip[i + 1];
-    for (int k = kb; k < ke; k++) {
-        const int2 en = a.inc[k];
-        const int j = en.x;
-        const bool iL = en.y >= 0;  // ghosts (j >= N) always pair with a physical L
-        const T *s = a.sa + 4 * (size_t)(iL ? en.y : ~en.y);
-        const T s0 = s[0], s1 = s[1], s2 = s[2];
+    const int deg = a.ip[i + 1] - a.ip[i];
+    size_t idx = (size_t)a.sptr[i / kSlice] + (i % kSlice);
+    for (int k = 0; k < deg; k++, idx += kSlice) {
+        const int jr = ld_stream(a.nbr + idx);
+        const bool iL = jr >= 0;  // ghosts (jr >= N) always pair with a physical L
+        const int j = iL ? jr : ~jr;
+        const T s0 = ld_stream(a.isx + idx), s1 = ld_stream(a.isy + idx), s2 = ld_stream(a.isz + idx);
         const T *rj = a.qa + kQA * (size_t)j;
         T QL[5], QR[5];
         if (ORDER == 2 && j < a.N) {
@@ -567,7 +667,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(ResArgs<T> a) {
     o[10] = dt;
     o[11] = T(0.0);
     if (bad) flag_set(a.flag, 1);
-    if (STATS && kb < ke) {
+    if (STATS && deg > 0) {
         for (int t = 0; t < 3; t++) {
             atomic_min_nonneg(a.eig + 2 * t, (double)emin[t]);
             atomic_max_nonneg(a.eig + 2 * t + 1, (double)emax[t]);
@@ -645,13 +745,19 @@ __global__ __launch_bounds__(kBlock) void k_forces_partial(int N, int E, int B, 
         const double *r = qa + kQA * (size_t)bnode[e];
         Eos<double> s;
         eos_face(g, r, 0.0, 0.0, 0.0, s);
-        const double *S = sa + 4 * ((size_t)E + e);
+        const double *S = sa + 4 * (size_t)e;
         v[0] += s.p * S[0];
         v[1] += s.p * S[1];
         v[2] += s.p * S[2];
         v[3] += s.p * sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
     }
     block_reduce_store<4>(v, part + 4 * blockIdx.x);
+}
+
+// Fills n ints; replaces hipMemsetAsync so the launch sequence can be
+// captured in a graph.
+__global__ void k_fill_int(int n, int v, int *p) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
 template <class TD, class TS>
@@ -691,11 +797,24 @@ template <class T>
 int launch_gradient(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p) {
     const int nb = xcd_grid((m->N + kBlock - 1) / kBlock);
     if (p->gradient == CFD_GRAD_GG)
-        hipLaunchKernelGGL(k_gg_grad<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->inc_ptr, m->inc,
-                           b.sa, b.vol, b.qa, b.qb);
+        hipLaunchKernelGGL(k_gg_grad<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->sptr, m->nbr,
+                           b.isx, b.isy, b.isz, b.vol, b.qa, b.qb);
     else
         hipLaunchKernelGGL(k_lsq_grad<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->n2n_ia, m->n2n_ja,
                            b.lsq, b.qa, b.qb);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+
+// Roe pressure correction after the limiter (Limiters.cpp:71-110).
+template <class T>
+static int launch_roe_correction(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, int nb, T g, T frac) {
+    if (p->flux != CFD_FLUX_ROE) return CFD_OK;
+    const int epoch = ++m->mark_epoch;
+    hipLaunchKernelGGL(k_roe_mark<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->sptr, m->nbr, b.qa,
+                       b.qb, g, frac, m->mark, epoch, m->risk);
+    hipLaunchKernelGGL(k_roe_zero<T>, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, m->mark,
+                       epoch, b.qb);
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;
 }
@@ -707,23 +826,37 @@ int launch_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p) {
     const T k3 = (T)pow(p->venkat_k, 3.0);  // host pow, as the reference's glibc pow
     const T g = (T)p->gamma;
     if (p->limiter != CFD_LIM_VENKAT && p->limiter != CFD_LIM_BJ) return CFD_OK;
-    CFD_HIP(m, hipMemsetAsync(m->risk, 0, sizeof(int), m->stream));
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, m->stream, 1, 0, m->risk);
     if (p->limiter == CFD_LIM_VENKAT)
-        hipLaunchKernelGGL((k_limiter<T, 1>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->n2n_ia,
-                           m->n2n_ja, b.qa, b.qb, b.vol, g, frac, k3, m->risk);
+        hipLaunchKernelGGL((k_limiter<T, 1>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->sptr,
+                           m->nbr, b.qa, b.qb, b.vol, g, frac, k3, m->risk);
     else
-        hipLaunchKernelGGL((k_limiter<T, 2>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->n2n_ia,
-                           m->n2n_ja, b.qa, b.qb, b.vol, g, frac, k3, m->risk);
+        hipLaunchKernelGGL((k_limiter<T, 2>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->sptr,
+                           m->nbr, b.qa, b.qb, b.vol, g, frac, k3, m->risk);
     CFD_HIP(m, hipGetLastError());
-    if (p->flux == CFD_FLUX_ROE) {
-        const int epoch = ++m->mark_epoch;
-        hipLaunchKernelGGL(k_roe_mark<T>, dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->order, m->n2n_ia, m->n2n_ja,
-                           b.qa, b.qb, g, frac, m->mark, epoch, m->risk);
-        hipLaunchKernelGGL(k_roe_zero<T>, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N,
-                           m->mark, epoch, b.qb);
-        CFD_HIP(m, hipGetLastError());
+    return launch_roe_correction<T>(m, b, p, nb, g, frac);
+}
+
+// Gradient then limiter; one fused kernel for the Green-Gauss gradient.
+template <class T>
+int launch_gradient_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p) {
+    if (p->gradient != CFD_GRAD_GG || (p->limiter != CFD_LIM_VENKAT && p->limiter != CFD_LIM_BJ)) {
+        int rc = launch_gradient<T>(m, b, p);
+        return rc ? rc : launch_limiter<T>(m, b, p);
     }
-    return CFD_OK;
+    const int nb = xcd_grid((m->N + kBlock - 1) / kBlock);
+    const T frac = (p->limiter_order == 2) ? T(0.5) : T(0.0);
+    const T k3 = (T)pow(p->venkat_k, 3.0);
+    const T g = (T)p->gamma;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, m->stream, 1, 0, m->risk);
+    if (p->limiter == CFD_LIM_VENKAT)
+        hipLaunchKernelGGL((k_gg_limiter<T, 1>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->sptr,
+                           m->nbr, b.isx, b.isy, b.isz, b.vol, b.qa, b.qb, g, frac, k3, m->risk);
+    else
+        hipLaunchKernelGGL((k_gg_limiter<T, 2>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->sptr,
+                           m->nbr, b.isx, b.isy, b.isz, b.vol, b.qa, b.qb, g, frac, k3, m->risk);
+    CFD_HIP(m, hipGetLastError());
+    return launch_roe_correction<T>(m, b, p, nb, g, frac);
 }
 
 template <class T, int FLUX>
@@ -745,7 +878,7 @@ static void launch_res_flux(cfd_mesh *m, const ResArgs<T> &a, int order, bool li
 
 template <class T>
 int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats) {
-    ResArgs<T> a{m->N, m->NT, m->order, m->inc_ptr, m->inc, b.sa, b.qa, b.qb, b.res, (T)p->gamma,
+    ResArgs<T> a{m->N, m->NT, m->inc_ptr, m->sptr, m->nbr, b.isx, b.isy, b.isz, b.qa, b.qb, b.res, (T)p->gamma,
                  (p->limiter_order == 2) ? T(0.5) : T(0.0), p->entropy_fix, m->eigbits, m->flag};
     const int nb = xcd_grid((m->N + kBlock - 1) / kBlock);
     const bool lim = p->limiter != CFD_LIM_NONE;
@@ -812,9 +945,15 @@ int launch_residual_to_fp64(cfd_mesh *m) {
 
 int launch_convert_geometry_fp32(cfd_mesh *m) {
     hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for((size_t)4 * (m->E + m->B))), dim3(kBlock), 0,
-                       m->stream, (size_t)4 * (m->E + m->B), m->d.sa, m->f.sa);
+                       m->stream, (size_t)4 * m->B, m->d.sa, m->f.sa);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for((size_t)m->N)), dim3(kBlock), 0, m->stream,
                        (size_t)m->N, m->d.vol, m->f.vol);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(m->nsell)), dim3(kBlock), 0, m->stream, m->nsell,
+                       m->d.isx, m->f.isx);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(m->nsell)), dim3(kBlock), 0, m->stream, m->nsell,
+                       m->d.isy, m->f.isy);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(m->nsell)), dim3(kBlock), 0, m->stream, m->nsell,
+                       m->d.isz, m->f.isz);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for((size_t)8 * m->N)), dim3(kBlock), 0, m->stream,
                        (size_t)8 * m->N, m->d.lsq, m->f.lsq);
     CFD_HIP(m, hipGetLastError());
@@ -827,5 +966,7 @@ template int launch_gradient<double>(cfd_mesh *, ResidualBufs<double> &, const c
 template int launch_gradient<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *);
 template int launch_limiter<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *);
 template int launch_limiter<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *);
+template int launch_gradient_limiter<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *);
+template int launch_gradient_limiter<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *);
 template int launch_residual<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *, bool);
 template int launch_residual<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *, bool);

@@ -1,0 +1,257 @@
+// serve.hip - continuous-batching steady-solve server over the C ABI.
+//
+// Requests queue in arrival order.  Worker threads (max_inflight of them)
+// each take the next request that fits the HBM budget, run it to convergence
+// on its own mesh handle and HIP stream, and post a completion; a request
+// that arrives while others are on the GPU is admitted as soon as a worker
+// and memory free up, so solves of different sizes overlap on the device.
+// Device meshes (state, CRS structure, Jacobian and Krylov workspace) stay
+// resident after a solve and are reused by later requests with the same mesh
+// key; idle meshes are evicted least-recently-used when memory is needed.
+//
+// Every kernel on the solve path is deterministic (no atomics in sums,
+// fixed-shape reductions), so a request's result does not depend on what
+// else runs concurrently.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "cfd_internal.h"
+
+namespace {
+using Clock = std::chrono::steady_clock;
+
+struct Pending {
+    cfd_request req;
+    double t_submit;
+    int64_t need;   // bytes to reserve if no resident mesh can be reused
+    int bypassed;   // later requests admitted ahead of this one
+};
+
+struct Resident {
+    int64_t key;
+    cfd_mesh *mesh;
+    int64_t bytes;  // bytes reserved against the budget
+    int32_t krylov_m, mixed;
+    bool busy;
+    uint64_t last_use;
+};
+
+int krylov_of(const cfd_solve_opts &o) { return o.lin.solver == CFD_SOLVER_FGMRES ? o.lin.krylov_m : 0; }
+}  // namespace
+
+struct cfd_server {
+    cfd_ctx *ctx = nullptr;
+    cfd_server_opts opts{};
+    Clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Pending> queue;
+    std::deque<cfd_completion> done;
+    std::list<Resident> meshes;
+    int64_t in_use = 0, peak = 0;
+    int running = 0;
+    uint64_t tick = 0;
+    int64_t admitted = 0, reused = 0, evicted = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+// Frees idle resident meshes, least recently used first, until `need` more
+// bytes fit the budget.  Called with the server lock held.
+bool make_room(cfd_server *s, int64_t need) {
+    while (s->in_use + need > s->opts.hbm_budget) {
+        auto victim = s->meshes.end();
+        for (auto it = s->meshes.begin(); it != s->meshes.end(); ++it)
+            if (!it->busy && (victim == s->meshes.end() || it->last_use < victim->last_use)) victim = it;
+        if (victim == s->meshes.end()) return false;
+        cfd_mesh_destroy(victim->mesh);
+        s->in_use -= victim->bytes;
+        s->meshes.erase(victim);
+        s->evicted++;
+    }
+    return true;
+}
+
+// Picks the first request of the look-ahead window that can start now and
+// reserves its memory.  A request that has been bypassed max_bypass times
+// blocks the window until it is admitted.  Called with the server lock held.
+bool admit(cfd_server *s, Pending &job, Resident *&slot) {
+    const size_t look = std::min<size_t>(s->queue.size(), (size_t)std::max(1, s->opts.lookahead));
+    for (size_t q = 0; q < look; q++) {
+        Pending &p = s->queue[q];
+        const int km = krylov_of(p.req.opts);
+        Resident *r = nullptr;
+        for (auto &cm : s->meshes)
+            if (!cm.busy && cm.key == p.req.mesh_key && cm.mixed == p.req.opts.lin.mixed && cm.krylov_m >= km) {
+                r = &cm;
+                break;
+            }
+        if (r) {
+            s->reused++;
+        } else if (make_room(s, p.need)) {
+            s->meshes.push_back(Resident{p.req.mesh_key, nullptr, p.need, km, p.req.opts.lin.mixed, false, 0});
+            r = &s->meshes.back();
+            s->in_use += p.need;
+            s->peak = std::max(s->peak, s->in_use);
+        }
+        if (r) {
+            r->busy = true;
+            r->last_use = ++s->tick;
+            for (size_t k = 0; k < q; k++) s->queue[k].bypassed++;
+            job = p;
+            slot = r;
+            s->queue.erase(s->queue.begin() + (long)q);
+            s->admitted++;
+            return true;
+        }
+        if (p.bypassed >= s->opts.max_bypass) break;
+    }
+    return false;
+}
+
+cfd_completion run_one(cfd_server *s, const Pending &p, Resident *slot) {
+    cfd_completion c{};
+    c.id = p.req.id;
+    c.t_submit = p.t_submit;
+    c.t_admit = s->now();
+    c.footprint = slot->bytes;
+    int rc;
+    if (!slot->mesh) {
+        rc = cfd_mesh_create(s->ctx, p.req.mesh, &p.req.params, &slot->mesh);
+    } else {
+        c.reused = 1;
+        rc = cfd_mesh_reset(slot->mesh, &p.req.params);
+    }
+    if (!rc) rc = cfd_solve_steady(slot->mesh, &p.req.params, &p.req.opts, &c.result, p.req.rms_hist);
+    if (!rc && p.req.q_out) rc = cfd_field_get(slot->mesh, CFD_FIELD_Q, p.req.q_out);
+    c.status = rc;
+    if (rc) snprintf(c.message, sizeof(c.message), "%s", cfd_last_error(s->ctx));
+    c.t_done = s->now();
+    return c;
+}
+
+void worker(cfd_server *s) {
+    hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        Pending job{};
+        Resident *slot = nullptr;
+        bool got = false;
+        while (!s->stop && !(got = admit(s, job, slot))) s->cv_work.wait(lk);
+        if (!got) break;
+        s->running++;
+        lk.unlock();
+        cfd_completion c = run_one(s, job, slot);
+        const int64_t actual = slot->mesh ? cfd_mesh_device_bytes(slot->mesh) : 0;
+        lk.lock();
+        s->running--;
+        if (c.status != CFD_OK || !s->opts.keep_resident) {
+            if (slot->mesh) cfd_mesh_destroy(slot->mesh);
+            s->in_use -= slot->bytes;
+            for (auto it = s->meshes.begin(); it != s->meshes.end(); ++it)
+                if (&*it == slot) {
+                    s->meshes.erase(it);
+                    break;
+                }
+        } else {
+            if (actual > slot->bytes) {  // estimate exceeded: account what is really held
+                s->in_use += actual - slot->bytes;
+                slot->bytes = actual;
+                s->peak = std::max(s->peak, s->in_use);
+            }
+            c.footprint = slot->bytes;
+            slot->busy = false;
+            slot->last_use = ++s->tick;
+        }
+        c.device_bytes = actual;
+        s->done.push_back(c);
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" int cfd_server_create(cfd_ctx *ctx, const cfd_server_opts *o, cfd_server **out) {
+    if (!ctx || !o || !out || o->max_inflight < 1 || o->max_inflight > 64 || o->hbm_budget <= 0)
+        return cfd_set_error(ctx, CFD_E_ARG, "cfd_server_create: invalid options");
+    cfd_server *s = new (std::nothrow) cfd_server();
+    if (!s) return cfd_set_error(ctx, CFD_E_OOM, "cfd_server_create: out of host memory");
+    s->ctx = ctx;
+    s->opts = *o;
+    if (s->opts.lookahead < 1) s->opts.lookahead = 16;
+    if (s->opts.max_bypass < 1) s->opts.max_bypass = 8;
+    s->t0 = Clock::now();
+    for (int w = 0; w < o->max_inflight; w++) s->workers.emplace_back(worker, s);
+    *out = s;
+    return CFD_OK;
+}
+
+extern "C" int cfd_server_submit(cfd_server *s, const cfd_request *r) {
+    if (!s || !r || !r->mesh) return cfd_set_error(s ? s->ctx : nullptr, CFD_E_ARG, "cfd_server_submit: null argument");
+    const cfd_mesh_desc *d = r->mesh;
+    const int64_t need = cfd_footprint_bytes(d->n_node, d->n_edge, d->n_bedge, krylov_of(r->opts), r->opts.lin.mixed);
+    if (need > s->opts.hbm_budget)
+        return cfd_set_error(s->ctx, CFD_E_OOM, "request %lld needs %lld bytes, budget is %lld", (long long)r->id,
+                             (long long)need, (long long)s->opts.hbm_budget);
+    std::lock_guard<std::mutex> g(s->mu);
+    s->queue.push_back(Pending{*r, s->now(), need, 0});
+    s->cv_work.notify_one();
+    return CFD_OK;
+}
+
+extern "C" int cfd_server_poll(cfd_server *s, cfd_completion *out, int32_t max, double wait_seconds) {
+    if (!s || !out || max < 1) return CFD_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_seconds > 0)
+        s->cv_done.wait_for(lk, std::chrono::duration<double>(wait_seconds), [s] { return !s->done.empty(); });
+    int n = 0;
+    while (n < max && !s->done.empty()) {
+        out[n++] = s->done.front();
+        s->done.pop_front();
+    }
+    return n;
+}
+
+extern "C" int cfd_server_get_stats(cfd_server *s, cfd_server_stats *st) {
+    if (!s || !st) return CFD_E_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    st->bytes_in_use = s->in_use;
+    st->peak_bytes = s->peak;
+    st->queued = (int32_t)s->queue.size();
+    st->running = s->running;
+    st->resident = (int32_t)s->meshes.size();
+    st->pad_ = 0;
+    st->admitted = s->admitted;
+    st->reused = s->reused;
+    st->evicted = s->evicted;
+    st->clock = s->now();
+    return CFD_OK;
+}
+
+extern "C" void cfd_server_destroy(cfd_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+        s->queue.clear();
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    hipSetDevice(s->ctx->device);
+    for (auto &cm : s->meshes)
+        if (cm.mesh) cfd_mesh_destroy(cm.mesh);
+    delete s;
+}

@@ -20,7 +20,8 @@
  *   - Every call is ordered on the handle's own HIP stream; calls on one
  *     handle are not thread-safe, different handles are independent.
  *   - Return value: CFD_OK or a negative CFD_E_* code; cfd_last_error()
- *     describes the last failure on the context.
+ *     describes the last failure seen by the calling thread (else the last
+ *     failure on the context).
  */
 #ifndef CFD_HIP_H
 #define CFD_HIP_H
@@ -189,6 +190,11 @@ int cfd_bc(cfd_mesh *mesh, const cfd_params *p, int iter);
 int cfd_gradient(cfd_mesh *mesh, const cfd_params *p);
 /* Compute_Limiter (Euler3D/src/Limiters.cpp:20-110). */
 int cfd_limiter(cfd_mesh *mesh, const cfd_params *p);
+/* cfd_gradient followed by cfd_limiter, as the steady loop runs them before
+ * the residual (Euler3D/src/Solver_Steady_Implicit.cpp:94-99); with the
+ * Green-Gauss gradient both run in one pass over the node incidences.  Same
+ * results as the two calls. */
+int cfd_gradient_limiter(cfd_mesh *mesh, const cfd_params *p);
 /* Compute_Residual(AddTime) (Euler3D/src/Residual.cpp:18-49): convective and
  * dissipative residual and the time-step accumulator, left on the device.
  * eig6 (host, may be NULL) receives the eigenvalue extrema of the Roe flux. */
@@ -224,6 +230,67 @@ int cfd_forces(cfd_mesh *mesh, const cfd_params *p, int wall_tag, double alpha_d
  * max_iter entries) may be NULL. */
 int cfd_solve_steady(cfd_mesh *mesh, const cfd_params *p, const cfd_solve_opts *o, cfd_solve_result *res,
                      double *rms_hist);
+
+/* Resets the state of a resident mesh to the freestream of p (the initial
+ * condition of Solver_Set_Initial_Conditions, Euler3D/src/Solver.cpp:340-362)
+ * and invalidates its Jacobian and preconditioner, so the mesh can serve
+ * another solve without rebuilding its structure. */
+int cfd_mesh_reset(cfd_mesh *mesh, const cfd_params *p);
+
+/* ---- serving -------------------------------------------------------------
+ * No counterpart in the reference, which solves one case per process.  A
+ * server owns worker threads that run queued steady solves concurrently on
+ * one GPU: requests are admitted in arrival order (with bounded look-ahead)
+ * when a worker is free and their HBM footprint fits the budget; device
+ * meshes stay resident after a solve and serve later requests with the same
+ * mesh_key.  Results do not depend on concurrency (deterministic kernels). */
+typedef struct cfd_server cfd_server;
+
+typedef struct cfd_server_opts {
+    int32_t max_inflight;   /* concurrent solves (worker threads, one stream each), 1..64 */
+    int32_t lookahead;      /* queue entries considered for admission (<= 0: 16) */
+    int32_t max_bypass;     /* admissions ahead of a request before it blocks the window (<= 0: 8) */
+    int32_t keep_resident;  /* 1: keep device meshes for reuse by requests with the same mesh_key */
+    int64_t hbm_budget;     /* device bytes the server may hold (meshes, workspaces) */
+} cfd_server_opts;
+
+typedef struct cfd_request {
+    int64_t id;
+    int64_t mesh_key;           /* requests with equal keys must pass the same mesh */
+    const cfd_mesh_desc *mesh;  /* host arrays, valid until the completion is polled */
+    cfd_params params;
+    cfd_solve_opts opts;
+    double *q_out;              /* host [5][N+B] final state, or NULL */
+    double *rms_hist;           /* host [opts.max_iter] residual history, or NULL */
+} cfd_request;
+
+typedef struct cfd_completion {
+    int64_t id;
+    int32_t status;             /* CFD_OK or CFD_E_* */
+    int32_t reused;             /* 1 if a resident mesh was reused */
+    cfd_solve_result result;
+    double t_submit, t_admit, t_done;  /* seconds on the server clock */
+    int64_t footprint;          /* bytes reserved for the request's mesh */
+    int64_t device_bytes;       /* bytes the mesh actually held */
+    char message[160];          /* error text when status != CFD_OK */
+} cfd_completion;
+
+typedef struct cfd_server_stats {
+    int64_t bytes_in_use, peak_bytes;
+    int32_t queued, running, resident, pad_;
+    int64_t admitted, reused, evicted;
+    double clock;
+} cfd_server_stats;
+
+int cfd_server_create(cfd_ctx *ctx, const cfd_server_opts *opts, cfd_server **out);
+/* Queues a request; returns CFD_E_OOM at once if it can never fit the budget. */
+int cfd_server_submit(cfd_server *server, const cfd_request *req);
+/* Moves up to max finished requests into out, waiting up to wait_seconds for
+ * the first; returns how many were written. */
+int cfd_server_poll(cfd_server *server, cfd_completion *out, int32_t max, double wait_seconds);
+int cfd_server_get_stats(cfd_server *server, cfd_server_stats *st);
+/* Drops queued requests, waits for running ones, frees every resident mesh. */
+void cfd_server_destroy(cfd_server *server);
 
 #ifdef __cplusplus
 }
