@@ -135,6 +135,7 @@ extern "C" void cfd_mesh_destroy(cfd_mesh *m) {
     if (!m) return;
     hipSetDevice(m->ctx->device);
     if (m->stream) hipStreamSynchronize(m->stream);
+    drop_krylov_graphs(m);
     for (void *p : m->allocs) hipFree(p);
     if (m->hred) hipHostFree(m->hred);
     if (m->hflag) hipHostFree(m->hflag);
@@ -401,6 +402,8 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
             if (!rc) m->epoch = upload(m, zeros.data(), 1, &rc);
             const char *sf = getenv("CFD_SYNC_FREE");
             m->sync_free = !(sf && sf[0] == '0');
+            const char *gr = getenv("CFD_GRAPHS");
+            m->use_graphs = !(gr && gr[0] == '0');
         }
         if (rc) {
             cfd_mesh_destroy(m);
@@ -448,17 +451,27 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
 
 extern "C" int64_t cfd_mesh_device_bytes(const cfd_mesh *m) { return m ? m->bytes : 0; }
 
+// Device bytes of a mesh handle after a steady solve with FGMRES(km): every
+// allocation of cfd_mesh_create plus the Jacobian, ILU(0) and Krylov
+// workspace.  Layout padding is bounded: SELL slices pad each slice to its
+// widest node (taken as <= 25% of the incidences plus one slice), levels pad
+// to whole wavefronts (counted for up to 4096 levels).
 extern "C" int64_t cfd_footprint_bytes(int32_t N, int32_t E, int32_t B, int32_t km, int32_t mixed) {
-    const int64_t n = N, e = E, b = B, nnz = n + 2 * e;
+    const int64_t n = N, e = E, b = B, nnz = n + 2 * e, inc = 2 * e + b;
     const int64_t s = mixed ? 4 : 8;
+    const int64_t nsell = inc + inc / 4 + (int64_t)kSlice * 32;
+    const int64_t lvl = std::min<int64_t>(n, 4096);
     int64_t bytes = 0;
-    bytes += 8 * (kQA * (n + b) + kQB * n + 4 * b + n + 8 * n + 12 * n + 5 * n);  // state, geometry, residuals
-    bytes += 4 * (n + 1) + 8 * e + 4 * (n + 1 + nnz + n) + 12 * b + 4 * n;       // topology, BSR pattern
-    bytes += 4 * 7 * n + 4 * 4 * n + 8 * n;                                      // level schedules, flags
-    bytes += (32 * (2 * e + b) * 5) / 4 + 4 * (n / kSlice + 2);  // SELL incidences (<= 25% slice padding)
-    bytes += 25 * s * nnz * 2 + 25 * s * n;        // Jacobian, ILU(0) factors, inverse pivots
-    if (mixed) bytes += 8 * 25 * 0;                // FP64 values are not kept in mixed mode
-    bytes += s * 5 * n * (2 * (int64_t)km + 2) + 8 * 5 * n * 3;  // Krylov basis, work, b, x, r
+    // qa, qb, lsq, res, vol, dt, boundary areas, right-hand side, update
+    bytes += 8 * (kQA * (n + b) + kQB * n + 8 * n + kRES * n + n + n + 4 * b + 5 * n + 5 * n);
+    // order, inc_ptr, sptr, n2n lists, boundary node/type/tag, marks, ready flags, epoch, risk
+    bytes += 4 * (n + (n + 1) + (n / kSlice + 2) + (n + 1) + 2 * e + 3 * b + n + n + 2);
+    bytes += nsell * (4 + 4 + 3 * 8);                       // nbr, soff, isx, isy, isz
+    bytes += 4 * ((n + 1) + nnz + n);                       // BSR ia, ja, iau
+    bytes += 4 * (2 * n + 3 * n + (11 + 63 + 11) * lvl);    // level rows and padded slot lists
+    bytes += 8 * (1024 * 65 + 4 * 65 + 6) + 4 * 4;         // reduction scratch, eigenvalue bits, flags
+    bytes += 25 * s * nnz * 2 + 25 * s * n;                 // Jacobian, ILU(0) factors, inverse pivots
+    bytes += s * 5 * n * (2 * (int64_t)km + 2) + 8 * 5 * n; // Krylov V, Z, w; FP64 true residual
     return bytes;
 }
 

@@ -56,6 +56,14 @@ struct LevelSchedule {
     int n12 = 0, n64 = 0;
 };
 
+// Captured graphs of the FGMRES Arnoldi steps (one per step index), valid
+// for one Krylov buffer, restart length and preconditioner.
+struct KrylovGraphs {
+    const void *key = nullptr, *lu = nullptr;
+    int prec = -1, m = 0;
+    std::vector<void *> exec;  // hipGraphExec_t
+};
+
 struct cfd_mesh {
     cfd_ctx *ctx = nullptr;
     hipStream_t stream = nullptr;
@@ -110,6 +118,8 @@ struct cfd_mesh {
     bool prec_mixed = false;
 
     // Krylov workspace
+    bool use_graphs = true;   // replay captured Arnoldi steps
+    KrylovGraphs kgraph;
     int kry_m = 0;
     bool kry_mixed = false;
     double *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr;
@@ -182,3 +192,4 @@ int launch_residual_to_fp64(cfd_mesh *m);
 
 int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed);
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);
+void drop_krylov_graphs(cfd_mesh *m);

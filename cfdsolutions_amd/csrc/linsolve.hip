@@ -706,6 +706,16 @@ __global__ __launch_bounds__(kBlock) void k_scale(size_t n, const TS *src, TD *d
         dst[i] = (TD)((double)src[i] / denom);
 }
 
+// dst = src / sqrt(*norm2): the Arnoldi normalisation with the norm read from
+// device memory (the host evaluates the same correctly rounded sqrt for its
+// Givens rotations), so the step can be enqueued without a host round trip.
+template <class TS, class TD>
+__global__ __launch_bounds__(kBlock) void k_scale_dev(size_t n, const TS *src, TD *dst, const double *norm2) {
+    const double denom = sqrt(norm2[0]);
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        dst[i] = (TD)((double)src[i] / denom);
+}
+
 // x += sum_k y_k Z_k
 template <class TV>
 __global__ __launch_bounds__(kBlock) void k_xupdate(size_t n, int nv, const TV *__restrict__ Z, size_t ld,
@@ -846,6 +856,65 @@ int ensure_krylov(cfd_mesh *m, int mk, bool mixed) {
     return CFD_OK;
 }
 
+// One Arnoldi step j of FGMRES: V_j = w / ||w|| (j > 0, norm of the previous
+// step), Z_j = M^-1 V_j, w = A Z_j, two CGS passes against V_0..V_j, ||w||^2,
+// and the copy of the Hessenberg column to the host.  The sequence depends
+// only on j and the mesh's buffers, so it is captured once per j as a graph.
+template <class TM, class TV>
+void arnoldi_step(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, TV *V, TV *Z, TV *w, int j) {
+    const size_t L = 5 * (size_t)m->N;
+    const int gN = grid_cap(L, 4096);
+    const int gS = (int)((L + kBlock - 1) / kBlock);
+    TV *vj = V + L * j, *zj = Z + L * j;
+    if (j > 0)
+        hipLaunchKernelGGL((k_scale_dev<TV, TV>), dim3(gN), dim3(kBlock), 0, m->stream, L, w, vj,
+                           m->redout + 2 * (kMaxKrylov + 1));
+    apply_prec<TM, TV>(m, m->prec_kind, lu, dinv, vj, zj);
+    hipLaunchKernelGGL((k_spmv<TM, TV, TV>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a, zj, w,
+                       nullptr);
+    // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
+    mdot<TV>(m, j + 1, V, L, w, 0);
+    hipLaunchKernelGGL(k_maxpy<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, V, L, m->redout, w);
+    mdot<TV>(m, j + 1, V, L, w, kMaxKrylov + 1);
+    hipLaunchKernelGGL(k_maxpy<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, V, L, m->redout + kMaxKrylov + 1,
+                       w);
+    mdot<TV>(m, 1, w, 0, w, 2 * (kMaxKrylov + 1));
+    hipMemcpyAsync(m->hred, m->redout, sizeof(double) * (2 * (kMaxKrylov + 1) + 1), hipMemcpyDeviceToHost, m->stream);
+}
+
+// Enqueues Arnoldi step j, replaying its captured graph (captured on first use).
+template <class TM, class TV>
+int enqueue_step(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, TV *V, TV *Z, TV *w, int j, int mk) {
+    if (!m->use_graphs) {
+        arnoldi_step<TM, TV>(m, a, lu, dinv, V, Z, w, j);
+        CFD_HIP(m, hipGetLastError());
+        return CFD_OK;
+    }
+    KrylovGraphs &kg = m->kgraph;
+    if (kg.key != (const void *)V || kg.prec != m->prec_kind || kg.m != mk || kg.lu != (const void *)lu) {
+        drop_krylov_graphs(m);
+        kg.key = V;
+        kg.lu = lu;
+        kg.prec = m->prec_kind;
+        kg.m = mk;
+        kg.exec.assign(mk, nullptr);
+    }
+    if (!kg.exec[j]) {
+        hipGraph_t g = nullptr;
+        CFD_HIP(m, hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        arnoldi_step<TM, TV>(m, a, lu, dinv, V, Z, w, j);
+        const hipError_t e1 = hipStreamEndCapture(m->stream, &g);
+        if (e1 != hipSuccess) return cfd_hip_check(m->ctx, e1, "hipStreamEndCapture");
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e2 != hipSuccess) return cfd_hip_check(m->ctx, e2, "hipGraphInstantiate");
+        kg.exec[j] = ex;
+    }
+    CFD_HIP(m, hipGraphLaunch((hipGraphExec_t)kg.exec[j], m->stream));
+    return CFD_OK;
+}
+
 // Restarted right-preconditioned FGMRES(m) with CGS2; same algorithm as
 // orc_fgmres.  TM: matrix and preconditioner storage, TV: Krylov vectors.
 template <class TM, class TV>
@@ -875,19 +944,8 @@ int fgmres(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM *lu, c
         gv[0] = beta;
         int kdim = 0;
         for (int j = 0; j < mk; j++) {
-            TV *vj = V + L * j, *zj = Z + L * j;
-            apply_prec<TM, TV>(m, m->prec_kind, lu, dinv, vj, zj);
-            hipLaunchKernelGGL((k_spmv<TM, TV, TV>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a, zj,
-                               w, nullptr);
-            // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
-            mdot<TV>(m, j + 1, V, L, w, 0);
-            hipLaunchKernelGGL(k_maxpy<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, V, L, m->redout, w);
-            mdot<TV>(m, j + 1, V, L, w, kMaxKrylov + 1);
-            hipLaunchKernelGGL(k_maxpy<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, V, L,
-                               m->redout + kMaxKrylov + 1, w);
-            mdot<TV>(m, 1, w, 0, w, 2 * (kMaxKrylov + 1));
-            CFD_HIP(m, hipMemcpyAsync(hr, m->redout, sizeof(double) * (2 * (kMaxKrylov + 1) + 1),
-                                      hipMemcpyDeviceToHost, m->stream));
+            int rc = enqueue_step<TM, TV>(m, a, lu, dinv, V, Z, w, j, mk);
+            if (rc) return rc;
             CFD_HIP(m, hipStreamSynchronize(m->stream));
             for (int k = 0; k <= j; k++) hcol[k] = 0.0 + hr[k];
             for (int k = 0; k <= j; k++) hcol[k] += hr[kMaxKrylov + 1 + k];
@@ -910,8 +968,7 @@ int fgmres(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM *lu, c
             kdim = j + 1;
             res = fabs(gv[j + 1]);
             if (!(res == res)) return cfd_set_error(m->ctx, CFD_E_NONFINITE, "FGMRES: non-finite residual");
-            if (hn != 0.0)
-                hipLaunchKernelGGL((k_scale<TV, TV>), dim3(gN), dim3(kBlock), 0, m->stream, L, w, V + L * (j + 1), hn);
+            // the next step starts with V_{j+1} = w / hn (k_scale_dev)
             if (res <= tol || hn == 0.0) break;
         }
         for (int k = kdim - 1; k >= 0; k--) {
@@ -942,6 +999,13 @@ int fgmres(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM *lu, c
 }
 
 }  // namespace
+
+void drop_krylov_graphs(cfd_mesh *m) {
+    for (auto &g : m->kgraph.exec)
+        if (g) hipGraphExecDestroy((hipGraphExec_t)g);
+    m->kgraph.exec.clear();
+    m->kgraph.key = nullptr;
+}
 
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st) {
     if (!m->jac_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "cfd_linsolve: no Jacobian assembled");

@@ -63,6 +63,41 @@ def test_sync_free_sweeps_match_level_launches(ctx, mixed):
     assert np.array_equal(out[0][2], out[1][2])
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_graph_replays_match_eager(ctx, mixed):
+    """Captured Arnoldi-step graphs, replayed over three Newton steps (new matrix values each time, eager work and
+    synchronisations in between), give the bits of the eager launches at every replay."""
+    print("graph replays vs eager, mixed", mixed, flush=True)
+    m = cmesh.bump_channel(16, 6, 6)
+    p = cfd.make_params(mach=0.5, alpha_deg=1.0, flux="roe", order=1)
+    runs = {}
+    old = os.environ.get("CFD_GRAPHS")
+    try:
+        for graphs in (False, True):
+            os.environ["CFD_GRAPHS"] = "1" if graphs else "0"
+            dm = cfd.DeviceMesh(ctx, m, p)
+            out = []
+            for it in range(3):
+                dm.bc(p, it)
+                dm.residual(p)
+                dm.delta_t(p, it)
+                dm.jacobian(p, rhs_mode=1, mixed=mixed)
+                st = dm.linsolve(cfd.SOLVER_FGMRES, cfd.PREC_ILU0, krylov_m=20, max_restarts=2, rtol=1e-6,
+                                 mixed=mixed)
+                out.append((st.iterations, st.final_res, dm.get(cfd.FIELD_X)))
+                dm.update(1.0)
+            runs[graphs] = out
+            dm.close()
+    finally:
+        if old is None:
+            os.environ.pop("CFD_GRAPHS", None)
+        else:
+            os.environ["CFD_GRAPHS"] = old
+    for (i0, r0, x0), (i1, r1, x1) in zip(runs[False], runs[True]):
+        assert i0 == i1 and r0 == r1
+        assert np.array_equal(x0, x1)
+
+
 REQS = [  # (mesh key, dims, flux, order, mach, alpha)
     (1, (12, 4, 4), "roe", 1, 0.5, 0.0),
     (2, (16, 6, 4), "hllc", 1, 0.6, 2.0),
