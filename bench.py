@@ -190,7 +190,8 @@ def solve_leg(ctx):
     return res
 
 
-def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202):
+def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0.3, 0.85), aoa=(-2.0, 6.0),
+              fluxes=("roe", "hllc"), stream=True):
     """Steady-solve requests on bump channels of log-uniform size in [nmin, nmax] nodes, bump height U(0.05,
     0.15), Mach U(0.3, 0.85), AoA U(-2, 6) deg, Roe or HLLC (first order), to RMS 1e-10.  The plain FP64 GPU
     path solves them one at a time (meshes resident); the server solves them in mixed precision with
@@ -202,8 +203,8 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202):
     pool = [mesh.bump_channel(*bump_dims(n), bump_h=h) for n, h in zip(sizes, heights)]
     reqs = []
     for _ in range(n_req):
-        reqs.append((int(rng.integers(n_meshes)), float(rng.uniform(0.3, 0.85)), float(rng.uniform(-2.0, 6.0)),
-                     "roe" if rng.random() < 0.5 else "hllc"))
+        reqs.append((int(rng.integers(n_meshes)), float(rng.uniform(*mach)), float(rng.uniform(*aoa)),
+                     fluxes[int(rng.integers(len(fluxes)))]))
 
     def params(r):
         return cfd.make_params(mach=r[1], alpha_deg=r[2], flux=r[3], order=1)
@@ -248,8 +249,11 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202):
 
     burst, span_b, lat_b = run(np.zeros(n_req))
     cap = n_req / span_b
-    gaps = rng.exponential(1.0 / (0.8 * cap), n_req)
-    stream, span_s, lat_s = run(np.cumsum(gaps) - gaps[0])
+    if stream:
+        gaps = rng.exponential(1.0 / (0.8 * cap), n_req)
+        stream, span_s, lat_s = run(np.cumsum(gaps) - gaps[0])
+    else:
+        stream, span_s, lat_s = burst, span_b, lat_b
     st = srv.stats()
     srv.close()
     bad = [c for c in burst + stream if c["status"] != 0 or not c["result"].converged]
@@ -258,9 +262,10 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202):
         f64 = np.array(res64[c["id"]].force)
         ferr = max(ferr, float(np.max(np.abs(np.array(c["result"].force) - f64)) / np.max(np.abs(f64))))
     base = n_req / t64
-    res = {"workload": "%d requests on %d bump channels of %d-%d nodes (log-uniform), 1st-order Roe/HLLC, "
-                       "M 0.3-0.85, AoA -2..6, RMS < 1e-10, %d in flight" % (n_req, n_meshes, min(m.n_node for m in pool),
-                                                                         max(m.n_node for m in pool), inflight),
+    res = {"workload": "%d requests on %d bump channels of %d-%d nodes (log-uniform), 1st-order %s, "
+                       "M %.2f-%.2f, AoA %g..%g, RMS < 1e-10, %d in flight" % (
+                           n_req, n_meshes, min(m.n_node for m in pool), max(m.n_node for m in pool),
+                           "/".join(fluxes), mach[0], mach[1], aoa[0], aoa[1], inflight),
            "fp64_sequential_solves_per_s": base,
            "serving_capacity_solves_per_s": cap,
            "serving_vs_fp64": cap / base,
@@ -291,6 +296,8 @@ def main():
     ap.add_argument("--serve-meshes", type=int, default=6)
     ap.add_argument("--serve-nodes", type=str, default="20000,150000", help="min,max nodes of the serving meshes")
     ap.add_argument("--serve-inflight", type=int, default=6)
+    ap.add_argument("--batch-cases", type=int, default=0,
+                    help="also run N small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) through the server")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (scale factor for HBM counters)")
     args = ap.parse_args()
@@ -375,6 +382,10 @@ def main():
         solves = solve_leg(ctx)
         nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
         serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight)
+        if args.batch_cases:
+            serving["small_cases"] = serve_leg(ctx, args.batch_cases, min(args.batch_cases, 16), 8000, 40000,
+                                               args.serve_inflight, seed=7, mach=(0.3, 0.8), aoa=(-2.0, 4.0),
+                                               fluxes=("roe",), stream=False)
     ctx.close()
     r_achieved = variants["roe-fp64-GL+R2"]["r_gbs"]
     out = {

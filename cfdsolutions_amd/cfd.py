@@ -380,6 +380,23 @@ class Server:
             pass
 
 
+def batch_solve(ctx, meshes, params, alphas=None, **solve_kw):
+    """cfd_batch_solve: solves len(meshes) independent cases as one union mesh.  params is one Params per case
+    (same scheme, own freestream); solve_kw are the shared solver options; alphas the per-case angles for lift and
+    drag.  Returns (status, [SolveResult])."""
+    K = len(meshes)
+    keep = [mesh_desc(m) for m in meshes]
+    descs = (MeshDesc * K)(*[d for d, _ in keep])
+    ps = (Params * K)(*params)
+    alphas = alphas if alphas is not None else [0.0] * K
+    kw = dict(solve_kw)
+    kw.pop("alpha_deg", None)
+    os_ = (SolveOpts * K)(*[solve_opts(alpha_deg=a, **kw) for a in alphas])
+    res = (SolveResult * K)()
+    rc = lib().cfd_batch_solve(ctx.h, K, descs, ps, os_, res)
+    return rc, [res[k] for k in range(K)]
+
+
 def footprint_bytes(n_node, n_edge, n_bedge, krylov_m=30, mixed=True):
     return lib().cfd_footprint_bytes(int(n_node), int(n_edge), int(n_bedge), int(krylov_m), int(bool(mixed)))
 

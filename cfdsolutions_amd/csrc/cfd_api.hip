@@ -139,6 +139,9 @@ extern "C" void cfd_mesh_destroy(cfd_mesh *m) {
     for (void *p : m->allocs) hipFree(p);
     if (m->hred) hipHostFree(m->hred);
     if (m->hflag) hipHostFree(m->hflag);
+    if (m->hbred) hipHostFree(m->hbred);
+    if (m->hbcoef) hipHostFree(m->hbcoef);
+    if (m->hbkdim) hipHostFree(m->hbkdim);
     if (m->stream) hipStreamDestroy(m->stream);
     delete m;
 }
@@ -156,6 +159,14 @@ static void freestream_q(const cfd_params *p, double *q5) {
 }
 
 extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params *p, cfd_mesh **out) {
+    return cfd_mesh_create_cases(ctx, d, p, nullptr, out);
+}
+
+// node_case (may be NULL): case index of every node of a union of
+// independent meshes; the internal numbering then keeps each case's nodes
+// contiguous (case-major, Morton order within a case).
+int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params *p, const int *node_case,
+                          cfd_mesh **out) {
     if (!ctx || !d || !p || !out) return CFD_E_ARG;
     const int N = d->n_node, E = d->n_edge, B = d->n_bedge;
     if (N <= 0 || E < 0 || B < 0 || !d->edge || !d->edge_area || (B && (!d->bedge || !d->bedge_area || !d->bedge_type)) ||
@@ -208,16 +219,33 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
         // vectors stay in the reference ordering.
         std::vector<int> order(N), iperm(N);
         {
-            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-            for (int i = 0; i < N; i++)
+            // bounding box per case, so a case of a union is numbered exactly
+            // as the same mesh on its own
+            int ncs = 1;
+            if (node_case)
+                for (int i = 0; i < N; i++) ncs = std::max(ncs, node_case[i] + 1);
+            std::vector<double> blo(3 * (size_t)ncs, 1e300), bhi(3 * (size_t)ncs, -1e300);
+            for (int i = 0; i < N; i++) {
+                const size_t cb = 3 * (size_t)(node_case ? node_case[i] : 0);
                 for (int k = 0; k < 3; k++) {
-                    lo[k] = std::min(lo[k], d->xyz[3 * (size_t)i + k]);
-                    hi[k] = std::max(hi[k], d->xyz[3 * (size_t)i + k]);
+                    blo[cb + k] = std::min(blo[cb + k], d->xyz[3 * (size_t)i + k]);
+                    bhi[cb + k] = std::max(bhi[cb + k], d->xyz[3 * (size_t)i + k]);
                 }
-            std::vector<std::pair<uint64_t, int>> key(N);
+            }
+            struct Key {
+                int cs;
+                uint64_t morton;
+                int node;
+                bool operator<(const Key &o) const {
+                    return cs != o.cs ? cs < o.cs : (morton != o.morton ? morton < o.morton : node < o.node);
+                }
+            };
+            std::vector<Key> key(N);
             for (int i = 0; i < N; i++) {
                 uint64_t c = 0;
                 uint32_t q3[3];
+                const double *lo = &blo[3 * (size_t)(node_case ? node_case[i] : 0)];
+                const double *hi = &bhi[3 * (size_t)(node_case ? node_case[i] : 0)];
                 for (int k = 0; k < 3; k++) {
                     const double span = hi[k] - lo[k];
                     const double f = span > 0 ? (d->xyz[3 * (size_t)i + k] - lo[k]) / span : 0.0;
@@ -225,12 +253,12 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
                 }
                 for (int bit = 20; bit >= 0; bit--)
                     for (int k = 0; k < 3; k++) c = (c << 1) | ((q3[k] >> bit) & 1u);
-                key[i] = {c, i};
+                key[i] = {node_case ? node_case[i] : 0, c, i};
             }
             std::sort(key.begin(), key.end());
             for (int t = 0; t < N; t++) {
-                order[t] = key[t].second;
-                iperm[key[t].second] = t;
+                order[t] = key[t].node;
+                iperm[key[t].node] = t;
             }
         }
         m->h_order = order;
@@ -332,6 +360,34 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
             size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
             for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) soff[idx] = off[k];
         }
+        // ILU(0) merge map (see cfd_mesh::ilu_mqq)
+        std::vector<int> moff(m->nnz, 0), mqq;
+        {
+            int64_t tot = 0;
+            for (int i = 0; i < N; i++)
+                for (int kk = m->h_ia[i]; kk < m->h_iau[i]; kk++) {
+                    moff[kk] = (int)tot;
+                    const int k = m->h_ja[kk];
+                    tot += m->h_ia[k + 1] - (m->h_iau[k] + 1);
+                    if (tot > INT32_MAX) {
+                        cfd_mesh_destroy(m);
+                        return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: ILU(0) merge map too large");
+                    }
+                }
+            mqq.assign((size_t)tot, -1);
+            for (int i = 0; i < N; i++)
+                for (int kk = m->h_ia[i]; kk < m->h_iau[i]; kk++) {
+                    const int k = m->h_ja[kk];
+                    int qq = kk + 1;
+                    const int qe = m->h_ia[i + 1];
+                    for (int pp = m->h_iau[k] + 1, t2 = 0; pp < m->h_ia[k + 1]; pp++, t2++) {
+                        while (qq < qe && m->h_ja[qq] < m->h_ja[pp]) qq++;
+                        if (qq < qe && m->h_ja[qq] == m->h_ja[pp]) mqq[(size_t)moff[kk] + t2] = qq;
+                    }
+                }
+            m->ilu_nmap = tot;
+            if (mqq.empty()) mqq.push_back(-1);  // keeps the device array non-empty
+        }
         std::vector<int> lrows, urows;
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, false, m->lower.ptr, lrows, m->lower.maxwidth);
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, true, m->upper.ptr, urows, m->upper.maxwidth);
@@ -384,16 +440,15 @@ extern "C" int cfd_mesh_create(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_p
         if (!rc) m->ia = upload(m, m->h_ia.data(), m->h_ia.size(), &rc);
         if (!rc) m->ja = upload(m, m->h_ja.data(), m->h_ja.size(), &rc);
         if (!rc) m->iau = upload(m, m->h_iau.data(), m->h_iau.size(), &rc);
+        if (!rc) m->ilu_moff = upload(m, moff.data(), moff.size(), &rc);
+        if (!rc) m->ilu_mqq = upload(m, mqq.data(), mqq.size(), &rc);
         if (!rc) m->lower.rows = upload(m, lrows.data(), lrows.size(), &rc);
         if (!rc) m->upper.rows = upload(m, urows.data(), urows.size(), &rc);
         {
-            std::vector<int> s12, s64;
+            std::vector<int> s12;
             pad_levels(m->lower.ptr, lrows, 12, s12);
-            pad_levels(m->lower.ptr, lrows, 64, s64);
             m->lower.n12 = (int)s12.size();
-            m->lower.n64 = (int)s64.size();
             if (!rc) m->lower.slots12 = upload(m, s12.data(), s12.size(), &rc);
-            if (!rc) m->lower.slots64 = upload(m, s64.data(), s64.size(), &rc);
             pad_levels(m->upper.ptr, urows, 12, s12);
             m->upper.n12 = (int)s12.size();
             if (!rc) m->upper.slots12 = upload(m, s12.data(), s12.size(), &rc);
@@ -467,8 +522,9 @@ extern "C" int64_t cfd_footprint_bytes(int32_t N, int32_t E, int32_t B, int32_t 
     // order, inc_ptr, sptr, n2n lists, boundary node/type/tag, marks, ready flags, epoch, risk
     bytes += 4 * (n + (n + 1) + (n / kSlice + 2) + (n + 1) + 2 * e + 3 * b + n + n + 2);
     bytes += nsell * (4 + 4 + 3 * 8);                       // nbr, soff, isx, isy, isz
-    bytes += 4 * ((n + 1) + nnz + n);                       // BSR ia, ja, iau
-    bytes += 4 * (2 * n + 3 * n + (11 + 63 + 11) * lvl);    // level rows and padded slot lists
+    bytes += 4 * ((n + 1) + nnz + n) + 4 * nnz;             // BSR ia, ja, iau; ILU(0) merge offsets
+    bytes += 4 * 8 * nnz;                                   // ILU(0) merge map (<= 8 upper blocks per lower entry)
+    bytes += 4 * (2 * n + 2 * n + (11 + 11) * lvl);        // level rows and padded slot lists
     bytes += 8 * (1024 * 65 + 4 * 65 + 6) + 4 * 4;         // reduction scratch, eigenvalue bits, flags
     bytes += 25 * s * nnz * 2 + 25 * s * n;                 // Jacobian, ILU(0) factors, inverse pivots
     bytes += s * 5 * n * (2 * (int64_t)km + 2) + 8 * 5 * n; // Krylov V, Z, w; FP64 true residual

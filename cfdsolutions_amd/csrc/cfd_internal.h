@@ -50,10 +50,26 @@ struct LevelSchedule {
     int nlevels = 0;
     int maxwidth = 0;
     // rows in level order with every level padded (-1) to a multiple of 12
-    // (sync-free sweeps, 12 rows per wavefront) and of 64 (sync-free ILU(0),
-    // one row per lane)
-    int *slots12 = nullptr, *slots64 = nullptr;
-    int n12 = 0, n64 = 0;
+    // (sync-free sweeps, 12 rows of 5 lanes per wavefront)
+    int *slots12 = nullptr;
+    int n12 = 0;
+};
+
+constexpr int kMaxKrylov = 64;   // largest FGMRES restart length
+constexpr int kDotParts = 512;   // partial sums of one Krylov dot product
+constexpr int kBOut = 2 * (kMaxKrylov + 1) + 1;  // per-case results of one batched Arnoldi step
+
+// Block map of a segmented reduction over the cases of a union mesh: case c
+// gets nb[c] blocks that stride over its own range exactly as a launch of
+// nb[c] blocks over that case alone does, and its partial sums are combined
+// in the same order, so per-case results equal the single-mesh ones bit for
+// bit.
+struct SegMap {
+    int nblocks = 0;
+    int2 *blk = nullptr;                 // device [nblocks]: (case, block within the case)
+    int *first = nullptr;                // device [ncase+1]: first block of each case
+    int *off = nullptr, *len = nullptr;  // device [ncase]: range of each case
+    std::vector<int> h_first;
 };
 
 // Captured graphs of the FGMRES Arnoldi steps (one per step index), valid
@@ -102,6 +118,11 @@ struct cfd_mesh {
     int nnz = 0;
     std::vector<int> h_ia, h_ja, h_iau;
     int *ia = nullptr, *ja = nullptr, *iau = nullptr;
+    // ILU(0) merge map: for lower entry kk = (i, k) of the BSR pattern and the
+    // t-th upper block of row k, ilu_mqq[ilu_moff[kk] + t] is the slot of the
+    // block of row i in the same column, or -1
+    int *ilu_moff = nullptr, *ilu_mqq = nullptr;
+    int64_t ilu_nmap = 0;
     double *a = nullptr, *b = nullptr, *x = nullptr;
     float *af = nullptr;      // FP32 Jacobian (mixed)
     double *lu = nullptr, *dinv = nullptr;
@@ -135,12 +156,29 @@ struct cfd_mesh {
     int res_prec = CFD_FP64;   // precision of the last residual
     bool hllc_check = false;   // HLLC anomaly flag not yet read back
     int64_t q_version = 0, f_version = -1;  // FP32 mirror of the state is current when equal
+
+    // union of independent cases (batched solves); ncase = 0 for one mesh.
+    // Nodes, boundary edges and [N][5] vector entries of case c occupy one
+    // contiguous range in both numberings.
+    int ncase = 0;
+    std::vector<int> case_node, case_bedge;  // host offsets [ncase+1]
+    double *case_inf = nullptr;              // device [ncase][8] freestream rho, u, v, w, p, outflow p
+    int *bcase = nullptr;                    // device [B] case of each boundary edge
+    int *ecase = nullptr;                    // device [N] case of each node
+    SegMap seg_rms, seg_dot, seg_forces;
+    double *bpart = nullptr;                 // device partial sums of segmented reductions
+    double *bred = nullptr, *hbred = nullptr;  // per-case results [ncase][kBOut] (device, pinned host)
+    double *bcoef = nullptr, *hbcoef = nullptr;  // [ncase][kMaxKrylov+1] per-case coefficients (device, pinned)
+    int *bkdim = nullptr, *hbkdim = nullptr;     // [ncase] per-case Krylov dimension (device, pinned)
+    KrylovGraphs kgraph_b;                   // batched Arnoldi steps
 };
 
 // ---------------------------------------------------------------------------
 // helpers implemented in cfd_api.hip
 // ---------------------------------------------------------------------------
 int cfd_set_error(cfd_ctx *ctx, int code, const char *fmt, ...);
+int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params *p, const int *node_case,
+                          cfd_mesh **out);
 int cfd_hip_check(cfd_ctx *ctx, hipError_t e, const char *what);
 void *cfd_dalloc(cfd_mesh *m, size_t bytes);  // device allocation tracked on the mesh
 int cfd_ensure_fp32(cfd_mesh *m);
@@ -192,4 +230,8 @@ int launch_residual_to_fp64(cfd_mesh *m);
 
 int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed);
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);
+// Batched FGMRES over a union mesh: solves the cases with solve[c] != 0,
+// writes each case's iterations and final residual estimate.
+int run_linsolve_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const std::vector<char> &solve,
+                       std::vector<int> &iters, std::vector<double> &final_res);
 void drop_krylov_graphs(cfd_mesh *m);

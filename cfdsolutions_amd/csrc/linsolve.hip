@@ -20,8 +20,6 @@ using namespace cfd;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxKrylov = 64;
-constexpr int kDotParts = 512;
 
 // ---------------------------------------------------------------------------
 // Jacobian blocks
@@ -254,46 +252,76 @@ __device__ __forceinline__ void bmul(const TM *A, const TM *B, TM *C) {
 // orc_ilu0.  Factors in TM; the pivot inverse is computed in FP64.
 // ---------------------------------------------------------------------------
 template <class TM>
-__global__ __launch_bounds__(64) void k_ilu0_level(int nrows, const int *__restrict__ rows, const int *__restrict__ ia,
-                                                   const int *__restrict__ ja, const int *__restrict__ iau, TM *lu,
-                                                   TM *dinv, int *flag) {
-    const int t = blockIdx.x * 64 + threadIdx.x;
-    if (t >= nrows) return;
-    const int i = rows[t];
-    TM L[25], U[25], P[25];
-    for (int kk = ia[i]; kk < iau[i]; kk++) {
-        const int k = ja[kk];
-        TM *Lk = lu + 25 * (size_t)kk;
-        for (int q = 0; q < 25; q++) U[q] = Lk[q];
-        bmul(U, dinv + 25 * (size_t)k, L);
-        for (int q = 0; q < 25; q++) Lk[q] = L[q];
-        int pp = iau[k] + 1, qq = kk + 1;
-        const int pe = ia[k + 1], qe = ia[i + 1];
-        while (pp < pe && qq < qe) {
-            const int cp = ja[pp], cq = ja[qq];
-            if (cp == cq) {
-                const TM *Up = lu + 25 * (size_t)pp;
-                for (int q = 0; q < 25; q++) U[q] = Up[q];
-                bmul(L, U, P);
-                TM *T = lu + 25 * (size_t)qq;
-                for (int q = 0; q < 25; q++) T[q] = T[q] - P[q];
-                pp++;
-                qq++;
-            } else if (cp < cq) {
-                pp++;
-            } else {
-                qq++;
+__global__ __launch_bounds__(kBlock) void k_ilu0_level(int nrows, const int *__restrict__ rows,
+                                                      const int *__restrict__ ia, const int *__restrict__ ja,
+                                                      const int *__restrict__ iau, const int *__restrict__ moff,
+                                                      const int *__restrict__ mqq, TM *lu, TM *dinv,
+                                                      const TM *__restrict__ luk, const TM *__restrict__ dinvk,
+                                                      int *flag) {
+    // luk / dinvk view the blocks and inverse pivots of rows of earlier
+    // levels, which this launch only reads (it writes rows of one level).
+    // 12 rows of 5 lanes per wavefront; lane r owns row r of every block of
+    // its block row, so the products of two 5x5 blocks are split by rows with
+    // the per-entry summation order of bmul.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / 5, r = lane - 5 * slot;
+    const int t = (blockIdx.x * (kBlock / 64) + wave) * 12 + slot;
+    const bool active = slot < 12 && t < nrows;
+    const int i = active ? rows[t] : 0;
+    TM drow[5] = {TM(0.0), TM(0.0), TM(0.0), TM(0.0), TM(0.0)};
+    if (active) {
+        for (int kk = ia[i]; kk < iau[i]; kk++) {
+            const int k = ja[kk];
+            TM *Lk = lu + 25 * (size_t)kk + 5 * r;
+            const TM *Dk = dinvk + 25 * (size_t)k;
+            TM Lr[5], U[5];
+#pragma unroll
+            for (int q = 0; q < 5; q++) U[q] = Lk[q];
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                TM s = TM(0.0);
+#pragma unroll
+                for (int q = 0; q < 5; q++) s += U[q] * Dk[5 * q + c];
+                Lr[c] = s;
+            }
+#pragma unroll
+            for (int c = 0; c < 5; c++) Lk[c] = Lr[c];
+            // upper blocks of row k that meet a block of row i, in ascending
+            // column order (the precomputed merge of the two rows)
+            const int p0 = iau[k] + 1, np = ia[k + 1] - p0, m0 = moff[kk];
+#pragma unroll 2
+            for (int t2 = 0; t2 < np; t2++) {
+                const int qq = mqq[m0 + t2];
+                if (qq < 0) continue;
+                const TM *Up = luk + 25 * (size_t)(p0 + t2);
+                TM *T = lu + 25 * (size_t)qq + 5 * r;
+#pragma unroll
+                for (int c = 0; c < 5; c++) {
+                    TM s = TM(0.0);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) s += Lr[q] * Up[5 * q + c];
+                    T[c] = T[c] - s;
+                }
             }
         }
+        const TM *Dp = lu + 25 * (size_t)iau[i] + 5 * r;
+#pragma unroll
+        for (int c = 0; c < 5; c++) drow[c] = Dp[c];
     }
+    // every lane of the row gathers the diagonal block and inverts it (FP64)
+    const int base = lane - r;
     double Dg[25], Di[25];
-    const TM *Dp = lu + 25 * (size_t)iau[i];
-    for (int q = 0; q < 25; q++) Dg[q] = (double)Dp[q];
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+#pragma unroll
+        for (int c = 0; c < 5; c++) Dg[5 * q + c] = (double)__shfl(drow[c], base + q, 64);
+    if (!active) return;
     if (!inv5(Dg, Di)) {
-        atomicAdd(flag + 2, 1);
+        if (r == 0) atomicAdd(flag + 2, 1);
         for (int q = 0; q < 25; q++) Di[q] = 0.0;
     }
-    for (int q = 0; q < 25; q++) dinv[25 * (size_t)i + q] = (TM)Di[q];
+#pragma unroll
+    for (int c = 0; c < 5; c++) dinv[25 * (size_t)i + 5 * r + c] = (TM)Di[5 * r + c];
 }
 
 // Inverse diagonal blocks (block-Jacobi).
@@ -367,36 +395,63 @@ __global__ __launch_bounds__(kBlock) void k_trsv_level(int nrows, const int *__r
 }
 
 // ---------------------------------------------------------------------------
-// Sync-free triangular sweeps: one launch per sweep instead of one per level.
+// Sync-free triangular sweeps (ILU(0) apply): one launch per sweep instead of
+// one per level.  The factorization, once per Jacobian, stays one launch per
+// level.
 // Rows are taken in level order and every wavefront holds rows of a single
 // level (levels are padded to whole wavefronts, pad = -1), so a row waits
 // only on rows of earlier wavefronts.  Workgroups are dispatched in index
 // order, so the earliest unfinished row always has a slot and the sweep
-// always progresses.  A finished row publishes its values, then sets
-// ready[row] = epoch with release semantics; a dependent row spins on the
-// flag (acquire, which also invalidates this CU's L1) before reading them.
-// The epoch is read from device memory (bumped by k_epoch_bump before each
-// sweep), so ready[] never needs clearing.  A wait that exceeds kSpinMax
-// polls raises flag[3]; every waiter also watches flag[3] and gives up once
-// it is set, so a sweep always ends, after at most one bounded wait.
+// always progresses.
+//
+// Each XCD has its own L2 and the L2s are not coherent with one another; an
+// agent-scope acquire invalidates the whole local L2, so it must not sit in a
+// polling loop.  The values one row hands to another are written with
+// coherent (relaxed agent-scope atomic) stores that go past the L2, then the
+// row sets ready[row] = epoch with an agent-scope release store (one L2
+// write-back per finished row).  A waiting row polls the flag with coherent
+// loads, orders
+// its later loads after the poll (workgroup acquire fence: no cache
+// invalidation) and reads the handed-over values with coherent loads, which
+// bypass stale cache lines.  The epoch is read from device
+// memory (bumped by k_epoch_bump before each sweep), so ready[] never needs
+// clearing.  A wait that exceeds kSpinMax polls raises flag[3]; every waiter
+// also watches flag[3] and gives up once it is set, so a sweep always ends,
+// after at most one bounded wait.
 // ---------------------------------------------------------------------------
-constexpr int kSpinMax = 1 << 18;
+constexpr int kSpinMax = 1 << 22;
 
 __global__ void k_epoch_bump(int *epoch) {
     if (threadIdx.x == 0 && blockIdx.x == 0) epoch[0] = epoch[0] + 1;
 }
 
-__device__ __forceinline__ bool wait_ready(const int *ready, int j, int ep, int *flag) {
-    for (int it = 0; it < kSpinMax; it++) {
-        if (__hip_atomic_load(ready + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == ep) return true;
-        if ((it & 255) == 255 && __hip_atomic_load(flag + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-            return false;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    atomicOr(flag + 3, 1);
-    return false;
+template <class T>
+__device__ __forceinline__ T ld_coh(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_coh(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ bool wait_ready(const int *ready, int j, int ep, int *flag) {
+    bool ok = false;
+    for (int it = 0; it < kSpinMax; it++) {
+        if (ld_coh(ready + j) == ep) {
+            ok = true;
+            break;
+        }
+        if ((it & 255) == 255 && ld_coh(flag + 3) != 0) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (!ok && ld_coh(flag + 3) == 0) atomicOr(flag + 3, 1);
+    return ok;
+}
+
+// The row's stores are visible device-wide before its flag is set (an
+// s_waitcnt on the coherent stores alone does not order them before the flag
+// for readers on other XCDs).
 __device__ __forceinline__ void publish_ready(int *ready, int i, int ep) {
     __hip_atomic_store(ready + i, ep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -418,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_trsv_sf(int nslots, const int *__res
     if (i >= 0) {
         const int kb = backward ? iau[i] + 1 : ia[i];
         const int ke = backward ? ia[i + 1] : iau[i];
-        y = backward ? z[5 * (size_t)i + r] : rhs[5 * (size_t)i + r];
+        y = backward ? ld_coh(z + 5 * (size_t)i + r) : rhs[5 * (size_t)i + r];
         for (int kk = kb; kk < ke; kk++) {
             const int j = ja[kk];
             ok = wait_ready(ready, j, ep, flag) && ok;
@@ -426,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_trsv_sf(int nslots, const int *__res
             const TV *zj = z + 5 * (size_t)j;
             TV s = TV(0.0);
 #pragma unroll
-            for (int c = 0; c < 5; c++) s += (TV)A[c] * zj[c];
+            for (int c = 0; c < 5; c++) s += (TV)A[c] * ld_coh(zj + c);
             y = y - s;
         }
     }
@@ -444,60 +499,8 @@ __global__ __launch_bounds__(kBlock) void k_trsv_sf(int nslots, const int *__res
             y = s;
         }
     }
-    if (i >= 0) z[5 * (size_t)i + r] = y;
-    __threadfence();
+    if (i >= 0) st_coh(z + 5 * (size_t)i + r, y);
     if (i >= 0 && r == 0) publish_ready(ready, i, ep);
-    if (!ok) atomicOr(flag + 3, 1);
-}
-
-// Sync-free block ILU(0) factorization, same arithmetic as k_ilu0_level; one
-// row per thread, wavefronts padded to 64 rows of one level.
-template <class TM>
-__global__ __launch_bounds__(64) void k_ilu0_sf(int nslots, const int *__restrict__ slots, const int *__restrict__ ia,
-                                                const int *__restrict__ ja, const int *__restrict__ iau, TM *lu,
-                                                TM *dinv, int *flag, int *ready, const int *epoch) {
-    const int t = blockIdx.x * 64 + threadIdx.x;
-    const int i = t < nslots ? slots[t] : -1;
-    if (i < 0) return;
-    const int ep = epoch[0];
-    bool ok = true;
-    TM L[25], U[25], P[25];
-    for (int kk = ia[i]; kk < iau[i]; kk++) {
-        const int k = ja[kk];
-        ok = wait_ready(ready, k, ep, flag) && ok;
-        TM *Lk = lu + 25 * (size_t)kk;
-        for (int q = 0; q < 25; q++) U[q] = Lk[q];
-        bmul(U, dinv + 25 * (size_t)k, L);
-        for (int q = 0; q < 25; q++) Lk[q] = L[q];
-        int pp = iau[k] + 1, qq = kk + 1;
-        const int pe = ia[k + 1], qe = ia[i + 1];
-        while (pp < pe && qq < qe) {
-            const int cp = ja[pp], cq = ja[qq];
-            if (cp == cq) {
-                const TM *Up = lu + 25 * (size_t)pp;
-                for (int q = 0; q < 25; q++) U[q] = Up[q];
-                bmul(L, U, P);
-                TM *T = lu + 25 * (size_t)qq;
-                for (int q = 0; q < 25; q++) T[q] = T[q] - P[q];
-                pp++;
-                qq++;
-            } else if (cp < cq) {
-                pp++;
-            } else {
-                qq++;
-            }
-        }
-    }
-    double Dg[25], Di[25];
-    const TM *Dp = lu + 25 * (size_t)iau[i];
-    for (int q = 0; q < 25; q++) Dg[q] = (double)Dp[q];
-    if (!inv5(Dg, Di)) {
-        atomicAdd(flag + 2, 1);
-        for (int q = 0; q < 25; q++) Di[q] = 0.0;
-    }
-    for (int q = 0; q < 25; q++) dinv[25 * (size_t)i + q] = (TM)Di[q];
-    __threadfence();
-    publish_ready(ready, i, ep);
     if (!ok) atomicOr(flag + 3, 1);
 }
 
@@ -680,12 +683,20 @@ __global__ __launch_bounds__(kBlock) void k_mdot(size_t n, int nv, const TV *__r
     }
 }
 
-__global__ void k_mdot_final(int nparts, int nv, const double *part, double *out) {
-    const int k = threadIdx.x;
-    if (k >= nv) return;
+// Sum of the partials of dot k (block k): lane l adds partials l, l+64, ...
+// in order, then the 64 lane sums are combined by a fixed butterfly.
+__device__ __forceinline__ double wave_sum_parts(const double *part, int p0, int np, int k) {
+    const int lane = threadIdx.x;
     double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[(size_t)p * (kMaxKrylov + 1) + k];
-    out[k] = s;
+    for (int p = lane; p < np; p += 64) s += part[(size_t)(p0 + p) * (kMaxKrylov + 1) + k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    return s;
+}
+
+__global__ __launch_bounds__(64) void k_mdot_final(int nparts, int nv, const double *part, double *out) {
+    const int k = blockIdx.x;
+    const double s = wave_sum_parts(part, 0, nparts, k);
+    if (threadIdx.x == 0) out[k] = s;
 }
 
 // w -= sum_k h_k V_k (k ascending, as CGS in the oracle)
@@ -744,6 +755,92 @@ __global__ void k_fill(size_t n, double v, double *x) {
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) x[i] = v;
 }
 
+// ---- union of independent cases (batched solves) ------------------------
+// Per-case dot products: block (c, b) of the segment map reduces over case
+// c's range with the stride of an nb[c]-block launch on that case alone, so
+// each case's dots are bit-identical to k_mdot on the case by itself.
+template <class TV>
+__global__ __launch_bounds__(kBlock) void k_mdot_seg(const int2 *__restrict__ blk, const int *__restrict__ first,
+                                                     const int *__restrict__ off, const int *__restrict__ len, int nv,
+                                                     const TV *__restrict__ V, size_t ld, const TV *__restrict__ w,
+                                                     double *__restrict__ part) {
+    __shared__ double sh[kBlock];
+    const int2 cb = blk[blockIdx.x];
+    const size_t P = (size_t)(first[cb.x + 1] - first[cb.x]);
+    const size_t o = (size_t)off[cb.x], n = (size_t)len[cb.x];
+    double acc[kMaxKrylov + 1];
+    for (int k = 0; k < nv; k++) acc[k] = 0.0;
+    for (size_t i = cb.y * (size_t)kBlock + threadIdx.x; i < n; i += P * kBlock) {
+        const double wi = (double)w[o + i];
+        for (int k = 0; k < nv; k++) acc[k] += (double)V[ld * k + o + i] * wi;
+    }
+    for (int k = 0; k < nv; k++) {
+        sh[threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * (kMaxKrylov + 1) + k] = sh[0];
+        __syncthreads();
+    }
+}
+
+// out[c][slot + k] = sum of case c's partials, combined as k_mdot_final
+// (grid: dots x cases)
+__global__ __launch_bounds__(64) void k_mdot_seg_final(const int *first, int nv, const double *part, double *out,
+                                                       int slot) {
+    const int k = blockIdx.x, c = blockIdx.y;
+    const double s = wave_sum_parts(part, first[c], first[c + 1] - first[c], k);
+    if (threadIdx.x == 0) out[(size_t)c * kBOut + slot + k] = s;
+}
+
+// w -= sum_k h_{c,k} V_k with the coefficients of each entry's case
+template <class TV>
+__global__ __launch_bounds__(kBlock) void k_maxpy_seg(size_t n, int nv, const int *__restrict__ ecase,
+                                                      const TV *__restrict__ V, size_t ld,
+                                                      const double *__restrict__ out, int slot, TV *__restrict__ w) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const double *h = out + (size_t)ecase[i / 5] * kBOut + slot;
+        double wi = (double)w[i];
+        for (int k = 0; k < nv; k++) wi = wi - h[k] * (double)V[ld * k + i];
+        w[i] = (TV)wi;
+    }
+}
+
+// dst = src / sqrt(out[c][slot]) (Arnoldi normalisation, per case)
+template <class TS, class TD>
+__global__ __launch_bounds__(kBlock) void k_scale_dev_seg(size_t n, const int *__restrict__ ecase, const TS *src,
+                                                          TD *dst, const double *__restrict__ out, int slot) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const double denom = sqrt(out[(size_t)ecase[i / 5] * kBOut + slot]);
+        dst[i] = (TD)((double)src[i] / denom);
+    }
+}
+
+// dst = src / den[c] (first Krylov vector, per case)
+template <class TS, class TD>
+__global__ __launch_bounds__(kBlock) void k_scale_seg(size_t n, const int *__restrict__ ecase, const TS *src, TD *dst,
+                                                      const double *__restrict__ den) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        dst[i] = (TD)((double)src[i] / den[(size_t)ecase[i / 5] * (kMaxKrylov + 1)]);
+}
+
+// x += sum_{k < kdim_c} y_{c,k} Z_k
+template <class TV>
+__global__ __launch_bounds__(kBlock) void k_xupdate_seg(size_t n, const int *__restrict__ ecase,
+                                                        const TV *__restrict__ Z, size_t ld,
+                                                        const double *__restrict__ y, const int *__restrict__ kdim,
+                                                        double *__restrict__ x) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const int c = ecase[i / 5];
+        const double *yc = y + (size_t)c * (kMaxKrylov + 1);
+        double xi = x[i];
+        for (int k = 0; k < kdim[c]; k++) xi = xi + yc[k] * (double)Z[ld * k + i];
+        x[i] = xi;
+    }
+}
+
 int grid_cap(size_t n, int cap) {
     size_t g = (n + kBlock - 1) / kBlock;
     if (g < 1) g = 1;
@@ -779,17 +876,13 @@ namespace {
 
 template <class TM>
 int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
-    if (kind == CFD_PREC_ILU0 && m->sync_free) {
-        CFD_HIP(m, hipMemcpyAsync(lu, a, sizeof(TM) * 25 * (size_t)m->nnz, hipMemcpyDeviceToDevice, m->stream));
-        hipLaunchKernelGGL(k_epoch_bump, dim3(1), dim3(64), 0, m->stream, m->epoch);
-        hipLaunchKernelGGL(k_ilu0_sf<TM>, dim3((m->lower.n64 + 63) / 64), dim3(64), 0, m->stream, m->lower.n64,
-                           m->lower.slots64, m->ia, m->ja, m->iau, lu, dinv, m->flag, m->ready, m->epoch);
-    } else if (kind == CFD_PREC_ILU0) {
+    if (kind == CFD_PREC_ILU0) {
         CFD_HIP(m, hipMemcpyAsync(lu, a, sizeof(TM) * 25 * (size_t)m->nnz, hipMemcpyDeviceToDevice, m->stream));
         for (int l = 0; l < m->lower.nlevels; l++) {
             const int b0 = m->lower.ptr[l], nr = m->lower.ptr[l + 1] - b0;
-            hipLaunchKernelGGL(k_ilu0_level<TM>, dim3((nr + 63) / 64), dim3(64), 0, m->stream, nr,
-                               m->lower.rows + b0, m->ia, m->ja, m->iau, lu, dinv, m->flag);
+            hipLaunchKernelGGL(k_ilu0_level<TM>, dim3((nr + 47) / 48), dim3(kBlock), 0, m->stream, nr,
+                               m->lower.rows + b0, m->ia, m->ja, m->iau, m->ilu_moff, m->ilu_mqq, lu, dinv, lu, dinv,
+                               m->flag);
         }
     } else if (kind == CFD_PREC_BJACOBI) {
         hipLaunchKernelGGL((k_block_inv<TM, TM>), dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream,
@@ -833,7 +926,7 @@ void mdot(cfd_mesh *m, int nv, const TV *V, size_t ld, const TV *w, int off) {
     const size_t L = 5 * (size_t)m->N;
     const int np = grid_cap(L, kDotParts);
     hipLaunchKernelGGL(k_mdot<TV>, dim3(np), dim3(kBlock), 0, m->stream, L, nv, V, ld, w, m->red);
-    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(kMaxKrylov + 1), 0, m->stream, np, nv, m->red, m->redout + off);
+    hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(64), 0, m->stream, np, nv, m->red, m->redout + off);
 }
 
 int ensure_krylov(cfd_mesh *m, int mk, bool mixed) {
@@ -900,18 +993,241 @@ int enqueue_step(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, TV *V, 
         kg.exec.assign(mk, nullptr);
     }
     if (!kg.exec[j]) {
+        // Relaxed mode: other threads driving other meshes may call
+        // allocation or synchronous copy functions meanwhile.  If the capture
+        // still fails, this mesh falls back to direct launches of the same
+        // sequence.
         hipGraph_t g = nullptr;
-        CFD_HIP(m, hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-        arnoldi_step<TM, TV>(m, a, lu, dinv, V, Z, w, j);
-        const hipError_t e1 = hipStreamEndCapture(m->stream, &g);
-        if (e1 != hipSuccess) return cfd_hip_check(m->ctx, e1, "hipStreamEndCapture");
         hipGraphExec_t ex = nullptr;
-        const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (e2 != hipSuccess) return cfd_hip_check(m->ctx, e2, "hipGraphInstantiate");
+        hipError_t e = hipStreamBeginCapture(m->stream, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            arnoldi_step<TM, TV>(m, a, lu, dinv, V, Z, w, j);
+            e = hipStreamEndCapture(m->stream, &g);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (g) hipGraphDestroy(g);
+        if (e != hipSuccess || !ex) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(m->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                hipGraph_t g2 = nullptr;
+                (void)hipStreamEndCapture(m->stream, &g2);
+                if (g2) hipGraphDestroy(g2);
+            }
+            (void)hipGetLastError();
+            m->use_graphs = false;
+            drop_krylov_graphs(m);
+            arnoldi_step<TM, TV>(m, a, lu, dinv, V, Z, w, j);
+            CFD_HIP(m, hipGetLastError());
+            return CFD_OK;
+        }
         kg.exec[j] = ex;
     }
     CFD_HIP(m, hipGraphLaunch((hipGraphExec_t)kg.exec[j], m->stream));
+    return CFD_OK;
+}
+
+// Per-case dots of V_0..V_{nv-1} with w into m->bred[c][slot..] (union of cases).
+template <class TV>
+void mdot_seg(cfd_mesh *m, int nv, const TV *V, size_t ld, const TV *w, int slot) {
+    const SegMap &sg = m->seg_dot;
+    hipLaunchKernelGGL(k_mdot_seg<TV>, dim3(sg.nblocks), dim3(kBlock), 0, m->stream, sg.blk, sg.first, sg.off, sg.len,
+                       nv, V, ld, w, m->bpart);
+    hipLaunchKernelGGL(k_mdot_seg_final, dim3(nv, m->ncase), dim3(64), 0, m->stream, sg.first, nv, m->bpart, m->bred,
+                       slot);
+}
+
+// Arnoldi step j of the batched FGMRES: the operations of arnoldi_step with
+// per-case dots, coefficients and norms.
+template <class TM, class TV>
+void arnoldi_step_batch(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, TV *V, TV *Z, TV *w, int j) {
+    const size_t L = 5 * (size_t)m->N;
+    const int gN = grid_cap(L, 4096);
+    const int gS = (int)((L + kBlock - 1) / kBlock);
+    TV *vj = V + L * j, *zj = Z + L * j;
+    if (j > 0)
+        hipLaunchKernelGGL((k_scale_dev_seg<TV, TV>), dim3(gN), dim3(kBlock), 0, m->stream, L, m->ecase, w, vj,
+                           m->bred, 2 * (kMaxKrylov + 1));
+    apply_prec<TM, TV>(m, m->prec_kind, lu, dinv, vj, zj);
+    hipLaunchKernelGGL((k_spmv<TM, TV, TV>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a, zj, w,
+                       nullptr);
+    mdot_seg<TV>(m, j + 1, V, L, w, 0);
+    hipLaunchKernelGGL(k_maxpy_seg<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, m->ecase, V, L, m->bred, 0, w);
+    mdot_seg<TV>(m, j + 1, V, L, w, kMaxKrylov + 1);
+    hipLaunchKernelGGL(k_maxpy_seg<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, m->ecase, V, L, m->bred,
+                       kMaxKrylov + 1, w);
+    mdot_seg<TV>(m, 1, w, 0, w, 2 * (kMaxKrylov + 1));
+    hipMemcpyAsync(m->hbred, m->bred, sizeof(double) * kBOut * (size_t)m->ncase, hipMemcpyDeviceToHost, m->stream);
+}
+
+template <class TM, class TV>
+int enqueue_step_batch(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, TV *V, TV *Z, TV *w, int j, int mk) {
+    if (!m->use_graphs) {
+        arnoldi_step_batch<TM, TV>(m, a, lu, dinv, V, Z, w, j);
+        CFD_HIP(m, hipGetLastError());
+        return CFD_OK;
+    }
+    KrylovGraphs &kg = m->kgraph_b;
+    if (kg.key != (const void *)V || kg.prec != m->prec_kind || kg.m != mk || kg.lu != (const void *)lu) {
+        for (auto &g : kg.exec)
+            if (g) hipGraphExecDestroy((hipGraphExec_t)g);
+        kg.key = V;
+        kg.lu = lu;
+        kg.prec = m->prec_kind;
+        kg.m = mk;
+        kg.exec.assign(mk, nullptr);
+    }
+    if (!kg.exec[j]) {
+        // Relaxed mode: other threads driving other meshes may call
+        // allocation or synchronous copy functions meanwhile.  If the capture
+        // still fails, this mesh falls back to direct launches of the same
+        // sequence.
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipStreamBeginCapture(m->stream, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            arnoldi_step_batch<TM, TV>(m, a, lu, dinv, V, Z, w, j);
+            e = hipStreamEndCapture(m->stream, &g);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (g) hipGraphDestroy(g);
+        if (e != hipSuccess || !ex) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(m->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                hipGraph_t g2 = nullptr;
+                (void)hipStreamEndCapture(m->stream, &g2);
+                if (g2) hipGraphDestroy(g2);
+            }
+            (void)hipGetLastError();
+            m->use_graphs = false;
+            drop_krylov_graphs(m);
+            arnoldi_step_batch<TM, TV>(m, a, lu, dinv, V, Z, w, j);
+            CFD_HIP(m, hipGetLastError());
+            return CFD_OK;
+        }
+        kg.exec[j] = ex;
+    }
+    CFD_HIP(m, hipGraphLaunch((hipGraphExec_t)kg.exec[j], m->stream));
+    return CFD_OK;
+}
+
+// Batched restarted FGMRES(m): every case of the union runs the algorithm of
+// fgmres() on its own rows, in lockstep; a case that has converged (or is
+// not being solved) keeps riding along but its results are not used.
+template <class TM, class TV>
+int fgmres_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM *lu, const TM *dinv, TV *V, TV *Z,
+                 TV *w, const std::vector<char> &solve, std::vector<int> &iters, std::vector<double> &fres) {
+    const int K = m->ncase, mk = o->krylov_m;
+    const size_t L = 5 * (size_t)m->N;
+    const int gN = grid_cap(L, 4096);
+    const int gS = (int)((L + kBlock - 1) / kBlock);
+    const size_t ldc = kMaxKrylov + 1;
+    hipLaunchKernelGGL(k_fill, dim3(gN), dim3(kBlock), 0, m->stream, L, 0.0, m->x);
+    CFD_HIP(m, hipMemcpyAsync(m->r, m->b, sizeof(double) * L, hipMemcpyDeviceToDevice, m->stream));
+    mdot_seg<double>(m, 1, m->r, 0, m->r, 0);
+    CFD_HIP(m, hipMemcpyAsync(m->hbred, m->bred, sizeof(double) * kBOut * K, hipMemcpyDeviceToHost, m->stream));
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    std::vector<double> beta(K), tol(K), res(K);
+    std::vector<char> live(K), cyc(K), step(K);
+    std::vector<int> kdim(K);
+    std::vector<double> H((size_t)K * (mk + 1) * mk, 0.0), cs((size_t)K * mk), sn((size_t)K * mk),
+        gv((size_t)K * (mk + 1)), hcol(mk + 2), y((size_t)mk);
+    iters.assign(K, 0);
+    fres.assign(K, 0.0);
+    for (int c = 0; c < K; c++) {
+        beta[c] = sqrt(m->hbred[(size_t)c * kBOut]);
+        tol[c] = o->rtol * beta[c];
+        res[c] = beta[c];
+        live[c] = solve[c] && beta[c] > tol[c] && beta[c] > 0.0;
+    }
+    for (int cycle = 0; cycle <= o->max_restarts; cycle++) {
+        bool any = false;
+        for (int c = 0; c < K; c++) {
+            cyc[c] = step[c] = live[c];
+            kdim[c] = 0;
+            any = any || live[c];
+            m->hbcoef[(size_t)c * ldc] = live[c] ? beta[c] : 1.0;
+            for (int i = 0; i <= mk; i++) gv[(size_t)c * (mk + 1) + i] = 0.0;
+            gv[(size_t)c * (mk + 1)] = beta[c];
+        }
+        if (!any) break;
+        CFD_HIP(m, hipMemcpyAsync(m->bcoef, m->hbcoef, sizeof(double) * ldc * K, hipMemcpyHostToDevice, m->stream));
+        hipLaunchKernelGGL((k_scale_seg<double, TV>), dim3(gN), dim3(kBlock), 0, m->stream, L, m->ecase, m->r, V,
+                           m->bcoef);
+        for (int j = 0; j < mk; j++) {
+            int rc = enqueue_step_batch<TM, TV>(m, a, lu, dinv, V, Z, w, j, mk);
+            if (rc) return rc;
+            CFD_HIP(m, hipStreamSynchronize(m->stream));
+            bool stepping = false;
+            for (int c = 0; c < K; c++) {
+                if (!step[c]) continue;
+                const double *hr = m->hbred + (size_t)c * kBOut;
+                double *Hc = H.data() + (size_t)c * (mk + 1) * mk, *csc = cs.data() + (size_t)c * mk,
+                       *snc = sn.data() + (size_t)c * mk, *gvc = gv.data() + (size_t)c * (mk + 1);
+                for (int k = 0; k <= j; k++) hcol[k] = 0.0 + hr[k];
+                for (int k = 0; k <= j; k++) hcol[k] += hr[kMaxKrylov + 1 + k];
+                const double hn = sqrt(hr[2 * (kMaxKrylov + 1)]);
+                hcol[j + 1] = hn;
+                for (int k = 0; k < j; k++) {
+                    const double t = csc[k] * hcol[k] + snc[k] * hcol[k + 1];
+                    hcol[k + 1] = -snc[k] * hcol[k] + csc[k] * hcol[k + 1];
+                    hcol[k] = t;
+                }
+                const double den = sqrt(hcol[j] * hcol[j] + hcol[j + 1] * hcol[j + 1]);
+                csc[j] = hcol[j] / den;
+                snc[j] = hcol[j + 1] / den;
+                hcol[j] = den;
+                hcol[j + 1] = 0.0;
+                gvc[j + 1] = -snc[j] * gvc[j];
+                gvc[j] = csc[j] * gvc[j];
+                for (int k = 0; k <= j; k++) Hc[(size_t)k * mk + j] = hcol[k];
+                iters[c]++;
+                kdim[c] = j + 1;
+                res[c] = fabs(gvc[j + 1]);
+                if (!(res[c] == res[c]))
+                    return cfd_set_error(m->ctx, CFD_E_NONFINITE, "batched FGMRES: non-finite residual in case %d", c);
+                if (res[c] <= tol[c] || hn == 0.0) step[c] = false;
+                stepping = stepping || step[c];
+            }
+            if (!stepping) break;
+        }
+        for (int c = 0; c < K; c++) {
+            double *yc = m->hbcoef + (size_t)c * ldc;
+            for (int k = 0; k <= kMaxKrylov; k++) yc[k] = 0.0;
+            m->hbkdim[c] = cyc[c] ? kdim[c] : 0;
+            if (!cyc[c]) continue;
+            const double *Hc = H.data() + (size_t)c * (mk + 1) * mk, *gvc = gv.data() + (size_t)c * (mk + 1);
+            for (int k = kdim[c] - 1; k >= 0; k--) {
+                double t = gvc[k];
+                for (int q = k + 1; q < kdim[c]; q++) t -= Hc[(size_t)k * mk + q] * y[q];
+                y[k] = t / Hc[(size_t)k * mk + k];
+            }
+            for (int k = 0; k < kdim[c]; k++) yc[k] = y[k];
+        }
+        CFD_HIP(m, hipMemcpyAsync(m->bcoef, m->hbcoef, sizeof(double) * ldc * K, hipMemcpyHostToDevice, m->stream));
+        CFD_HIP(m, hipMemcpyAsync(m->bkdim, m->hbkdim, sizeof(int) * K, hipMemcpyHostToDevice, m->stream));
+        hipLaunchKernelGGL(k_xupdate_seg<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, m->ecase, Z, L, m->bcoef,
+                           m->bkdim, m->x);
+        any = false;
+        for (int c = 0; c < K; c++) {
+            if (cyc[c] && res[c] <= tol[c]) live[c] = false;
+            any = any || live[c];
+        }
+        if (!any) break;
+        // true residual for the restart
+        hipLaunchKernelGGL((k_spmv<TM, double, double>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a,
+                           m->x, m->r, m->b);
+        mdot_seg<double>(m, 1, m->r, 0, m->r, 0);
+        CFD_HIP(m, hipMemcpyAsync(m->hbred, m->bred, sizeof(double) * kBOut * K, hipMemcpyDeviceToHost, m->stream));
+        CFD_HIP(m, hipStreamSynchronize(m->stream));
+        for (int c = 0; c < K; c++) {
+            if (!live[c]) continue;
+            beta[c] = sqrt(m->hbred[(size_t)c * kBOut]);
+            res[c] = beta[c];
+            live[c] = cycle < o->max_restarts && beta[c] > tol[c] && beta[c] > 0.0;
+        }
+    }
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    for (int c = 0; c < K; c++) fres[c] = res[c];
     return CFD_OK;
 }
 
@@ -1000,11 +1316,18 @@ int fgmres(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM *lu, c
 
 }  // namespace
 
+namespace {
+int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o);
+int read_solve_flags(cfd_mesh *m);
+}  // namespace
+
 void drop_krylov_graphs(cfd_mesh *m) {
-    for (auto &g : m->kgraph.exec)
-        if (g) hipGraphExecDestroy((hipGraphExec_t)g);
-    m->kgraph.exec.clear();
-    m->kgraph.key = nullptr;
+    for (KrylovGraphs *kg : {&m->kgraph, &m->kgraph_b}) {
+        for (auto &g : kg->exec)
+            if (g) hipGraphExecDestroy((hipGraphExec_t)g);
+        kg->exec.clear();
+        kg->key = nullptr;
+    }
 }
 
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st) {
@@ -1034,7 +1357,7 @@ int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st
                 const int np = grid_cap(m->N, kDotParts);
                 hipLaunchKernelGGL(k_vsum_partial, dim3(np), dim3(kBlock), 0, m->stream, (size_t)m->N, m->rowbuf,
                                    m->red);
-                hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(kMaxKrylov + 1), 0, m->stream, np, 1, m->red,
+                hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(64), 0, m->stream, np, 1, m->red,
                                    m->redout);
                 CFD_HIP(m, hipMemcpyAsync(m->hred, m->redout, sizeof(double), hipMemcpyDeviceToHost, m->stream));
                 CFD_HIP(m, hipStreamSynchronize(m->stream));
@@ -1065,7 +1388,36 @@ int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st
         CFD_HIP(m, hipStreamSynchronize(m->stream));
         return CFD_OK;
     }
-    // FGMRES
+    int rc = prepare_fgmres(m, o);
+    if (rc) return rc;
+    if (o->mixed)
+        rc = fgmres<float, float>(m, o, m->af, m->luf, m->dinvf, m->Vf, m->Zf, m->wf, st);
+    else
+        rc = fgmres<double, double>(m, o, m->a, m->lu, m->dinv, m->V, m->Z, m->w, st);
+    if (rc) return rc;
+    return read_solve_flags(m);
+}
+
+int run_linsolve_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const std::vector<char> &solve,
+                       std::vector<int> &iters, std::vector<double> &final_res) {
+    if (m->ncase < 1) return cfd_set_error(m->ctx, CFD_E_STATE, "batched solve on a mesh that is not a union of cases");
+    if (!m->jac_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "batched linsolve: no Jacobian assembled");
+    if (o->solver != CFD_SOLVER_FGMRES) return cfd_set_error(m->ctx, CFD_E_ARG, "batched solves use FGMRES");
+    CFD_HIP(m, hipMemsetAsync(m->flag, 0, 4 * sizeof(int), m->stream));
+    int rc = prepare_fgmres(m, o);
+    if (rc) return rc;
+    if (o->mixed)
+        rc = fgmres_batch<float, float>(m, o, m->af, m->luf, m->dinvf, m->Vf, m->Zf, m->wf, solve, iters, final_res);
+    else
+        rc = fgmres_batch<double, double>(m, o, m->a, m->lu, m->dinv, m->V, m->Z, m->w, solve, iters, final_res);
+    if (rc) return rc;
+    return read_solve_flags(m);
+}
+
+namespace {
+
+// Jacobian precision check, Krylov workspace and preconditioner for FGMRES.
+int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
     if (o->krylov_m < 1 || o->krylov_m > kMaxKrylov)
         return cfd_set_error(m->ctx, CFD_E_ARG, "krylov_m must be in [1, %d]", kMaxKrylov);
     const bool mixed = o->mixed != 0;
@@ -1088,14 +1440,16 @@ int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st
         if (rc) return rc;
         m->prec_valid = true;
     }
-    if (mixed)
-        rc = fgmres<float, float>(m, o, m->af, m->luf, m->dinvf, m->Vf, m->Zf, m->wf, st);
-    else
-        rc = fgmres<double, double>(m, o, m->a, m->lu, m->dinv, m->V, m->Z, m->w, st);
-    if (rc) return rc;
+    return CFD_OK;
+}
+
+// Error flags raised by the preconditioner and the sweeps.
+int read_solve_flags(cfd_mesh *m) {
     CFD_HIP(m, hipMemcpyAsync(m->hflag, m->flag, 4 * sizeof(int), hipMemcpyDeviceToHost, m->stream));
     CFD_HIP(m, hipStreamSynchronize(m->stream));
     if (m->hflag[2]) return cfd_set_error(m->ctx, CFD_E_SINGULAR, "zero pivot in %d diagonal blocks", m->hflag[2]);
     if (m->hflag[3]) return cfd_set_error(m->ctx, CFD_E_HIP, "sync-free triangular sweep exceeded its wait bound");
     return CFD_OK;
 }
+
+}  // namespace

@@ -83,6 +83,10 @@ struct BcArgs {
     T *qa;
     T g, inf_rho, inf_u, inf_v, inf_w, inf_p, out_p;
     int *flag;
+    // union of independent cases: freestream of each case (rho, u, v, w, p,
+    // outflow p) and the case of each boundary edge; NULL for one case
+    const double *case_inf;
+    const int *bcase;
 };
 
 template <class T>
@@ -106,8 +110,17 @@ __global__ __launch_bounds__(kBlock) void k_bc(BcArgs<T> a) {
     for (int k = 0; k < 5; k++) Qa[k] = T(0.5) * (QL[k] + QR[k]);
     eos_face(g, Qa, nx, ny, nz, Z);
     const T l1 = Z.ubar, l4 = l1 + Z.c, l5 = l1 - Z.c;
-    const T infP = a.inf_p;
-    T pb = infP, rb = a.inf_rho, ub = a.inf_u, vb = a.inf_v, wb = a.inf_w;
+    T inf_rho = a.inf_rho, inf_u = a.inf_u, inf_v = a.inf_v, inf_w = a.inf_w, infP = a.inf_p, out_p = a.out_p;
+    if (a.case_inf) {
+        const double *f = a.case_inf + 8 * (size_t)a.bcase[e];
+        inf_rho = (T)f[0];
+        inf_u = (T)f[1];
+        inf_v = (T)f[2];
+        inf_w = (T)f[3];
+        infP = (T)f[4];
+        out_p = (T)f[5];
+    }
+    T pb = infP, rb = inf_rho, ub = inf_u, vb = inf_v, wb = inf_w;
     int type = a.btype[e], inst = type;
     bool bad = false;
     switch (type) {
@@ -143,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_bc(BcArgs<T> a) {
         break;
     case 5:
     case 9:
-        pb = (inst == 5) ? a.out_p : infP;
+        pb = (inst == 5) ? out_p : infP;
         rb = I.rho + (pb - I.p) / (Z.c * Z.c);
         ub = I.u - nx * (pb - I.p) / (Z.rho * Z.c);
         vb = I.v - ny * (pb - I.p) / (Z.rho * Z.c);
@@ -158,12 +171,12 @@ __global__ __launch_bounds__(kBlock) void k_bc(BcArgs<T> a) {
         wb = I.w;
         break;
     case 8: {
-        const T temp = nx * (a.inf_u - I.u) + ny * (a.inf_v - I.v) + nz * (a.inf_w - I.w);
+        const T temp = nx * (inf_u - I.u) + ny * (inf_v - I.v) + nz * (inf_w - I.w);
         pb = T(0.5) * (infP + I.p - Z.rho * Z.c * temp);
-        rb = a.inf_rho + (pb - infP) / (Z.c * Z.c);
-        ub = a.inf_u + nx * (pb - infP) / (Z.rho * Z.c);
-        vb = a.inf_v + ny * (pb - infP) / (Z.rho * Z.c);
-        wb = a.inf_w + nz * (pb - infP) / (Z.rho * Z.c);
+        rb = inf_rho + (pb - infP) / (Z.c * Z.c);
+        ub = inf_u + nx * (pb - infP) / (Z.rho * Z.c);
+        vb = inf_v + ny * (pb - infP) / (Z.rho * Z.c);
+        wb = inf_w + nz * (pb - infP) / (Z.rho * Z.c);
         break;
     }
     default: break;  // 4, 6, 10: freestream
@@ -787,7 +800,8 @@ template <class T>
 int launch_bc(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, int iter) {
     if (m->B == 0) return CFD_OK;
     BcArgs<T> a{m->N, m->E, m->B, iter, p->zpg_iter, m->bnode, m->btype, b.sa, b.qa, (T)p->gamma, (T)p->inf_rho,
-                (T)p->inf_u, (T)p->inf_v, (T)p->inf_w, (T)p->inf_p, (T)p->outflow_p, m->flag};
+                (T)p->inf_u, (T)p->inf_v, (T)p->inf_w, (T)p->inf_p, (T)p->outflow_p, m->flag, m->case_inf,
+                m->bcase};
     hipLaunchKernelGGL(k_bc<T>, dim3((m->B + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, a);
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;

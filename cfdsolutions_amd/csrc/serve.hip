@@ -130,6 +130,10 @@ cfd_completion run_one(cfd_server *s, const Pending &p, Resident *slot) {
     int rc;
     if (!slot->mesh) {
         rc = cfd_mesh_create(s->ctx, p.req.mesh, &p.req.params, &slot->mesh);
+        // Several worker threads drive their meshes at once; graph capture on
+        // one thread can be invalidated by another thread's allocations, so
+        // served meshes launch their Krylov steps directly.
+        if (!rc) slot->mesh->use_graphs = false;
     } else {
         c.reused = 1;
         rc = cfd_mesh_reset(slot->mesh, &p.req.params);

@@ -237,6 +237,18 @@ int cfd_solve_steady(cfd_mesh *mesh, const cfd_params *p, const cfd_solve_opts *
  * another solve without rebuilding its structure. */
 int cfd_mesh_reset(cfd_mesh *mesh, const cfd_params *p);
 
+/* ---- batched small cases ---------------------------------------------------
+ * No counterpart in the reference.  Solves ncase independent steady cases in
+ * one union mesh: every kernel launch (and every captured Krylov step)
+ * covers all cases, while each case keeps its own freestream, residual norm,
+ * Krylov dots and rotations, convergence test and forces.  Each case's
+ * iterates are bit-identical to cfd_solve_steady on that case alone.  All
+ * cases must share the scheme fields of cfd_params (everything except the
+ * freestream) and the solver fields of cfd_solve_opts (everything except
+ * alpha_deg); results[c].seconds is the time of the whole batch. */
+int cfd_batch_solve(cfd_ctx *ctx, int32_t ncase, const cfd_mesh_desc *meshes, const cfd_params *params,
+                    const cfd_solve_opts *opts, cfd_solve_result *results);
+
 /* ---- serving -------------------------------------------------------------
  * No counterpart in the reference, which solves one case per process.  A
  * server owns worker threads that run queued steady solves concurrently on

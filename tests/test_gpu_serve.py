@@ -127,6 +127,14 @@ def test_server_results_match_alone(ctx):
         got = srv.poll(wait=60.0)
         assert got, "server made no progress in 60 s"
         done += got
+    # with every mesh idle, a request on a known geometry runs on its resident mesh
+    k, d, flux, order, mach, alpha = REQS[0]
+    srv.submit(len(REQS), k, meshes[k], cfd.make_params(mach=mach, alpha_deg=alpha, flux=flux, order=order),
+               want_state=True, alpha_deg=alpha, **SOLVE)
+    again = srv.poll(wait=60.0)
+    assert len(again) == 1 and again[0]["reused"]
+    again[0]["id"] = 0
+    done += again
     st = srv.stats()
     srv.close()
     assert st["reused"] >= 1
@@ -169,3 +177,31 @@ def test_server_rejects_request_larger_than_budget(ctx):
         srv.submit(0, 1, m, p, **SOLVE)
     assert e.value.code == cfd.E_OOM
     srv.close()
+
+
+def test_batch_solve_matches_alone(ctx):
+    """Cases solved together in one union mesh reproduce, bit for bit, the iteration counts, residual norms and
+    forces of each case solved alone."""
+    cases = [((12, 4, 4), 0.08, 0.5, 0.0), ((16, 6, 4), 0.12, 0.6, 2.0), ((14, 5, 5), 0.1, 0.45, -1.0),
+             ((12, 4, 4), 0.1, 0.7, 3.0)]
+    meshes = [cmesh.bump_channel(*d, bump_h=h) for d, h, _, _ in cases]
+    params = [cfd.make_params(mach=mach, alpha_deg=a, flux="roe", order=1) for _, _, mach, a in cases]
+    kw = dict(SOLVE)
+    alone = []
+    for m, p, (_, _, _, a) in zip(meshes, params, cases):
+        dm = cfd.DeviceMesh(ctx, m, p)
+        r, _ = dm.solve_steady(p, alpha_deg=a, **kw)
+        alone.append(r)
+        dm.close()
+    rc, batch = cfd.batch_solve(ctx, meshes, params, alphas=[a for *_, a in cases], **kw)
+    assert rc == 0, lib_error(ctx)
+    for r0, r1 in zip(alone, batch):
+        assert r1.converged and r1.status == 0
+        assert (r1.iterations, r1.krylov_total) == (r0.iterations, r0.krylov_total)
+        assert r1.first_rms == r0.first_rms and r1.final_rms == r0.final_rms
+        assert list(r1.force) == list(r0.force)
+        assert r1.lift == r0.lift and r1.drag == r0.drag
+
+
+def lib_error(ctx):
+    return cfd.lib().cfd_last_error(ctx.h).decode()
