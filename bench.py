@@ -27,6 +27,11 @@ import os
 import sys
 import time
 
+# The serving leg overlaps many solves, one HIP stream each; HIP maps a
+# process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set), so
+# ask for enough queues before the runtime initialises.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -151,6 +156,20 @@ def cpu_baseline(m, p, q):
     return time.perf_counter() - t0, rc, rd
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the 160^3 benchmark, from the committed rocprofv3 counter summary
+    (2 x FETCH_SIZE + WRITE_SIZE, calibrated against 1 GiB device copies); None if it is not there."""
+    import re
+    path = os.path.join(ROOT, "profiles", "residual_cube160_fp64_hbm_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"- %s .*= ([0-9.]+) GB against" % re.escape(kernel), line)
+        if m:
+            return int(float(m.group(1)) * 1e9)
+    return None
+
+
 def bump_dims(n_nodes):
     """Bump-channel hex counts (3n, n, n) whose node count is closest to n_nodes."""
     n = max(4, int(round((n_nodes / 3.0) ** (1.0 / 3.0))))
@@ -163,6 +182,8 @@ SOLVE_KW = dict(max_iter=400, tol=1e-10, krylov_m=30, max_restarts=4, rtol=1e-3)
 def solve_leg(ctx):
     """Bump channel 48x16x16 (14,161 nodes), first-order Roe, M 0.5, AoA 0, backward Euler, CFL 1 -> 1000,
     FGMRES(30) + ILU(0), to RMS 1e-10: plain FP64 vs mixed precision, each alone on the GPU."""
+    from cfdsolutions_amd import cfd, mesh
+
     m = mesh.bump_channel(48, 16, 16, bump_h=0.1)
     p = cfd.make_params(mach=0.5, alpha_deg=0.0, flux="roe", order=1)
     out = {}
@@ -197,6 +218,8 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
     path solves them one at a time (meshes resident); the server solves them in mixed precision with
     continuous batching, first as a burst (capacity) and then as a Poisson stream at 80% of that capacity
     (latency)."""
+    from cfdsolutions_amd import cfd, mesh
+
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_meshes))
     heights = rng.uniform(0.05, 0.15, n_meshes)
@@ -219,6 +242,8 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
                                         **SOLVE_KW)
         t64 += out.seconds
         res64.append(out)
+        print("bench: serve baseline request %d/%d: %d nodes, %.2f s" % (len(res64), n_req, pool[r[0]].n_node,
+                                                                         out.seconds), file=sys.stderr, flush=True)
     for d in dms:
         d.close()
     print("bench: serve baseline fp64 %d solves in %.2f s" % (n_req, t64), file=sys.stderr, flush=True)
@@ -242,7 +267,11 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
                            **SOLVE_KW)
                 k += 1
             wait = 0.0 if k >= n_req else max(0.0, arrivals[k] - (time.perf_counter() - t0))
-            done += srv.poll(wait=min(wait, 0.05) if k < n_req else 1.0)
+            got = srv.poll(wait=min(wait, 0.05) if k < n_req else 1.0)
+            for c in got:
+                print("bench: served request %d (%.2f s latency)" % (c["id"], c["t_done"] - c["t_submit"]),
+                      file=sys.stderr, flush=True)
+            done += got
         t_end = max(c["t_done"] for c in done)
         lat = np.array([c["t_done"] - c["t_submit"] for c in done])
         return done, t_end - t_start, lat
@@ -282,6 +311,44 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
     return res
 
 
+def batch_leg(ctx, n_case, nmin=8000, nmax=40000, seed=4):
+    """Bump channels of log-uniform size in [nmin, nmax] nodes, bump height U(0.05, 0.15), Mach U(0.3, 0.8), AoA
+    U(-2, 4) deg, first-order Roe, to RMS 1e-10.  The plain FP64 GPU path solves them one at a time (meshes
+    resident); cfd_batch_solve solves all of them in one union mesh with the mixed-precision linear solve."""
+    from cfdsolutions_amd import cfd, mesh
+
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_case))
+    meshes = [mesh.bump_channel(*bump_dims(n), bump_h=h) for n, h in zip(sizes, rng.uniform(0.05, 0.15, n_case))]
+    machs, alphas = rng.uniform(0.3, 0.8, n_case), rng.uniform(-2.0, 4.0, n_case)
+    params = [cfd.make_params(mach=mk, alpha_deg=a, flux="roe", order=1) for mk, a in zip(machs, alphas)]
+    t64, r64 = 0.0, []
+    for m, p, a in zip(meshes, params, alphas):
+        dm = cfd.DeviceMesh(ctx, m, p)
+        r, _ = dm.solve_steady(p, alpha_deg=a, precond=cfd.PREC_ILU0, mixed=False, check=False, **SOLVE_KW)
+        t64 += r.seconds
+        r64.append(r)
+        dm.close()
+        print("bench: batch baseline case %d/%d: %d nodes, %.2f s" % (len(r64), n_case, m.n_node, r.seconds),
+              file=sys.stderr, flush=True)
+    rc, rb = cfd.batch_solve(ctx, meshes, params, alphas=list(alphas), precond=cfd.PREC_ILU0, mixed=True, **SOLVE_KW)
+    tb = rb[0].seconds if rc == 0 else float("nan")
+    ferr = max(float(np.max(np.abs(np.array(b.force) - np.array(a.force))) / np.max(np.abs(a.force)))
+               for a, b in zip(r64, rb)) if rc == 0 else float("nan")
+    nodes = [m.n_node for m in meshes]
+    res = {"workload": "%d bump channels of %d-%d nodes (log-uniform), 1st-order Roe, M 0.3-0.8, AoA -2..4, "
+                       "RMS < 1e-10" % (n_case, min(nodes), max(nodes)),
+           "fp64_sequential_solves_per_s": n_case / t64,
+           "batch_mixed_solves_per_s": n_case / tb,
+           "batch_vs_fp64": t64 / tb,
+           "batch_seconds": tb, "status": int(rc),
+           "converged": int(sum(1 for b in rb if b.converged)),
+           "force_rel_err_mixed_vs_fp64": ferr}
+    print("bench: batch %d cases fp64 sequential %.2f solves/s, batch mixed %.2f solves/s (%.2fx), force err %.2e" %
+          (n_case, n_case / t64, n_case / tb, t64 / tb, ferr), file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,12 +359,14 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--only", action="store_true", help="run only the headline variant (for profiling)")
     ap.add_argument("--no-solves", action="store_true", help="skip the steady-solve and serving legs")
-    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--legs", type=str, default="residual,solve,serve,batch",
+                    help="comma list of legs to run: residual, solve, serve, batch")
+    ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-meshes", type=int, default=6)
-    ap.add_argument("--serve-nodes", type=str, default="20000,150000", help="min,max nodes of the serving meshes")
-    ap.add_argument("--serve-inflight", type=int, default=6)
-    ap.add_argument("--batch-cases", type=int, default=0,
-                    help="also run N small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) through the server")
+    ap.add_argument("--serve-nodes", type=str, default="10000,50000", help="min,max nodes of the serving meshes")
+    ap.add_argument("--serve-inflight", type=int, default=12)
+    ap.add_argument("--batch-cases", type=int, default=32,
+                    help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (scale factor for HBM counters)")
     args = ap.parse_args()
@@ -321,6 +390,21 @@ def main():
         torch.cuda.synchronize()
         print("bench: calibration copy of %d MiB done" % args.calibrate_copy, file=sys.stderr, flush=True)
         del a, b
+
+    legs = set(x.strip() for x in args.legs.split(",") if x.strip())
+    if "residual" not in legs:
+        ctx = cfd.Context(0)
+        out = {"metric": "converged solves/s & p99 latency (mixed vs FP64 GPU); residual-eval GB/s", "legs": sorted(legs)}
+        if "solve" in legs:
+            out["solves"] = solve_leg(ctx)
+        if "serve" in legs:
+            nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
+            out["serving"] = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight)
+        if "batch" in legs and args.batch_cases:
+            out["batched_small_cases"] = batch_leg(ctx, args.batch_cases)
+        ctx.close()
+        print(json.dumps(out))
+        return
 
     t0 = time.perf_counter()
     m = mesh.jittered_cube(args.n, jitter=0.2, seed=20261202)
@@ -376,18 +460,19 @@ def main():
                    parity_max_rel_diff=float(rel))
         print("bench: cpu oracle %.2f s, GPU-vs-oracle max rel diff %.3e" % (secs, rel), file=sys.stderr, flush=True)
     dm.close()
-    solves = serving = None
+    solves = serving = batched = None
     if not args.no_solves and not args.only:
         del dm
-        solves = solve_leg(ctx)
-        nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
-        serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight)
-        if args.batch_cases:
-            serving["small_cases"] = serve_leg(ctx, args.batch_cases, min(args.batch_cases, 16), 8000, 40000,
-                                               args.serve_inflight, seed=7, mach=(0.3, 0.8), aoa=(-2.0, 4.0),
-                                               fluxes=("roe",), stream=False)
+        if "solve" in legs:
+            solves = solve_leg(ctx)
+        if "serve" in legs:
+            nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
+            serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight)
+        if "batch" in legs and args.batch_cases:
+            batched = batch_leg(ctx, args.batch_cases)
     ctx.close()
     r_achieved = variants["roe-fp64-GL+R2"]["r_gbs"]
+    traffic_bytes = measured_traffic("k_residual") if args.n == 160 else None
     out = {
         "metric": "converged solves/s & p99 latency (mixed vs FP64 GPU); residual-eval GB/s",
         "value": round(head["gbs"], 2),
@@ -407,12 +492,15 @@ def main():
                    "nodes": N, "edges": E, "ghosts": B, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_residual<double,Roe,2nd order,limited>",
                      "achieved": round(r_achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(r_achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(r_achieved / PEAK_HBM_GBS, 4), "traffic": traffic_bytes,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/residual_cube160_fp64_hbm_counters.md"
+                     if traffic_bytes else None,
                      "algorithmic_bytes_per_launch": bytes64["R2split"],
                      "avg_launch_ms": round(variants["roe-fp64-GL+R2"]["r_ms"], 4)},
         "cpu_baseline": cpu,
         "solves": solves,
         "serving": serving,
+        "batched_small_cases": batched,
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},
     }
