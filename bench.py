@@ -186,38 +186,39 @@ def solve_leg(ctx):
 
     m = mesh.bump_channel(48, 16, 16, bump_h=0.1)
     p = cfd.make_params(mach=0.5, alpha_deg=0.0, flux="roe", order=1)
+    runs = {"fp64": (cfd.PREC_ILU0, False), "mixed": (cfd.PREC_ILU0, True),
+            "mixed_multicolour_ilu": (cfd.PREC_ILU0_MC, True)}
     out = {}
-    for mixed in (False, True):
+    for name, (precond, mixed) in runs.items():
         dm = cfd.DeviceMesh(ctx, m, p)
-        r, _ = dm.solve_steady(p, precond=cfd.PREC_ILU0, mixed=mixed, **SOLVE_KW)  # warm-up (JIT, allocation)
+        r, _ = dm.solve_steady(p, precond=precond, mixed=mixed, **SOLVE_KW)  # warm-up (allocation)
         dm.reset(p)
-        r, hist = dm.solve_steady(p, precond=cfd.PREC_ILU0, mixed=mixed, **SOLVE_KW)
-        out[mixed] = (r, dm.get(cfd.FIELD_Q))
+        r, hist = dm.solve_steady(p, precond=precond, mixed=mixed, **SOLVE_KW)
+        out[name] = (r, dm.get(cfd.FIELD_Q))
         dm.close()
-    (r0, q0), (r1, q1) = out[False], out[True]
-    rel_f = float(np.max(np.abs(np.array(r1.force) - np.array(r0.force))) / np.max(np.abs(r0.force)))
-    rel_q = float(np.max(np.abs(q1 - q0)) / np.max(np.abs(q0)))
-    res = {"workload": "bump channel 48x16x16 (%d nodes), 1st-order Roe, M 0.5, FGMRES(30)+ILU(0), RMS < 1e-10"
-                       % m.n_node,
-           "fp64": {"solves_per_s": 1.0 / r0.seconds, "seconds": r0.seconds, "newton": r0.iterations,
-                    "krylov": r0.krylov_total, "converged": bool(r0.converged)},
-           "mixed": {"solves_per_s": 1.0 / r1.seconds, "seconds": r1.seconds, "newton": r1.iterations,
-                     "krylov": r1.krylov_total, "converged": bool(r1.converged),
-                     "force_rel_err_vs_fp64": rel_f, "state_rel_err_vs_fp64": rel_q},
-           "speedup_mixed": r0.seconds / r1.seconds}
-    print("bench: solve 48x16x16 fp64 %.3f s (%d/%d), mixed %.3f s (%d/%d), force err %.2e" % (
-        r0.seconds, r0.iterations, r0.krylov_total, r1.seconds, r1.iterations, r1.krylov_total, rel_f),
-        file=sys.stderr, flush=True)
+    r0, q0 = out["fp64"]
+    res = {"workload": "bump channel 48x16x16 (%d nodes), 1st-order Roe, M 0.5, FGMRES(30)+ILU(0), CFL 1 -> 1000, "
+                       "RMS < 1e-10" % m.n_node}
+    for name, (r, q) in out.items():
+        res[name] = {"solves_per_s": 1.0 / r.seconds, "seconds": r.seconds, "newton": r.iterations,
+                     "krylov": r.krylov_total, "converged": bool(r.converged),
+                     "force_rel_err_vs_fp64": float(np.max(np.abs(np.array(r.force) - np.array(r0.force))) /
+                                                    np.max(np.abs(r0.force))),
+                     "state_rel_err_vs_fp64": float(np.max(np.abs(q - q0)) / np.max(np.abs(q0))),
+                     "speedup_vs_fp64": r0.seconds / r.seconds}
+        print("bench: solve 48x16x16 %-22s %.3f s (%d newton / %d krylov), force err %.2e" % (
+            name, r.seconds, r.iterations, r.krylov_total, res[name]["force_rel_err_vs_fp64"]),
+            file=sys.stderr, flush=True)
     return res
 
 
 def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0.3, 0.85), aoa=(-2.0, 6.0),
               fluxes=("roe", "hllc"), stream=True):
     """Steady-solve requests on bump channels of log-uniform size in [nmin, nmax] nodes, bump height U(0.05,
-    0.15), Mach U(0.3, 0.85), AoA U(-2, 6) deg, Roe or HLLC (first order), to RMS 1e-10.  The plain FP64 GPU
-    path solves them one at a time (meshes resident); the server solves them in mixed precision with
-    continuous batching, first as a burst (capacity) and then as a Poisson stream at 80% of that capacity
-    (latency)."""
+    0.15), Mach U(0.3, 0.85), AoA U(-2, 6) deg, Roe or HLLC (first order), to RMS 1e-10, FGMRES(30) with the
+    multicolour ILU(0).  The plain FP64 GPU path solves them one at a time (meshes resident); the server solves
+    them in mixed precision with continuous batching, first as a burst (capacity) and then as a Poisson stream at
+    80% of that capacity (latency)."""
     from cfdsolutions_amd import cfd, mesh
 
     rng = np.random.default_rng(seed)
@@ -238,7 +239,7 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
     for r in reqs:
         p = params(r)
         dms[r[0]].reset(p)
-        out, _ = dms[r[0]].solve_steady(p, alpha_deg=r[2], precond=cfd.PREC_ILU0, mixed=False, check=False,
+        out, _ = dms[r[0]].solve_steady(p, alpha_deg=r[2], precond=cfd.PREC_ILU0_MC, mixed=False, check=False,
                                         **SOLVE_KW)
         t64 += out.seconds
         res64.append(out)
@@ -250,7 +251,7 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
 
     srv = cfd.Server(ctx, hbm_budget=200 << 30, max_inflight=inflight)
     for k, m in enumerate(pool):  # structure resident before timing (one solve per mesh)
-        srv.submit(-1 - k, k, m, params(reqs[0]), alpha_deg=reqs[0][2], precond=cfd.PREC_ILU0, mixed=True,
+        srv.submit(-1 - k, k, m, params(reqs[0]), alpha_deg=reqs[0][2], precond=cfd.PREC_ILU0_MC, mixed=True,
                    **SOLVE_KW)
     got = 0
     while got < n_meshes:
@@ -263,7 +264,7 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
             now = time.perf_counter() - t0
             while k < n_req and arrivals[k] <= now:
                 r = reqs[k]
-                srv.submit(k, r[0], pool[r[0]], params(r), alpha_deg=r[2], precond=cfd.PREC_ILU0, mixed=True,
+                srv.submit(k, r[0], pool[r[0]], params(r), alpha_deg=r[2], precond=cfd.PREC_ILU0_MC, mixed=True,
                            **SOLVE_KW)
                 k += 1
             wait = 0.0 if k >= n_req else max(0.0, arrivals[k] - (time.perf_counter() - t0))
@@ -313,8 +314,9 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
 
 def batch_leg(ctx, n_case, nmin=8000, nmax=40000, seed=4):
     """Bump channels of log-uniform size in [nmin, nmax] nodes, bump height U(0.05, 0.15), Mach U(0.3, 0.8), AoA
-    U(-2, 4) deg, first-order Roe, to RMS 1e-10.  The plain FP64 GPU path solves them one at a time (meshes
-    resident); cfd_batch_solve solves all of them in one union mesh with the mixed-precision linear solve."""
+    U(-2, 4) deg, first-order Roe, to RMS 1e-10, FGMRES(30) with the multicolour ILU(0).  The plain FP64 GPU path
+    solves them one at a time (meshes resident); cfd_batch_solve solves all of them in one union mesh with the
+    mixed-precision linear solve."""
     from cfdsolutions_amd import cfd, mesh
 
     rng = np.random.default_rng(seed)
@@ -325,13 +327,14 @@ def batch_leg(ctx, n_case, nmin=8000, nmax=40000, seed=4):
     t64, r64 = 0.0, []
     for m, p, a in zip(meshes, params, alphas):
         dm = cfd.DeviceMesh(ctx, m, p)
-        r, _ = dm.solve_steady(p, alpha_deg=a, precond=cfd.PREC_ILU0, mixed=False, check=False, **SOLVE_KW)
+        r, _ = dm.solve_steady(p, alpha_deg=a, precond=cfd.PREC_ILU0_MC, mixed=False, check=False, **SOLVE_KW)
         t64 += r.seconds
         r64.append(r)
         dm.close()
         print("bench: batch baseline case %d/%d: %d nodes, %.2f s" % (len(r64), n_case, m.n_node, r.seconds),
               file=sys.stderr, flush=True)
-    rc, rb = cfd.batch_solve(ctx, meshes, params, alphas=list(alphas), precond=cfd.PREC_ILU0, mixed=True, **SOLVE_KW)
+    rc, rb = cfd.batch_solve(ctx, meshes, params, alphas=list(alphas), precond=cfd.PREC_ILU0_MC, mixed=True,
+                             **SOLVE_KW)
     tb = rb[0].seconds if rc == 0 else float("nan")
     ferr = max(float(np.max(np.abs(np.array(b.force) - np.array(a.force))) / np.max(np.abs(a.force)))
                for a, b in zip(r64, rb)) if rc == 0 else float("nan")

@@ -16,7 +16,7 @@ FLUX_ROE, FLUX_HLLC, FLUX_AUSMP = 0, 1, 2
 LIM_NONE, LIM_VENKAT, LIM_BJ = 0, 1, 2
 GRAD_LSQ, GRAD_GG = 0, 1
 FP64, FP32 = 0, 1
-PREC_NONE, PREC_BJACOBI, PREC_ILU0 = 0, 1, 2
+PREC_NONE, PREC_BJACOBI, PREC_ILU0, PREC_ILU0_MC = 0, 1, 2, 3
 SOLVER_FGMRES, SOLVER_BLOCK_GS, SOLVER_LUSGS = 0, 1, 2
 FIELD_Q, FIELD_GRAD, FIELD_PHI, FIELD_RES_CONV, FIELD_RES_DISS, FIELD_DT_ACC, FIELD_DT, FIELD_JAC_B, FIELD_X = range(9)
 

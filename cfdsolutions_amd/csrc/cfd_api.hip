@@ -77,6 +77,36 @@ extern "C" const char *cfd_last_error(const cfd_ctx *ctx) {
 // ---------------------------------------------------------------------------
 // mesh creation
 // ---------------------------------------------------------------------------
+// ILU(0) merge map of a BSR pattern: for lower entry kk = (i, k) and the t-th
+// upper block of row k, mqq[moff[kk] + t] is the slot of row i in the same
+// column, or -1.
+static bool build_merge_map(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau, int N,
+                            std::vector<int> &moff, std::vector<int> &mqq, int64_t &total) {
+    moff.assign(ia[N], 0);
+    int64_t tot = 0;
+    for (int i = 0; i < N; i++)
+        for (int kk = ia[i]; kk < iau[i]; kk++) {
+            moff[kk] = (int)tot;
+            const int k = ja[kk];
+            tot += ia[k + 1] - (iau[k] + 1);
+            if (tot > INT32_MAX) return false;
+        }
+    mqq.assign((size_t)tot, -1);
+    for (int i = 0; i < N; i++)
+        for (int kk = ia[i]; kk < iau[i]; kk++) {
+            const int k = ja[kk];
+            int qq = kk + 1;
+            const int qe = ia[i + 1];
+            for (int pp = iau[k] + 1, t2 = 0; pp < ia[k + 1]; pp++, t2++) {
+                while (qq < qe && ja[qq] < ja[pp]) qq++;
+                if (qq < qe && ja[qq] == ja[pp]) mqq[(size_t)moff[kk] + t2] = qq;
+            }
+        }
+    total = tot;
+    if (mqq.empty()) mqq.push_back(-1);  // keeps the device array non-empty
+    return true;
+}
+
 // Rows of each level followed by -1 up to a multiple of w.
 static void pad_levels(const std::vector<int> &ptr, const std::vector<int> &rows, int w, std::vector<int> &out) {
     out.clear();
@@ -361,38 +391,64 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) soff[idx] = off[k];
         }
         // ILU(0) merge map (see cfd_mesh::ilu_mqq)
-        std::vector<int> moff(m->nnz, 0), mqq;
-        {
-            int64_t tot = 0;
-            for (int i = 0; i < N; i++)
-                for (int kk = m->h_ia[i]; kk < m->h_iau[i]; kk++) {
-                    moff[kk] = (int)tot;
-                    const int k = m->h_ja[kk];
-                    tot += m->h_ia[k + 1] - (m->h_iau[k] + 1);
-                    if (tot > INT32_MAX) {
-                        cfd_mesh_destroy(m);
-                        return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: ILU(0) merge map too large");
-                    }
-                }
-            mqq.assign((size_t)tot, -1);
-            for (int i = 0; i < N; i++)
-                for (int kk = m->h_ia[i]; kk < m->h_iau[i]; kk++) {
-                    const int k = m->h_ja[kk];
-                    int qq = kk + 1;
-                    const int qe = m->h_ia[i + 1];
-                    for (int pp = m->h_iau[k] + 1, t2 = 0; pp < m->h_ia[k + 1]; pp++, t2++) {
-                        while (qq < qe && m->h_ja[qq] < m->h_ja[pp]) qq++;
-                        if (qq < qe && m->h_ja[qq] == m->h_ja[pp]) mqq[(size_t)moff[kk] + t2] = qq;
-                    }
-                }
-            m->ilu_nmap = tot;
-            if (mqq.empty()) mqq.push_back(-1);  // keeps the device array non-empty
+        std::vector<int> moff, mqq;
+        if (!build_merge_map(m->h_ia, m->h_ja, m->h_iau, N, moff, mqq, m->ilu_nmap)) {
+            cfd_mesh_destroy(m);
+            return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: ILU(0) merge map too large");
         }
         std::vector<int> lrows, urows;
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, false, m->lower.ptr, lrows, m->lower.maxwidth);
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, true, m->upper.ptr, urows, m->upper.maxwidth);
         m->lower.nlevels = (int)m->lower.ptr.size() - 1;
         m->upper.nlevels = (int)m->upper.ptr.size() - 1;
+        // multicolour ordering for ILU(0): greedy colouring of the node graph in
+        // reference order; rows sorted by (colour, index)
+        std::vector<int> mperm(N), mia(N + 1, 0), mja(m->nnz), miau(N), msrc(m->nnz), mmoff, mmqq, mlrows, murows;
+        {
+            std::vector<int> colour(N, -1), mark;
+            int nc = 0;
+            for (int i = 0; i < N; i++) {
+                mark.assign(nc + 1, 0);
+                for (int q = d->n2n_ia[i]; q < d->n2n_ia[i + 1]; q++) {
+                    const int cj = colour[d->n2n_ja[q]];
+                    if (cj >= 0) mark[cj] = 1;
+                }
+                int c = 0;
+                while (mark[c]) c++;
+                colour[i] = c;
+                nc = std::max(nc, c + 1);
+            }
+            m->ncolor = nc;
+            std::vector<int> cnt(nc + 1, 0), miperm(N);
+            for (int i = 0; i < N; i++) cnt[colour[i] + 1]++;
+            for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
+            for (int i = 0; i < N; i++) mperm[cnt[colour[i]]++] = i;  // ascending index within a colour
+            for (int t = 0; t < N; t++) miperm[mperm[t]] = t;
+            std::vector<std::pair<int, int>> row;
+            for (int t = 0; t < N; t++) {
+                const int oi = mperm[t];
+                row.clear();
+                for (int k = m->h_ia[oi]; k < m->h_ia[oi + 1]; k++) row.push_back({miperm[m->h_ja[k]], k});
+                std::sort(row.begin(), row.end());
+                int k2 = mia[t];
+                for (auto &e : row) {
+                    if (e.first == t) miau[t] = k2;
+                    mja[k2] = e.first;
+                    msrc[k2] = e.second;
+                    k2++;
+                }
+                mia[t + 1] = k2;
+            }
+            int64_t tot = 0;
+            if (!build_merge_map(mia, mja, miau, N, mmoff, mmqq, tot)) {
+                cfd_mesh_destroy(m);
+                return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: ILU(0) merge map too large");
+            }
+            build_levels(mia, mja, miau, N, false, m->mc_lower.ptr, mlrows, m->mc_lower.maxwidth);
+            build_levels(mia, mja, miau, N, true, m->mc_upper.ptr, murows, m->mc_upper.maxwidth);
+            m->mc_lower.nlevels = (int)m->mc_lower.ptr.size() - 1;
+            m->mc_upper.nlevels = (int)m->mc_upper.ptr.size() - 1;
+        }
         // neighbour lists (reference order within each list), internal numbering
         std::vector<int> nia(N + 1, 0), nja((size_t)d->n2n_ia[N]);
         for (int ii = 0; ii < N; ii++) {
@@ -442,6 +498,15 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         if (!rc) m->iau = upload(m, m->h_iau.data(), m->h_iau.size(), &rc);
         if (!rc) m->ilu_moff = upload(m, moff.data(), moff.size(), &rc);
         if (!rc) m->ilu_mqq = upload(m, mqq.data(), mqq.size(), &rc);
+        if (!rc) m->mc_perm = upload(m, mperm.data(), mperm.size(), &rc);
+        if (!rc) m->mc_ia = upload(m, mia.data(), mia.size(), &rc);
+        if (!rc) m->mc_ja = upload(m, mja.data(), mja.size(), &rc);
+        if (!rc) m->mc_iau = upload(m, miau.data(), miau.size(), &rc);
+        if (!rc) m->mc_src = upload(m, msrc.data(), msrc.size(), &rc);
+        if (!rc) m->mc_moff = upload(m, mmoff.data(), mmoff.size(), &rc);
+        if (!rc) m->mc_mqq = upload(m, mmqq.data(), mmqq.size(), &rc);
+        if (!rc) m->mc_lower.rows = upload(m, mlrows.data(), mlrows.size(), &rc);
+        if (!rc) m->mc_upper.rows = upload(m, murows.data(), murows.size(), &rc);
         if (!rc) m->lower.rows = upload(m, lrows.data(), lrows.size(), &rc);
         if (!rc) m->upper.rows = upload(m, urows.data(), urows.size(), &rc);
         {
@@ -524,6 +589,8 @@ extern "C" int64_t cfd_footprint_bytes(int32_t N, int32_t E, int32_t B, int32_t 
     bytes += nsell * (4 + 4 + 3 * 8);                       // nbr, soff, isx, isy, isz
     bytes += 4 * ((n + 1) + nnz + n) + 4 * nnz;             // BSR ia, ja, iau; ILU(0) merge offsets
     bytes += 4 * 8 * nnz;                                   // ILU(0) merge map (<= 8 upper blocks per lower entry)
+    bytes += 4 * (n + (n + 1) + 3 * nnz + n + 2 * n) + 4 * 8 * nnz;  // multicolour pattern, slot map, levels, merge map
+    bytes += s * 5 * n;                                     // multicolour work vector
     bytes += 4 * (2 * n + 2 * n + (11 + 11) * lvl);        // level rows and padded slot lists
     bytes += 8 * (1024 * 65 + 4 * 65 + 6) + 4 * 4;         // reduction scratch, eigenvalue bits, flags
     bytes += 25 * s * nnz * 2 + 25 * s * n;                 // Jacobian, ILU(0) factors, inverse pivots

@@ -130,6 +130,17 @@ struct cfd_mesh {
     double *gslu = nullptr;   // [N][25] no-pivot LU of the diagonal blocks (block GS)
     double *rowbuf = nullptr; // [N] per-row sums of the GS sweep
     LevelSchedule lower, upper;
+    // multicolour ILU(0): rows ordered by the colour of a greedy colouring of
+    // the node graph (in reference order), then by index, so the triangular
+    // factors have one level per colour
+    int ncolor = 0;
+    int *mc_perm = nullptr;                               // [N] original row of each permuted row
+    int *mc_ia = nullptr, *mc_ja = nullptr, *mc_iau = nullptr;  // permuted BSR pattern
+    int *mc_src = nullptr;                                // [nnz] original slot of each permuted slot
+    int *mc_moff = nullptr, *mc_mqq = nullptr;            // ILU(0) merge map of the permuted pattern
+    LevelSchedule mc_lower, mc_upper;
+    double *zp = nullptr;                                 // [N][5] permuted work vector
+    float *zpf = nullptr;
     bool sync_free = true;    // single-launch triangular sweeps (else one launch per level)
     int *ready = nullptr;     // [N] sweep completion epochs
     int *epoch = nullptr;     // [1] current sweep epoch

@@ -504,6 +504,64 @@ __global__ __launch_bounds__(kBlock) void k_trsv_sf(int nslots, const int *__res
     if (!ok) atomicOr(flag + 3, 1);
 }
 
+// Blocks of the matrix in multicolour order: dst[s] = src[map[s]].
+template <class TS, class TD>
+__global__ __launch_bounds__(kBlock) void k_gather_blocks(size_t nnz, const int *__restrict__ map,
+                                                          const TS *__restrict__ src, TD *__restrict__ dst) {
+    for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < 25 * nnz; t += (size_t)gridDim.x * kBlock) {
+        const size_t s = t / 25, q = t - 25 * s;
+        dst[t] = (TD)src[25 * (size_t)map[s] + q];
+    }
+}
+
+// One level of the multicolour ILU(0) sweeps; rows and columns are in
+// multicolour order, the right-hand side and the result in the reference
+// order (perm maps a multicolour row to its reference row).  Forward:
+// w_i = r_perm(i) - sum_{k<i} L_ik w_k.  Backward: w_i = D_i^-1 (w_i -
+// sum_{k>i} U_ik w_k), z_perm(i) = w_i.  5 lanes per row, as k_trsv_level.
+template <class TM, class TV>
+__global__ __launch_bounds__(kBlock) void k_trsv_mc(int nrows, const int *__restrict__ rows,
+                                                    const int *__restrict__ ia, const int *__restrict__ ja,
+                                                    const int *__restrict__ iau, const int *__restrict__ perm,
+                                                    const TM *__restrict__ lu, const TM *__restrict__ dinv,
+                                                    const TV *__restrict__ rhs, TV *w, TV *z, int backward) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / 5, r = lane - 5 * slot;
+    const int t = (blockIdx.x * (kBlock / 64) + wave) * 12 + slot;
+    const bool active = slot < 12 && t < nrows;
+    const int i = active ? rows[t] : 0;
+    TV y = TV(0.0);
+    if (active) {
+        const int kb = backward ? iau[i] + 1 : ia[i];
+        const int ke = backward ? ia[i + 1] : iau[i];
+        y = backward ? w[5 * (size_t)i + r] : rhs[5 * (size_t)perm[i] + r];
+        for (int kk = kb; kk < ke; kk++) {
+            const TM *A = lu + 25 * (size_t)kk + 5 * r;
+            const TV *wj = w + 5 * (size_t)ja[kk];
+            TV s = TV(0.0);
+#pragma unroll
+            for (int c = 0; c < 5; c++) s += (TV)A[c] * wj[c];
+            y = y - s;
+        }
+    }
+    if (!backward) {
+        if (active) w[5 * (size_t)i + r] = y;
+        return;
+    }
+    const int base = lane - r;
+    TV yy[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) yy[c] = __shfl(y, base + c, 64);
+    if (active) {
+        const TM *D = dinv + 25 * (size_t)i + 5 * r;
+        TV s = TV(0.0);
+#pragma unroll
+        for (int c = 0; c < 5; c++) s += (TV)D[c] * yy[c];
+        w[5 * (size_t)i + r] = s;
+        z[5 * (size_t)perm[i] + r] = s;
+    }
+}
+
 // z = D^-1 r (block-Jacobi apply)
 template <class TM, class TV>
 __global__ __launch_bounds__(kBlock) void k_bjac_apply(int N, const TM *dinv, const TV *rhs, TV *z) {
@@ -876,7 +934,16 @@ namespace {
 
 template <class TM>
 int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
-    if (kind == CFD_PREC_ILU0) {
+    if (kind == CFD_PREC_ILU0_MC) {
+        hipLaunchKernelGGL((k_gather_blocks<TM, TM>), dim3(grid_cap(25 * (size_t)m->nnz, 8192)), dim3(kBlock), 0,
+                           m->stream, (size_t)m->nnz, m->mc_src, a, lu);
+        for (int l = 0; l < m->mc_lower.nlevels; l++) {
+            const int b0 = m->mc_lower.ptr[l], nr = m->mc_lower.ptr[l + 1] - b0;
+            hipLaunchKernelGGL(k_ilu0_level<TM>, dim3((nr + 47) / 48), dim3(kBlock), 0, m->stream, nr,
+                               m->mc_lower.rows + b0, m->mc_ia, m->mc_ja, m->mc_iau, m->mc_moff, m->mc_mqq, lu, dinv,
+                               lu, dinv, m->flag);
+        }
+    } else if (kind == CFD_PREC_ILU0) {
         CFD_HIP(m, hipMemcpyAsync(lu, a, sizeof(TM) * 25 * (size_t)m->nnz, hipMemcpyDeviceToDevice, m->stream));
         for (int l = 0; l < m->lower.nlevels; l++) {
             const int b0 = m->lower.ptr[l], nr = m->lower.ptr[l + 1] - b0;
@@ -892,10 +959,28 @@ int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
     return CFD_OK;
 }
 
+template <class TV>
+TV *mc_work(cfd_mesh *m);
+template <>
+double *mc_work<double>(cfd_mesh *m) { return m->zp; }
+template <>
+float *mc_work<float>(cfd_mesh *m) { return m->zpf; }
+
 template <class TM, class TV>
 void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r, TV *z) {
     const size_t L = 5 * (size_t)m->N;
-    if (kind == CFD_PREC_ILU0 && m->sync_free) {
+    if (kind == CFD_PREC_ILU0_MC) {
+        TV *wk = mc_work<TV>(m);
+        for (int pass = 0; pass < 2; pass++) {
+            const LevelSchedule &ls = pass ? m->mc_upper : m->mc_lower;
+            for (int l = 0; l < ls.nlevels; l++) {
+                const int b0 = ls.ptr[l], nr = ls.ptr[l + 1] - b0;
+                const int nwaves = (nr + 11) / 12;
+                hipLaunchKernelGGL((k_trsv_mc<TM, TV>), dim3((nwaves + 3) / 4), dim3(kBlock), 0, m->stream, nr,
+                                   ls.rows + b0, m->mc_ia, m->mc_ja, m->mc_iau, m->mc_perm, lu, dinv, r, wk, z, pass);
+            }
+        }
+    } else if (kind == CFD_PREC_ILU0 && m->sync_free) {
         for (int pass = 0; pass < 2; pass++) {
             const LevelSchedule &ls = pass ? m->upper : m->lower;
             hipLaunchKernelGGL(k_epoch_bump, dim3(1), dim3(64), 0, m->stream, m->epoch);
@@ -932,6 +1017,10 @@ void mdot(cfd_mesh *m, int nv, const TV *V, size_t ld, const TV *w, int off) {
 int ensure_krylov(cfd_mesh *m, int mk, bool mixed) {
     if (m->kry_m >= mk && m->kry_mixed == mixed && (mixed ? m->Vf != nullptr : m->V != nullptr)) return CFD_OK;
     const size_t L = 5 * (size_t)m->N;
+    if (mixed && !m->zpf) m->zpf = (float *)cfd_dalloc(m, sizeof(float) * L);
+    if (!mixed && !m->zp) m->zp = (double *)cfd_dalloc(m, sizeof(double) * L);
+    if ((mixed && !m->zpf) || (!mixed && !m->zp))
+        return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (multicolour work vector)");
     if (mixed) {
         m->Vf = (float *)cfd_dalloc(m, sizeof(float) * L * (mk + 1));
         m->Zf = (float *)cfd_dalloc(m, sizeof(float) * L * mk);
@@ -1420,6 +1509,8 @@ namespace {
 int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
     if (o->krylov_m < 1 || o->krylov_m > kMaxKrylov)
         return cfd_set_error(m->ctx, CFD_E_ARG, "krylov_m must be in [1, %d]", kMaxKrylov);
+    if (o->precond < CFD_PREC_NONE || o->precond > CFD_PREC_ILU0_MC)
+        return cfd_set_error(m->ctx, CFD_E_ARG, "unknown preconditioner %d", o->precond);
     const bool mixed = o->mixed != 0;
     if (mixed != m->jac_mixed)
         return cfd_set_error(m->ctx, CFD_E_ARG, "Jacobian precision does not match linsolve precision");
@@ -1429,11 +1520,13 @@ int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
         m->prec_kind = o->precond;
         m->prec_mixed = mixed;
         if (mixed) {
-            if (o->precond == CFD_PREC_ILU0 && !m->luf) m->luf = (float *)cfd_dalloc(m, sizeof(float) * 25 * (size_t)m->nnz);
+            if ((o->precond == CFD_PREC_ILU0 || o->precond == CFD_PREC_ILU0_MC) && !m->luf)
+                m->luf = (float *)cfd_dalloc(m, sizeof(float) * 25 * (size_t)m->nnz);
             if (o->precond != CFD_PREC_NONE && !m->dinvf) m->dinvf = (float *)cfd_dalloc(m, sizeof(float) * 25 * (size_t)m->N);
             rc = build_prec<float>(m, o->precond, m->af, m->luf, m->dinvf);
         } else {
-            if (o->precond == CFD_PREC_ILU0 && !m->lu) m->lu = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->nnz);
+            if ((o->precond == CFD_PREC_ILU0 || o->precond == CFD_PREC_ILU0_MC) && !m->lu)
+                m->lu = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->nnz);
             if (o->precond != CFD_PREC_NONE && !m->dinv) m->dinv = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->N);
             rc = build_prec<double>(m, o->precond, m->a, m->lu, m->dinv);
         }

@@ -60,6 +60,7 @@ extern "C" {
 #define CFD_PREC_NONE 0
 #define CFD_PREC_BJACOBI 1
 #define CFD_PREC_ILU0 2
+#define CFD_PREC_ILU0_MC 3  /* ILU(0) of the matrix with rows in multicolour order */
 /* linear solvers */
 #define CFD_SOLVER_FGMRES 0
 #define CFD_SOLVER_BLOCK_GS 1  /* MC_Iterative_Block_LU_Jacobi_CRS, forward */

@@ -125,7 +125,38 @@ def test_jacobian_and_reference_sweeps(case):
     assert_bitexact(dm.get(cfd.FIELD_X), d["lusgs_x"], "LU-SGS solution")
 
 
-@pytest.mark.parametrize("precond", [cfd.PREC_ILU0, cfd.PREC_BJACOBI])
+def multicolour_order(n2n_ia, n2n_ja):
+    """Greedy colouring of the node graph in index order; rows sorted by (colour, index)."""
+    n = len(n2n_ia) - 1
+    col = -np.ones(n, dtype=np.int64)
+    for i in range(n):
+        used = set(int(c) for c in col[n2n_ja[n2n_ia[i]:n2n_ia[i + 1]]] if c >= 0)
+        c = 0
+        while c in used:
+            c += 1
+        col[i] = c
+    return np.lexsort((np.arange(n), col))
+
+
+def permute_bsr(ia, ja, a, perm):
+    """P A P^T of a BSR matrix (25-entry blocks) with rows taken in the order perm."""
+    n = len(ia) - 1
+    iperm = np.empty(n, np.int64)
+    iperm[perm] = np.arange(n)
+    a = np.asarray(a).reshape(-1, 25)
+    nia, nja, niau, na = [0], [], np.zeros(n, np.int32), []
+    for t in range(n):
+        cols = sorted((int(iperm[ja[k]]), k) for k in range(ia[perm[t]], ia[perm[t] + 1]))
+        for c, k in cols:
+            if c == t:
+                niau[t] = len(nja)
+            nja.append(c)
+            na.append(a[k])
+        nia.append(len(nja))
+    return np.array(nia, np.int32), np.array(nja, np.int32), niau, np.array(na)
+
+
+@pytest.mark.parametrize("precond", [cfd.PREC_ILU0, cfd.PREC_BJACOBI, cfd.PREC_ILU0_MC])
 def test_fgmres_matches_oracle(case, precond):
     d, dm, p = case
     if "jac_a" not in d:
@@ -134,7 +165,18 @@ def test_fgmres_matches_oracle(case, precond):
     dm.jacobian_set(d["jac_a"], d["jac_b"])
     st = dm.linsolve(solver=cfd.SOLVER_FGMRES, precond=precond, krylov_m=20, max_restarts=6, rtol=1e-8)
     x = dm.get(cfd.FIELD_X)
-    xo, its, _ = orc.fgmres(ia, ja, iau, d["jac_a"], d["jac_b"], m=20, max_restarts=6, rtol=1e-8, precond=precond)
+    if precond == cfd.PREC_ILU0_MC:
+        # the oracle's ILU(0) FGMRES on the system in multicolour order
+        perm = multicolour_order(d["n2n_ia"], d["n2n_ja"])
+        pia, pja, piau, pa = permute_bsr(ia, ja, d["jac_a"], perm)
+        pb = np.asarray(d["jac_b"]).reshape(-1, 5)[perm]
+        xp, its, _ = orc.fgmres(pia, pja, piau, pa, pb, m=20, max_restarts=6, rtol=1e-8, precond=cfd.PREC_ILU0)
+        xo = np.empty_like(np.asarray(xp).reshape(-1, 5))
+        xo[perm] = np.asarray(xp).reshape(-1, 5)
+    else:
+        xo, its, _ = orc.fgmres(ia, ja, iau, d["jac_a"], d["jac_b"], m=20, max_restarts=6, rtol=1e-8,
+                                precond=precond)
+        xo = np.asarray(xo).reshape(x.shape)
     assert abs(st.iterations - its) <= 1, (st.iterations, its)
     assert np.max(np.abs(x - xo)) <= 1e-9 * np.max(np.abs(xo))
     r = orc.bsr_mv(ia, ja, d["jac_a"], x) - d["jac_b"]

@@ -179,7 +179,8 @@ def test_server_rejects_request_larger_than_budget(ctx):
     srv.close()
 
 
-def test_batch_solve_matches_alone(ctx):
+@pytest.mark.parametrize("precond", [cfd.PREC_ILU0, cfd.PREC_ILU0_MC])
+def test_batch_solve_matches_alone(ctx, precond):
     """Cases solved together in one union mesh reproduce, bit for bit, the iteration counts, residual norms and
     forces of each case solved alone."""
     cases = [((12, 4, 4), 0.08, 0.5, 0.0), ((16, 6, 4), 0.12, 0.6, 2.0), ((14, 5, 5), 0.1, 0.45, -1.0),
@@ -187,6 +188,7 @@ def test_batch_solve_matches_alone(ctx):
     meshes = [cmesh.bump_channel(*d, bump_h=h) for d, h, _, _ in cases]
     params = [cfd.make_params(mach=mach, alpha_deg=a, flux="roe", order=1) for _, _, mach, a in cases]
     kw = dict(SOLVE)
+    kw["precond"] = precond
     alone = []
     for m, p, (_, _, _, a) in zip(meshes, params, cases):
         dm = cfd.DeviceMesh(ctx, m, p)

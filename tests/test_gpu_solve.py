@@ -88,3 +88,21 @@ def test_reference_linear_solver_newton_loop(ctx):
     r, hist = dm.solve_steady(p, max_iter=400, tol=1e-9, solver=cfd.SOLVER_BLOCK_GS, sweeps=20)
     assert r.converged, (r.iterations, r.final_rms)
     dm.close()
+
+
+def test_multicolour_ilu_solve_converges_to_the_same_state(ctx):
+    """The multicolour-ordered ILU(0) changes the linear-solve iterates, not the steady state: forces within 1e-8
+    of the natural-order solve, FP64 and mixed precision."""
+    m = cmesh.bump_channel(16, 6, 6)
+    p = cfd.make_params(mach=0.5, alpha_deg=1.0, flux="roe", order=1)
+    out = {}
+    for precond, mixed in ((cfd.PREC_ILU0, False), (cfd.PREC_ILU0_MC, False), (cfd.PREC_ILU0_MC, True)):
+        dm = cfd.DeviceMesh(ctx, m, p)
+        r, _ = dm.solve_steady(p, max_iter=300, tol=1e-10, krylov_m=30, max_restarts=4, rtol=1e-3, precond=precond,
+                               mixed=mixed, alpha_deg=1.0)
+        assert r.converged, (precond, mixed)
+        out[(precond, mixed)] = r
+        dm.close()
+    ref = out[(cfd.PREC_ILU0, False)]
+    for key, r in out.items():
+        np.testing.assert_allclose(list(r.force), list(ref.force), rtol=1e-8, atol=1e-10)
