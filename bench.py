@@ -212,6 +212,36 @@ def solve_leg(ctx):
     return res
 
 
+def solve3_leg(ctx):
+    """Bump channel 192x64x64 (815,425 nodes), second-order HLLC with Green-Gauss gradients and the
+    Venkatakrishnan limiter, M 0.7, AoA 2 deg, FGMRES(40) with the multicolour ILU(0), to RMS 1e-10: plain FP64 vs
+    mixed precision (FP32 Jacobian, ILU(0) and Krylov basis), each alone on the GPU."""
+    from cfdsolutions_amd import cfd, mesh
+
+    m = mesh.bump_channel(192, 64, 64, bump_h=0.1)
+    p = cfd.make_params(mach=0.7, alpha_deg=2.0, flux="hllc", order=2, limiter=cfd.LIM_VENKAT, gradient=cfd.GRAD_GG)
+    kw = dict(SOLVE_KW, krylov_m=40, alpha_deg=2.0, precond=cfd.PREC_ILU0_MC)
+    out = {}
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        dm = cfd.DeviceMesh(ctx, m, p)
+        r, _ = dm.solve_steady(p, mixed=mixed, check=False, **kw)
+        out[name] = (r, dm.get(cfd.FIELD_Q))
+        print("bench: solve 192x64x64 %-6s %.2f s (%d newton / %d krylov), converged %d, rms %.2e" % (
+            name, r.seconds, r.iterations, r.krylov_total, r.converged, r.final_rms), file=sys.stderr, flush=True)
+        dm.close()
+    (r0, q0), (r1, q1) = out["fp64"], out["mixed"]
+    return {"workload": "bump channel 192x64x64 (%d nodes), 2nd-order HLLC + GG + Venkatakrishnan, M 0.7, AoA 2, "
+                        "FGMRES(40) + multicolour ILU(0), RMS < 1e-10" % m.n_node,
+            "fp64": {"seconds": r0.seconds, "newton": r0.iterations, "krylov": r0.krylov_total,
+                     "converged": bool(r0.converged), "lift": r0.lift, "drag": r0.drag},
+            "mixed": {"seconds": r1.seconds, "newton": r1.iterations, "krylov": r1.krylov_total,
+                      "converged": bool(r1.converged), "lift": r1.lift, "drag": r1.drag,
+                      "force_rel_err_vs_fp64": float(np.max(np.abs(np.array(r1.force) - np.array(r0.force))) /
+                                                     np.max(np.abs(r0.force))),
+                      "state_rel_err_vs_fp64": float(np.max(np.abs(q1 - q0)) / np.max(np.abs(q0)))},
+            "speedup_mixed": r0.seconds / r1.seconds}
+
+
 def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0.3, 0.85), aoa=(-2.0, 6.0),
               fluxes=("roe", "hllc"), stream=True):
     """Steady-solve requests on bump channels of log-uniform size in [nmin, nmax] nodes, bump height U(0.05,
@@ -363,7 +393,7 @@ def main():
     ap.add_argument("--only", action="store_true", help="run only the headline variant (for profiling)")
     ap.add_argument("--no-solves", action="store_true", help="skip the steady-solve and serving legs")
     ap.add_argument("--legs", type=str, default="residual,solve,serve,batch",
-                    help="comma list of legs to run: residual, solve, serve, batch")
+                    help="comma list of legs to run: residual, solve, serve, batch, solve3 (192x64x64 solve)")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-meshes", type=int, default=6)
     ap.add_argument("--serve-nodes", type=str, default="10000,50000", help="min,max nodes of the serving meshes")
@@ -400,6 +430,8 @@ def main():
         out = {"metric": "converged solves/s & p99 latency (mixed vs FP64 GPU); residual-eval GB/s", "legs": sorted(legs)}
         if "solve" in legs:
             out["solves"] = solve_leg(ctx)
+        if "solve3" in legs:
+            out["solve_192x64x64"] = solve3_leg(ctx)
         if "serve" in legs:
             nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
             out["serving"] = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight)

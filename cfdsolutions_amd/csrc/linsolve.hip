@@ -719,26 +719,47 @@ __global__ __launch_bounds__(kBlock) void k_lusgs_init(int N, const double *b, d
 // Krylov vector kernels (deterministic reductions)
 // ---------------------------------------------------------------------------
 // part[blk][k] = sum over the block's elements of V_k . w, k < nv
+// Partial dot products of w with V_0..V_{nv-1} over this block's strided
+// elements [o + i0, o + n) (step `stride`), stored in part[0..nv).  The
+// products are accumulated in register chunks of 8 vectors (w is re-read per
+// chunk): each (thread, vector) sum visits the same elements in the same order
+// and each vector goes through the same block tree, whatever the chunking.
+template <class TV>
+__device__ __forceinline__ void block_dots(size_t o, size_t i0, size_t n, size_t stride, int nv,
+                                           const TV *__restrict__ V, size_t ld, const TV *__restrict__ w,
+                                           double *__restrict__ part) {
+    __shared__ double sh[8][kBlock];
+    for (int k0 = 0; k0 < nv; k0 += 8) {
+        double acc[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) acc[kk] = 0.0;
+        for (size_t i = i0; i < n; i += stride) {
+            const double wi = (double)w[o + i];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++)
+                if (k0 + kk < nv) acc[kk] += (double)V[ld * (k0 + kk) + o + i] * wi;
+        }
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) sh[kk][threadIdx.x] = acc[kk];
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) sh[kk][threadIdx.x] += sh[kk][threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            for (int kk = 0; kk < 8 && k0 + kk < nv; kk++) part[k0 + kk] = sh[kk][0];
+        __syncthreads();
+    }
+}
+
 template <class TV>
 __global__ __launch_bounds__(kBlock) void k_mdot(size_t n, int nv, const TV *__restrict__ V, size_t ld,
                                                  const TV *__restrict__ w, double *__restrict__ part) {
-    __shared__ double sh[kBlock];
-    double acc[kMaxKrylov + 1];
-    for (int k = 0; k < nv; k++) acc[k] = 0.0;
-    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
-        const double wi = (double)w[i];
-        for (int k = 0; k < nv; k++) acc[k] += (double)V[ld * k + i] * wi;
-    }
-    for (int k = 0; k < nv; k++) {
-        sh[threadIdx.x] = acc[k];
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[(size_t)blockIdx.x * (kMaxKrylov + 1) + k] = sh[0];
-        __syncthreads();
-    }
+    block_dots<TV>(0, blockIdx.x * (size_t)kBlock + threadIdx.x, n, (size_t)gridDim.x * kBlock, nv, V, ld, w,
+                   part + (size_t)blockIdx.x * (kMaxKrylov + 1));
 }
 
 // Sum of the partials of dot k (block k): lane l adds partials l, l+64, ...
@@ -822,26 +843,10 @@ __global__ __launch_bounds__(kBlock) void k_mdot_seg(const int2 *__restrict__ bl
                                                      const int *__restrict__ off, const int *__restrict__ len, int nv,
                                                      const TV *__restrict__ V, size_t ld, const TV *__restrict__ w,
                                                      double *__restrict__ part) {
-    __shared__ double sh[kBlock];
     const int2 cb = blk[blockIdx.x];
     const size_t P = (size_t)(first[cb.x + 1] - first[cb.x]);
-    const size_t o = (size_t)off[cb.x], n = (size_t)len[cb.x];
-    double acc[kMaxKrylov + 1];
-    for (int k = 0; k < nv; k++) acc[k] = 0.0;
-    for (size_t i = cb.y * (size_t)kBlock + threadIdx.x; i < n; i += P * kBlock) {
-        const double wi = (double)w[o + i];
-        for (int k = 0; k < nv; k++) acc[k] += (double)V[ld * k + o + i] * wi;
-    }
-    for (int k = 0; k < nv; k++) {
-        sh[threadIdx.x] = acc[k];
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[(size_t)blockIdx.x * (kMaxKrylov + 1) + k] = sh[0];
-        __syncthreads();
-    }
+    block_dots<TV>((size_t)off[cb.x], cb.y * (size_t)kBlock + threadIdx.x, (size_t)len[cb.x], P * kBlock, nv, V, ld,
+                   w, part + (size_t)blockIdx.x * (kMaxKrylov + 1));
 }
 
 // out[c][slot + k] = sum of case c's partials, combined as k_mdot_final
