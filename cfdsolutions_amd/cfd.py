@@ -18,6 +18,7 @@ GRAD_LSQ, GRAD_GG = 0, 1
 FP64, FP32 = 0, 1
 PREC_NONE, PREC_BJACOBI, PREC_ILU0, PREC_ILU0_MC = 0, 1, 2, 3
 SOLVER_FGMRES, SOLVER_BLOCK_GS, SOLVER_LUSGS = 0, 1, 2
+JAC_FD_CENTRAL, JAC_FD_FORWARD, JAC_FD_BACKWARD, JAC_FD_ALTERNATE = 0, 1, 2, 3
 FIELD_Q, FIELD_GRAD, FIELD_PHI, FIELD_RES_CONV, FIELD_RES_DISS, FIELD_DT_ACC, FIELD_DT, FIELD_JAC_B, FIELD_X = range(9)
 
 _FLUX_NAMES = {"roe": FLUX_ROE, "hllc": FLUX_HLLC, "ausm+": FLUX_AUSMP, "ausmp": FLUX_AUSMP}
@@ -267,6 +268,11 @@ class DeviceMesh:
 
     def jacobian(self, p, rhs_mode=1, mixed=False):
         self.ctx.check(lib().cfd_jacobian(self.h, ctypes.byref(p), int(rhs_mode), int(bool(mixed))))
+
+    def jacobian_fd(self, p, method=0, it=0, add_time=True, rhs_mode=0, mixed=False):
+        """Finite-difference Roe Jacobian (cfd_jacobian_fd); method JAC_FD_CENTRAL/FORWARD/BACKWARD/ALTERNATE."""
+        self.ctx.check(lib().cfd_jacobian_fd(self.h, ctypes.byref(p), int(method), int(it), int(bool(add_time)),
+                                             int(rhs_mode), int(bool(mixed))))
 
     def bsr_structure(self):
         nnz = lib().cfd_bsr_structure(self.h, None, None, None)

@@ -963,6 +963,40 @@ extern "C" int cfd_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int 
     return launch_jacobian(m, p, rhs_mode, mixed);
 }
 
+extern "C" int cfd_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, int add_time, int rhs_mode,
+                               int mixed) {
+    int rc = check_params(m, p);
+    if (rc) return rc;
+    if (method < CFD_JAC_FD_CENTRAL || method > CFD_JAC_FD_ALTERNATE || iter < 0)
+        return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_jacobian_fd: invalid method %d or iteration %d", method, iter);
+    if (p->precision != CFD_FP64)
+        return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_jacobian_fd: needs the FP64 state path (precision CFD_FP64)");
+    if (!m->residual_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "cfd_jacobian_fd: call cfd_residual first");
+    rc = check_residual_flags(m);
+    if (rc) return rc;
+    if (m->res_prec != CFD_FP64) {
+        rc = launch_residual_to_fp64(m);
+        if (rc) return rc;
+        m->res_prec = CFD_FP64;
+    }
+    rc = ensure_jacobian_storage(m, mixed);
+    if (rc) return rc;
+    CFD_HIP(m, hipMemsetAsync(m->flag, 0, 4 * sizeof(int), m->stream));
+    rc = launch_jacobian_fd(m, p, method, iter, add_time, rhs_mode, mixed);
+    if (rc) return rc;
+    // a boundary instance that does not apply, or a negative ghost pressure,
+    // in a perturbed state (the reference stops with error())
+    rc = read_flags(m);
+    if (rc) return rc;
+    if (m->hflag[3])
+        return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_jacobian_fd: %d perturbed boundary states without a wave "
+                             "configuration", m->hflag[3]);
+    if (m->hflag[0])
+        return cfd_set_error(m->ctx, CFD_E_NEGATIVE_PRESSURE, "cfd_jacobian_fd: %d perturbed ghost states with "
+                             "negative pressure", m->hflag[0]);
+    return CFD_OK;
+}
+
 extern "C" int32_t cfd_bsr_structure(cfd_mesh *m, int32_t *ia, int32_t *ja, int32_t *iau) {
     if (!m) return CFD_E_ARG;
     if (ia) memcpy(ia, m->h_ia.data(), sizeof(int) * m->h_ia.size());

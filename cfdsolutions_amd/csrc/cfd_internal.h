@@ -240,6 +240,8 @@ int launch_convert_geometry_fp32(cfd_mesh *m);
 int launch_residual_to_fp64(cfd_mesh *m);
 
 int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed);
+int launch_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, int add_time, int rhs_mode,
+                       int mixed);
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);
 // Batched FGMRES over a union mesh: solves the cases with solve[c] != 0,
 // writes each case's iterations and final residual estimate.

@@ -131,6 +131,114 @@ __device__ __forceinline__ T roe_flux(T g, int efix, T sx, T sy, T sz, const T *
 
 // Compute_HLLCFlux, Euler3D/src/HLLC_Fluxes.cpp:143-326.  Sets bad = 1 when no
 // wave configuration applies (reference: error()).
+// Freestream and outflow pressure seen by one boundary edge.
+template <class T>
+struct BcFree {
+    T rho, u, v, w, p, out_p;
+};
+
+// Freestream of case c of a union of cases: (rho, u, v, w, p, outflow p) at
+// case_inf[8c .. 8c+5].
+template <class T>
+__device__ __forceinline__ BcFree<T> bc_free_of_case(const double *case_inf, int c) {
+    const double *f = case_inf + 8 * (size_t)c;
+    return BcFree<T>{(T)f[0], (T)f[1], (T)f[2], (T)f[3], (T)f[4], (T)f[5]};
+}
+
+// Characteristic ghost state of one boundary edge:
+// Compute_Characteristic_BoundaryCondition_New, Euler3D/src/BC.cpp:381-613
+// (conservative variables, gauge pressure 0; the older per-edge
+// Compute_Characteristic_BoundaryCondition, Euler3D/src/BC.cpp:33-380, gives
+// the same state).  QL: the node, QR: the current ghost, s: outward area
+// vector, type: 0 wall, 1 far field, 2 inflow, 3 outflow, 4-7 subsonic /
+// supersonic inflow / outflow.  Returns 0, or 1 if the ghost pressure is
+// negative (QB still written), or 2 if no boundary instance applies (QB not
+// written).
+template <class T>
+__device__ __forceinline__ int bc_ghost(T g, const T *QL, const T *QR, T sx, T sy, T sz, int type, int iter, int zpg,
+                                        const BcFree<T> &f, T *QB) {
+    T nx, ny, nz, Qa[5];
+    area_normal(sx, sy, sz, nx, ny, nz);
+    Eos<T> I, Z;
+    eos_face(g, QL, nx, ny, nz, I);
+#pragma unroll
+    for (int k = 0; k < 5; k++) Qa[k] = T(0.5) * (QL[k] + QR[k]);
+    eos_face(g, Qa, nx, ny, nz, Z);
+    const T l1 = Z.ubar, l4 = l1 + Z.c, l5 = l1 - Z.c;
+    const T infP = f.p;
+    T pb = infP, rb = f.rho, ub = f.u, vb = f.v, wb = f.w;
+    int inst = type;
+    bool bad = false;
+    switch (type) {
+    case 0: break;  // Euler wall
+    case 1:         // far field
+        if ((l1 <= T(0.0)) && (l4 <= T(0.0)) && (l5 <= T(0.0))) inst = 10;
+        else if ((l1 >= T(0.0)) && (l4 >= T(0.0)) && (l5 >= T(0.0))) inst = 11;
+        else if ((l1 >= T(0.0)) && (l5 <= T(0.0)) && (l4 >= T(0.0))) inst = 9;
+        else if ((l1 <= T(0.0)) && (l4 >= T(0.0)) && (l5 <= T(0.0))) inst = 8;
+        else bad = true;
+        break;
+    case 2:  // inflow
+        if ((l1 <= T(0.0)) && (l4 >= T(0.0)) && (l5 <= T(0.0))) inst = 4;
+        else if ((l1 <= T(0.0)) && (l4 <= T(0.0)) && (l5 <= T(0.0))) inst = 6;
+        else inst = (fabs(l1) < Z.c) ? 4 : 6;
+        break;
+    case 3:  // outflow
+        if ((l1 >= T(0.0)) && (l5 <= T(0.0)) && (l4 >= T(0.0))) inst = 5;
+        else if ((l1 >= T(0.0)) && (l4 >= T(0.0)) && (l5 >= T(0.0))) inst = 7;
+        else inst = (fabs(l1) < Z.c) ? 5 : 7;
+        break;
+    case 4: case 5: case 6: case 7: break;
+    default: bad = true;
+    }
+    switch (inst) {
+    case 0:
+        pb = I.p + Z.rho * Z.c * I.ubar;
+        if (iter < zpg && pb < T(0.0)) pb = I.p;
+        rb = I.rho + (pb - I.p) / (Z.c * Z.c);
+        ub = I.u - nx * (pb - I.p) / (Z.rho * Z.c);
+        vb = I.v - ny * (pb - I.p) / (Z.rho * Z.c);
+        wb = I.w - nz * (pb - I.p) / (Z.rho * Z.c);
+        break;
+    case 5:
+    case 9:
+        pb = (inst == 5) ? f.out_p : infP;
+        rb = I.rho + (pb - I.p) / (Z.c * Z.c);
+        ub = I.u - nx * (pb - I.p) / (Z.rho * Z.c);
+        vb = I.v - ny * (pb - I.p) / (Z.rho * Z.c);
+        wb = I.w - nz * (pb - I.p) / (Z.rho * Z.c);
+        break;
+    case 7:
+    case 11:
+        pb = I.p;
+        rb = I.rho;
+        ub = I.u;
+        vb = I.v;
+        wb = I.w;
+        break;
+    case 8: {
+        const T temp = nx * (f.u - I.u) + ny * (f.v - I.v) + nz * (f.w - I.w);
+        pb = T(0.5) * (infP + I.p - Z.rho * Z.c * temp);
+        rb = f.rho + (pb - infP) / (Z.c * Z.c);
+        ub = f.u + nx * (pb - infP) / (Z.rho * Z.c);
+        vb = f.v + ny * (pb - infP) / (Z.rho * Z.c);
+        wb = f.w + nz * (pb - infP) / (Z.rho * Z.c);
+        break;
+    }
+    default: break;  // 4, 6, 10: freestream
+    }
+    if (bad) return 2;
+    // Get_TotalEnergy, Euler3D/src/Material.cpp:1012-1040
+    T et = ub * ub + vb * vb + wb * wb;
+    et = pb / ((g - T(1.0)) * rb) + T(0.5) * et;
+    QB[0] = rb;
+    QB[1] = rb * ub;
+    QB[2] = rb * vb;
+    QB[3] = rb * wb;
+    QB[4] = rb * et;
+    return pb < T(0.0) ? 1 : 0;
+}
+
 template <class T>
 __device__ __forceinline__ T hllc_flux(T g, T sx, T sy, T sz, const T *QL, const T *QR, T *F, int &bad) {
     T nx, ny, nz;

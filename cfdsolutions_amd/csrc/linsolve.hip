@@ -195,6 +195,228 @@ __global__ __launch_bounds__(128) void k_jacobian(JacArgs<TA> a) {
     }
 }
 
+// Compute_RoeFlux, Euler3D/src/Roe_Fluxes.cpp:552-905 (Roe branch,
+// conservative variables, no time term), returned as Conv + Diss per
+// component: Conv = (F_L + F_R)|S|/2, Diss = -(M P |Lambda| P^-1 M^-1)(Q_R - Q_L)|S|/2.
+__device__ __forceinline__ void roe_flux_matrix(double g, int efix, const double *QL, const double *QR, double sx,
+                                                double sy, double sz, double *F) {
+    double nx, ny, nz;
+    const double area = area_normal(sx, sy, sz, nx, ny, nz);
+    Eos<double> L, R;
+    eos_face(g, QL, nx, ny, nz, L);
+    eos_face(g, QR, nx, ny, nz, R);
+    double fL[5], fR[5];
+    fL[0] = L.rho * L.ubar;
+    fL[1] = L.u * fL[0] + L.p * nx;
+    fL[2] = L.v * fL[0] + L.p * ny;
+    fL[3] = L.w * fL[0] + L.p * nz;
+    fL[4] = L.ht * fL[0];
+    fR[0] = R.rho * R.ubar;
+    fR[1] = R.u * fR[0] + R.p * nx;
+    fR[2] = R.v * fR[0] + R.p * ny;
+    fR[3] = R.w * fR[0] + R.p * nz;
+    fR[4] = R.ht * fR[0];
+    double A[25];
+    roe_ajac(g, efix, QL, QR, nx, ny, nz, A);
+    double dQ[5];
+#pragma unroll
+    for (int t = 0; t < 5; t++) dQ[t] = QR[t] - QL[t];
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        double fa = 0.0;  // MC_Matrix_Mul_Vector
+#pragma unroll
+        for (int t = 0; t < 5; t++) fa += A[5 * r + t] * dQ[t];
+        const double conv = 0.5 * (fL[r] + fR[r]) * area;
+        const double diss = -0.5 * fa * area;
+        F[r] = conv + diss;
+    }
+}
+
+struct FdArgs {
+    int N, NT;
+    const int *order;
+    const int *ip, *sptr, *nbr, *soff, *iau;
+    const double *isx, *isy, *isz, *qa, *qb, *vol, *dt, *res;
+    const int *btype, *bcase;
+    const double *case_inf;
+    double *a;
+    float *af;
+    double *b;
+    double g, frac, coeff;
+    int efix, rhs_mode, add_time, order2, lim, plus, minus, iter, zpg;
+    BcFree<double> free;
+    int *flag;
+};
+
+// Face states of an interior edge: Compute_SecondOrderReconstructQ
+// (Euler3D/src/HigherOrderReconstructQ.cpp:20-71) for second order, the node
+// values for first order.  o: this node, n: its neighbour, iL: this node is
+// the L end.  Component c of the end `pend` (0 L, 1 R) is perturbed by dq if
+// perturb is set.  All choices are selects, so nothing is indexed at run time.
+struct FdNode {
+    double q[5], x[3], g[15], p[5];
+};
+
+__device__ __forceinline__ void fd_faces(const FdArgs &a, const FdNode &o, const FdNode &n, bool iL, bool perturb,
+                                         int pend, int c, double dq, double *QL, double *QR) {
+    double qL[5], qR[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const double l = iL ? o.q[m] : n.q[m], r = iL ? n.q[m] : o.q[m];
+        qL[m] = (perturb && pend == 0 && m == c) ? l + dq : l;
+        qR[m] = (perturb && pend == 1 && m == c) ? r + dq : r;
+    }
+    if (!a.order2) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            QL[m] = qL[m];
+            QR[m] = qR[m];
+        }
+        return;
+    }
+    const double xl0 = iL ? o.x[0] : n.x[0], xl1 = iL ? o.x[1] : n.x[1], xl2 = iL ? o.x[2] : n.x[2];
+    const double xr0 = iL ? n.x[0] : o.x[0], xr1 = iL ? n.x[1] : o.x[1], xr2 = iL ? n.x[2] : o.x[2];
+    const double RxL = 0.5 * (xr0 - xl0), RyL = 0.5 * (xr1 - xl1), RzL = 0.5 * (xr2 - xl2);
+    const double RxR = -RxL, RyR = -RyL, RzR = -RzL;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const double pL = a.lim ? (iL ? o.p[k] : n.p[k]) : 1.0, pR = a.lim ? (iL ? n.p[k] : o.p[k]) : 1.0;
+        const double glx = iL ? o.g[3 * k] : n.g[3 * k], gly = iL ? o.g[3 * k + 1] : n.g[3 * k + 1],
+                     glz = iL ? o.g[3 * k + 2] : n.g[3 * k + 2];
+        const double grx = iL ? n.g[3 * k] : o.g[3 * k], gry = iL ? n.g[3 * k + 1] : o.g[3 * k + 1],
+                     grz = iL ? n.g[3 * k + 2] : o.g[3 * k + 2];
+        QL[k] = qL[k] + pL * (a.frac * (qR[k] - qL[k]) + (1.0 - a.frac) * (glx * RxL + gly * RyL + glz * RzL));
+        QR[k] = qR[k] + pR * (a.frac * (qL[k] - qR[k]) + (1.0 - a.frac) * (grx * RxR + gry * RyR + grz * RzR));
+    }
+}
+
+// Loads the record of node j: state, coordinates, and for second order the
+// gradient and limiter.
+__device__ __forceinline__ void fd_load(const FdArgs &a, int j, FdNode &r) {
+    const double *ra = a.qa + kQA * (size_t)j;
+#pragma unroll
+    for (int m = 0; m < 5; m++) r.q[m] = ra[m];
+#pragma unroll
+    for (int m = 0; m < 3; m++) r.x[m] = ra[5 + m];
+    if (a.order2 && j < a.N) {
+        const double *rb = a.qb + kQB * (size_t)j;
+#pragma unroll
+        for (int f = 0; f < 15; f++) r.g[f] = rb[f];
+#pragma unroll
+        for (int m = 0; m < 5; m++) r.p[m] = rb[kPHI + m];
+    } else {
+#pragma unroll
+        for (int f = 0; f < 15; f++) r.g[f] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) r.p[m] = 1.0;
+    }
+}
+
+// Compute_Jacobian_FiniteDifference_Roe, Euler3D/src/Roe_Jacobian.cpp:236-776,
+// assembled by rows with ten lanes per node: lane (role, c) walks the node's
+// incidences in the reference's edge order and forms column c of
+//   role 0: the derivative of each incident flux with respect to this node,
+//           summed into the diagonal block (+ as L, - as R, then boundary
+//           edges, central, ghost recomputed per perturbation);
+//   role 1: the derivative with respect to the neighbour, written to the
+//           off-diagonal block (0 + dF/dR as L, 0 - dF/dL as R).
+// Each column costs two flux evaluations per incidence: perturbed +/- eps,
+// or perturbed and unperturbed for the one-sided methods (the reference
+// evaluates the unperturbed flux once per edge end and reuses it; the value
+// is the same).
+__global__ __launch_bounds__(128) void k_jacobian_fd(FdArgs a) {
+    const int t = blockIdx.x * 128 + threadIdx.x;
+    const int i = t / 10;  // internal node
+    if (i >= a.N) return;
+    const int role = (t % 10) / 5, c = t % 5;
+    const int ai = a.order[i];
+    const double eps = 1.0E-8;
+    FdNode own;
+    fd_load(a, i, own);
+    double D[5];  // column c of the diagonal block
+#pragma unroll
+    for (int r = 0; r < 5; r++) D[r] = (a.add_time && r == c) ? a.vol[i] / a.dt[i] : 0.0;
+    const int deg = a.ip[i + 1] - a.ip[i];
+    size_t idx = (size_t)a.sptr[i / kSlice] + (i % kSlice);
+    for (int k = 0; k < deg; k++, idx += kSlice) {
+        const int jr = ld_stream(a.nbr + idx);
+        const bool iL = jr >= 0;  // ghosts (jr >= N) always pair with a physical L
+        const int j = iL ? jr : ~jr;
+        const double sx = ld_stream(a.isx + idx), sy = ld_stream(a.isy + idx), sz = ld_stream(a.isz + idx);
+        if (j >= a.N) {  // boundary edge: central, this node perturbed
+            if (role) continue;
+            const int be = j - a.N;
+            const double *rg = a.qa + kQA * (size_t)j;
+            double qg[5];
+#pragma unroll
+            for (int m = 0; m < 5; m++) qg[m] = rg[m];
+            const BcFree<double> fr = a.case_inf ? bc_free_of_case<double>(a.case_inf, a.bcase[be]) : a.free;
+            double Fp[2][5];
+#pragma unroll
+            for (int pm = 0; pm < 2; pm++) {
+                double QP[5], QG[5];
+#pragma unroll
+                for (int m = 0; m < 5; m++) QP[m] = (m == c) ? (pm ? own.q[m] - eps : own.q[m] + eps) : own.q[m];
+                const int st = bc_ghost(a.g, QP, qg, sx, sy, sz, a.btype[be], a.iter, a.zpg, fr, QG);
+                if (st == 2) {
+                    atomicAdd(a.flag + 3, 1);
+#pragma unroll
+                    for (int m = 0; m < 5; m++) QG[m] = qg[m];
+                } else if (st == 1) {
+                    atomicAdd(a.flag + 0, 1);
+                }
+                roe_flux_matrix(a.g, a.efix, QP, QG, sx, sy, sz, Fp[pm]);
+            }
+#pragma unroll
+            for (int r = 0; r < 5; r++) D[r] += 0.5 * (Fp[0][r] - Fp[1][r]) / eps;
+            continue;
+        }
+        FdNode nb;
+        fd_load(a, j, nb);
+        // the perturbed end: this node (role 0) or the neighbour (role 1)
+        const int pend = ((role == 0) == iL) ? 0 : 1;
+        double FP[5], FM[5];
+        {
+            double QL[5], QR[5];
+            fd_faces(a, own, nb, iL, a.plus, pend, c, eps, QL, QR);
+            roe_flux_matrix(a.g, a.efix, QL, QR, sx, sy, sz, FP);
+            fd_faces(a, own, nb, iL, a.minus, pend, c, -eps, QL, QR);
+            roe_flux_matrix(a.g, a.efix, QL, QR, sx, sy, sz, FM);
+        }
+        double col[5];
+#pragma unroll
+        for (int r = 0; r < 5; r++) col[r] = a.coeff * (FP[r] - FM[r]) / eps;
+        if (role == 0) {
+#pragma unroll
+            for (int r = 0; r < 5; r++) D[r] = iL ? D[r] + col[r] : D[r] - col[r];
+        } else {
+            const size_t o = 25 * (size_t)ld_stream(a.soff + idx) + c;
+#pragma unroll
+            for (int r = 0; r < 5; r++) {
+                const double v = iL ? 0.0 + col[r] : 0.0 - col[r];
+                if (a.af) a.af[o + 5 * r] = (float)v;
+                else a.a[o + 5 * r] = v;
+            }
+        }
+    }
+    if (role == 0) {
+        const size_t o = 25 * (size_t)a.iau[ai] + c;
+#pragma unroll
+        for (int r = 0; r < 5; r++) {
+            if (a.af) a.af[o + 5 * r] = (float)D[r];
+            else a.a[o + 5 * r] = D[r];
+        }
+        if (c == 0) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                double r = a.res[kRES * (size_t)i + k];
+                if (a.rhs_mode == 1) r = r + a.res[kRES * (size_t)i + 5 + k];
+                a.b[5 * (size_t)ai + k] = -r;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Dense 5x5 helpers (same operation order as the CPU oracle)
 // ---------------------------------------------------------------------------
@@ -928,6 +1150,56 @@ int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed) {
                           rhs_mode};
         hipLaunchKernelGGL(k_jacobian<double>, dim3(nb), dim3(128), 0, m->stream, a);
     }
+    CFD_HIP(m, hipGetLastError());
+    m->jac_valid = true;
+    m->jac_mixed = mixed != 0;
+    m->prec_valid = false;
+    return CFD_OK;
+}
+
+int launch_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, int add_time, int rhs_mode,
+                       int mixed) {
+    int fdm = method;
+    if (fdm == CFD_JAC_FD_ALTERNATE) fdm = (iter % 2 == 0) ? CFD_JAC_FD_FORWARD : CFD_JAC_FD_BACKWARD;
+    FdArgs a{};
+    a.N = m->N;
+    a.NT = m->NT;
+    a.order = m->order;
+    a.ip = m->inc_ptr;
+    a.sptr = m->sptr;
+    a.nbr = m->nbr;
+    a.soff = m->soff;
+    a.iau = m->iau;
+    a.isx = m->d.isx;
+    a.isy = m->d.isy;
+    a.isz = m->d.isz;
+    a.qa = m->d.qa;
+    a.qb = m->d.qb;
+    a.vol = m->d.vol;
+    a.dt = m->dt;
+    a.res = m->d.res;
+    a.btype = m->btype;
+    a.bcase = m->bcase;
+    a.case_inf = m->case_inf;
+    a.a = mixed ? nullptr : m->a;
+    a.af = mixed ? m->af : nullptr;
+    a.b = m->b;
+    a.g = p->gamma;
+    a.frac = p->limiter_order == 2 ? 0.5 : 0.0;
+    a.coeff = fdm == CFD_JAC_FD_CENTRAL ? 0.5 : 1.0;
+    a.efix = p->entropy_fix;
+    a.rhs_mode = rhs_mode;
+    a.add_time = add_time;
+    a.order2 = p->order == 2;
+    a.lim = p->limiter != CFD_LIM_NONE;
+    a.plus = fdm == CFD_JAC_FD_CENTRAL || fdm == CFD_JAC_FD_FORWARD;
+    a.minus = fdm == CFD_JAC_FD_CENTRAL || fdm == CFD_JAC_FD_BACKWARD;
+    a.iter = iter;
+    a.zpg = p->zpg_iter;
+    a.free = BcFree<double>{p->inf_rho, p->inf_u, p->inf_v, p->inf_w, p->inf_p, p->outflow_p};
+    a.flag = m->flag;
+    const int nb = (int)((10 * (size_t)m->N + 127) / 128);
+    hipLaunchKernelGGL(k_jacobian_fd, dim3(nb), dim3(128), 0, m->stream, a);
     CFD_HIP(m, hipGetLastError());
     m->jac_valid = true;
     m->jac_mixed = mixed != 0;

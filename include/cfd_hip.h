@@ -210,6 +210,24 @@ int cfd_residual_rms(cfd_mesh *mesh, double *rms);
  * first-order approximate Roe Jacobian with V/dt on the diagonal.
  * rhs_mode 0: B = -Conv (Euler3D), 1: B = -(Conv+Diss). */
 int cfd_jacobian(cfd_mesh *mesh, const cfd_params *p, int rhs_mode, int mixed);
+/* Compute_Jacobian_FiniteDifference_Roe (Euler3D/src/Roe_Jacobian.cpp:236-776),
+ * selected by Compute_Jacobian_Roe (Euler3D/src/Roe_Jacobian.cpp:780-804) for
+ * the central, forward, backward and alternate methods: finite differences
+ * (eps 1e-8) of the matrix-form Roe flux Compute_RoeFlux
+ * (Euler3D/src/Roe_Fluxes.cpp:552-905) with respect to each end's state;
+ * second-order face states when p->order is 2.  Boundary edges are always
+ * central, with the characteristic ghost state recomputed for each
+ * perturbation.  CFD_JAC_FD_ALTERNATE is forward on even iterations and
+ * backward on odd ones.  add_time puts V/dt on the diagonal; rhs_mode as for
+ * cfd_jacobian.  Needs the FP64 state path (p->precision CFD_FP64) and a
+ * current residual (and gradient and limiter for second order); mixed stores
+ * the blocks in FP32. */
+#define CFD_JAC_FD_CENTRAL 0
+#define CFD_JAC_FD_FORWARD 1
+#define CFD_JAC_FD_BACKWARD 2
+#define CFD_JAC_FD_ALTERNATE 3
+int cfd_jacobian_fd(cfd_mesh *mesh, const cfd_params *p, int method, int iter, int add_time, int rhs_mode,
+                    int mixed);
 /* Copies the BSR structure (ia[N+1], ja[nnz], iau[N]; any may be NULL) and
  * returns nnz. */
 int32_t cfd_bsr_structure(cfd_mesh *mesh, int32_t *ia, int32_t *ja, int32_t *iau);

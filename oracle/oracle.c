@@ -873,6 +873,181 @@ void orc_jacobian(const orc_mesh *m, const orc_params *p, const double *q, const
     }
 }
 
+/* Compute_RoeFlux, Euler3D/src/Roe_Fluxes.cpp:552-905: Roe branch, conservative
+ * variables, AddTime FALSE.  The dissipation is the matrix form
+ * -1/2 (M P |Lambda| P^-1 M^-1)(Q_R - Q_L)|S|, with the matrix built exactly as
+ * Compute_RoeAJacobian builds it (same products in the same order). */
+static void roe_flux_matrix(const orc_params *P, const double *s, const double *QL, const double *QR,
+                            double *conv, double *diss) {
+    const double g = P->gamma;
+    double n[3];
+    const double area = area_normal(s, n);
+    eos_t L, R;
+    eos_face(g, QL, n[0], n[1], n[2], &L);
+    eos_face(g, QR, n[0], n[1], n[2], &R);
+    double fL[5], fR[5];
+    fL[0] = L.rho * L.ubar;
+    fL[1] = L.u * fL[0] + L.p * n[0];
+    fL[2] = L.v * fL[0] + L.p * n[1];
+    fL[3] = L.w * fL[0] + L.p * n[2];
+    fL[4] = L.ht * fL[0];
+    fR[0] = R.rho * R.ubar;
+    fR[1] = R.u * fR[0] + R.p * n[0];
+    fR[2] = R.v * fR[0] + R.p * n[1];
+    fR[3] = R.w * fR[0] + R.p * n[2];
+    fR[4] = R.ht * fR[0];
+    double A[5][5];
+    roe_ajac(P, QL, QR, n, A);
+    double dQ[5];
+    for (int i = 0; i < 5; i++) dQ[i] = QR[i] - QL[i];
+    for (int i = 0; i < 5; i++) {
+        double fa = 0.0; /* MC_Matrix_Mul_Vector */
+        for (int j = 0; j < 5; j++) fa += A[i][j] * dQ[j];
+        conv[i] = 0.5 * (fL[i] + fR[i]) * area;
+        diss[i] = -0.5 * fa * area;
+    }
+}
+
+double orc_edge_flux_matrix(const orc_params *p, const double *s, const double *ql, const double *qr, double *conv,
+                            double *diss) {
+    roe_flux_matrix(p, s, ql, qr, conv, diss);
+    return 0.0;
+}
+
+/* The states Compute_RoeFlux takes for edge (nl, nr): second-order
+ * reconstruction between physical nodes, node values otherwise. */
+static void fd_flux(const orc_mesh *m, const orc_params *P, const double *q, const double *grad, const double *phi,
+                    int nl, int nr, const double *s, double *conv, double *diss) {
+    const int N = m->n_node, NT = N + m->n_bedge;
+    double QL[5], QR[5];
+    if (nr < N && P->order == 2) {
+        reconstruct(m, P, q, grad, phi, nl, nr, QL, QR);
+    } else {
+        for (int k = 0; k < 5; k++) {
+            QL[k] = q[(size_t)k * NT + nl];
+            QR[k] = q[(size_t)k * NT + nr];
+        }
+    }
+    roe_flux_matrix(P, s, QL, QR, conv, diss);
+}
+
+/* Compute_Jacobian_FiniteDifference_Roe, Euler3D/src/Roe_Jacobian.cpp:236-776.
+ * Each node value is perturbed in place (a working copy of q) and restored,
+ * as the reference perturbs Q1..Q5.  One-sided methods evaluate the
+ * unperturbed flux with the first component and reuse it for the others;
+ * boundary edges are always central, with the characteristic ghost state
+ * recomputed from the perturbed node (Compute_Characteristic_BoundaryCondition,
+ * Euler3D/src/BC.cpp:33-380, the same ghost as bc_edge for gauge pressure 0)
+ * and restored after each flux.  Returns the number of boundary evaluations
+ * with a negative ghost pressure (reference: error()). */
+int orc_jacobian_fd(const orc_mesh *m, const orc_params *p, const double *q0, const double *grad,
+                    const double *phi, const double *dt, const double *res_conv, const double *res_diss,
+                    int rhs_mode, int method, int iter, int add_time, const int32_t *ia, const int32_t *ja,
+                    const int32_t *iau, double *a, double *b) {
+    const int N = m->n_node, NT = m->n_node + m->n_bedge;
+    const double eps = 1.0E-8;
+    double *q = (double *)malloc(sizeof(double) * 5 * (size_t)NT);
+    memcpy(q, q0, sizeof(double) * 5 * (size_t)NT);
+    int bad = 0;
+    memset(a, 0, sizeof(double) * 25 * (size_t)ia[N]);
+    for (int i = 0; i < N; i++) {
+        for (int k = 0; k < 5; k++) {
+            double r = res_conv[(size_t)k * N + i];
+            if (rhs_mode == 1 && res_diss) r = r + res_diss[(size_t)k * N + i];
+            b[5 * (size_t)i + k] = -r;
+        }
+        if (add_time)
+            for (int j = 0; j < 5; j++) a[25 * (size_t)iau[i] + 5 * j + j] = m->vol[i] / dt[i];
+    }
+    int fdm = method;
+    if (fdm == ORC_FD_ALTERNATE) fdm = (iter % 2 == 0) ? ORC_FD_FORWARD : ORC_FD_BACKWARD;
+    const double coeff = (fdm == ORC_FD_CENTRAL) ? 0.5 : 1.0;
+    const int plus = (fdm == ORC_FD_CENTRAL) || (fdm == ORC_FD_FORWARD);
+    const int minus = (fdm == ORC_FD_CENTRAL) || (fdm == ORC_FD_BACKWARD);
+    for (int e = 0; e < m->n_edge; e++) {
+        const int nl = m->edge[2 * e], nr = m->edge[2 * e + 1];
+        const double *s = m->edge_area + 3 * (size_t)e;
+        double d[2][5][5];
+        for (int side = 0; side < 2; side++) {
+            const int node = side ? nr : nl;
+            double FPc[5], FPd[5], FMc[5], FMd[5];
+            for (int c = 0; c < 5; c++) {
+                double *qc = q + (size_t)c * NT + node;
+                const double save = *qc;
+                if (c == 0) {
+                    *qc = plus ? save + eps : save;
+                    fd_flux(m, p, q, grad, phi, nl, nr, s, FPc, FPd);
+                    *qc = minus ? save - eps : save;
+                    fd_flux(m, p, q, grad, phi, nl, nr, s, FMc, FMd);
+                } else {
+                    if (plus) {
+                        *qc = save + eps;
+                        fd_flux(m, p, q, grad, phi, nl, nr, s, FPc, FPd);
+                    }
+                    if (minus) {
+                        *qc = save - eps;
+                        fd_flux(m, p, q, grad, phi, nl, nr, s, FMc, FMd);
+                    }
+                }
+                for (int i = 0; i < 5; i++) d[side][i][c] = coeff * ((FPc[i] + FPd[i]) - (FMc[i] + FMd[i])) / eps;
+                *qc = save;
+            }
+        }
+        int ofL = -1, ofR = -1;
+        for (int k = ia[nl]; k < ia[nl + 1]; k++)
+            if (ja[k] == nr) {
+                ofL = k;
+                break;
+            }
+        for (int k = ia[nr]; k < ia[nr + 1]; k++)
+            if (ja[k] == nl) {
+                ofR = k;
+                break;
+            }
+        double *AdL = a + 25 * (size_t)iau[nl], *AdR = a + 25 * (size_t)iau[nr];
+        double *AoL = a + 25 * (size_t)ofL, *AoR = a + 25 * (size_t)ofR;
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) {
+                AdL[5 * i + j] += d[0][i][j];
+                AdR[5 * i + j] -= d[1][i][j];
+                AoL[5 * i + j] += d[1][i][j];
+                AoR[5 * i + j] -= d[0][i][j];
+            }
+    }
+    for (int e = 0; e < m->n_bedge; e++) {
+        const int nl = m->bedge[2 * e], ng = m->bedge[2 * e + 1];
+        const double *s = m->bedge_area + 3 * (size_t)e;
+        double QG[5];
+        for (int k = 0; k < 5; k++) QG[k] = q[(size_t)k * NT + ng];
+        double d[5][5];
+        for (int c = 0; c < 5; c++) {
+            double F[2][2][5];
+            double *qc = q + (size_t)c * NT + nl;
+            const double save = *qc;
+            for (int pm = 0; pm < 2; pm++) {
+                *qc = pm ? save - eps : save + eps;
+                double QL[5], QB[5];
+                for (int k = 0; k < 5; k++) {
+                    QL[k] = q[(size_t)k * NT + nl];
+                    QB[k] = QG[k];
+                }
+                bad += bc_edge(p, QL, QG, s, m->bedge_type[e], iter, QB);
+                for (int k = 0; k < 5; k++) q[(size_t)k * NT + ng] = QB[k];
+                fd_flux(m, p, q, grad, phi, nl, ng, s, F[pm][0], F[pm][1]);
+                for (int k = 0; k < 5; k++) q[(size_t)k * NT + ng] = QG[k];
+            }
+            for (int i = 0; i < 5; i++)
+                d[i][c] = 0.5 * ((F[0][0][i] + F[0][1][i]) - (F[1][0][i] + F[1][1][i])) / eps;
+            *qc = save;
+        }
+        double *AdL = a + 25 * (size_t)iau[nl];
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) AdL[5 * i + j] += d[i][j];
+    }
+    free(q);
+    return bad;
+}
+
 /* MC_Iterative_Block_LU_Jacobi_CRS, forward direction,
  * MATH/src/MC_Iterative_Jacobi.c:346-535 */
 double orc_block_gs(int32_t n, const int32_t *ia, const int32_t *ja, const int32_t *iau, const double *a,

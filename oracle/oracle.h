@@ -97,6 +97,22 @@ void orc_jacobian(const orc_mesh *m, const orc_params *p, const double *q, const
                   const double *res_conv, const double *res_diss, int rhs_mode, const int32_t *ia,
                   const int32_t *ja, const int32_t *iau, double *a, double *b);
 
+/* Finite-difference Roe Jacobian (Compute_Jacobian_FiniteDifference_Roe,
+ * Euler3D/src/Roe_Jacobian.cpp:236-776) of the matrix-form Roe flux
+ * (Compute_RoeFlux, Euler3D/src/Roe_Fluxes.cpp:552-905), eps 1e-8.  method:
+ * ORC_FD_CENTRAL / FORWARD / BACKWARD / ALTERNATE (forward on even
+ * iterations, backward on odd); boundary edges are central.  q is [5][N+B]
+ * with the ghost states of the last boundary-condition pass; grad/phi are
+ * used for second order.  Returns the count of negative ghost pressures. */
+enum { ORC_FD_CENTRAL = 0, ORC_FD_FORWARD = 1, ORC_FD_BACKWARD = 2, ORC_FD_ALTERNATE = 3 };
+int orc_jacobian_fd(const orc_mesh *m, const orc_params *p, const double *q, const double *grad,
+                    const double *phi, const double *dt, const double *res_conv, const double *res_diss,
+                    int rhs_mode, int method, int iter, int add_time, const int32_t *ia, const int32_t *ja,
+                    const int32_t *iau, double *a, double *b);
+/* One edge of the matrix-form Roe flux (Compute_RoeFlux), for closed-form tests. */
+double orc_edge_flux_matrix(const orc_params *p, const double *s, const double *ql, const double *qr,
+                            double *conv, double *diss);
+
 /* Reference linear solvers on BSR (x [N][5]) */
 double orc_block_gs(int32_t n, const int32_t *ia, const int32_t *ja, const int32_t *iau, const double *a,
                     const double *b, int sweeps, double *x);

@@ -184,6 +184,30 @@ def jacobian(mesh, prm, q, dt, res_conv, res_diss, rhs_mode, ia, ja, iau):
     return a.reshape(-1, 25), b.reshape(-1, 5)
 
 
+FD_CENTRAL, FD_FORWARD, FD_BACKWARD, FD_ALTERNATE = 0, 1, 2, 3
+
+
+def jacobian_fd(mesh, prm, q, grad, phi, dt, res_conv, res_diss, rhs_mode, method, it, add_time, ia, ja, iau):
+    """Finite-difference Roe Jacobian (orc_jacobian_fd); returns (A blocks, B, negative ghost pressures)."""
+    a = np.zeros(25 * int(ia[-1]))
+    b = np.zeros(5 * mesh.n_node)
+    c = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.float64).ravel()
+    f = lib().orc_jacobian_fd
+    f.restype = ctypes.c_int
+    bad = f(ctypes.byref(mesh.s), ctypes.byref(prm), _p(c(q)), _p(c(grad)), _p(c(phi)), _p(c(dt)), _p(c(res_conv)),
+            _p(c(res_diss)), int(rhs_mode), int(method), int(it), int(add_time), _p(ia), _p(ja), _p(iau), _p(a),
+            _p(b))
+    return a.reshape(-1, 25), b.reshape(-1, 5), bad
+
+
+def edge_flux_matrix(prm, s, ql, qr):
+    """Matrix-form Roe flux of one edge (Compute_RoeFlux): (conv, diss)."""
+    conv, diss = np.zeros(5), np.zeros(5)
+    c = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    lib().orc_edge_flux_matrix(ctypes.byref(prm), _p(c(s)), _p(c(ql)), _p(c(qr)), _p(conv), _p(diss))
+    return conv, diss
+
+
 def block_gs(ia, ja, iau, a, b, sweeps):
     n = ia.size - 1
     x = np.zeros(5 * n)

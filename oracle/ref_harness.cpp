@@ -22,6 +22,7 @@
 #include "MC.h"
 #include "Material.h"
 #include "MeshIO.h"
+#include "Roe_Fluxes.h"
 #include "Solver.h"
 #include "SolverParameters.h"
 #include "Trim_Utils.h"
@@ -266,6 +267,27 @@ int main(int argc, char **argv) {
         for (int i = 0; i < N; i++)
             for (int r = 0; r < 5; r++) X[5 * (size_t)i + r] = SolverBlockMatrix.X[i][r];
         dump_d("lusgs_x", X, 5);
+        // finite-difference Jacobians of Compute_RoeFlux (Euler3D/src/Roe_Jacobian.cpp:236-776),
+        // every method of Compute_Jacobian_Roe (Euler3D/src/Roe_Jacobian.cpp:780-804)
+        const struct {
+            int method;
+            const char *name;
+        } fd[] = {{JACOBIAN_METHOD_CENTRAL, "central"}, {JACOBIAN_METHOD_FORWARD, "forward"},
+                  {JACOBIAN_METHOD_BACKWARD, "backward"}, {JACOBIAN_METHOD_ALTERNATE, "alternate"}};
+        for (const auto &f : fd) {
+            JacobianMethod = f.method;
+            Compute_Jacobian_FiniteDifference_Roe(TRUE, iter);
+            for (int b = 0; b < DIM; b++)
+                for (int r = 0; r < 5; r++)
+                    for (int c = 0; c < 5; c++) A[25 * (size_t)b + 5 * r + c] = SolverBlockMatrix.A[b][r][c];
+            for (int i = 0; i < N; i++)
+                for (int r = 0; r < 5; r++) Bv[5 * (size_t)i + r] = SolverBlockMatrix.B[i][r];
+            dump_d((std::string("jacfd_a_") + f.name).c_str(), A, 25);
+            dump_d((std::string("jacfd_b_") + f.name).c_str(), Bv, 5);
+        }
+        // the ghost states after the perturbations (the reference restores them)
+        dump_d("q_after_fd", gather5(Q1, Q2, Q3, Q4, Q5, N + NB), N + NB);
+        JacobianMethod = JACOBIAN_METHOD_APPROX;
     }
     printf("ref_harness: N=%d E=%d B=%d done\n", N, E, B);
     return 0;

@@ -48,6 +48,24 @@ CASES = {
 }
 
 
+# finite-difference Jacobians kept per case (all four methods where the
+# iteration parity exercises both branches of ALTERNATE; central elsewhere)
+FD_KEEP = {"bump10_roe1": ("central", "forward", "backward", "alternate"),
+           "bump10_roe2_venkat": ("central", "forward", "backward", "alternate"),
+           "cube5_roe2_bj_super": ("central",)}
+
+
+def prune_fd(name, data):
+    for k in [k for k in data if k.startswith("jacfd_b_")]:
+        assert np.array_equal(data[k], data["jac_b"]), k  # B = -Res for every method
+        del data[k]
+    data.pop("q_after_fd", None)
+    keep = FD_KEEP.get(name, ())
+    for k in [k for k in data if k.startswith("jacfd_a_")]:
+        if k[len("jacfd_a_"):] not in keep:
+            del data[k]
+
+
 def make_mesh(spec, path):
     if spec["kind"] == "bump":
         nx, ny, nz = spec["n"]
@@ -67,10 +85,12 @@ def main():
                     str(a["lorder"])]
             subprocess.run(args, check=True, stdout=subprocess.DEVNULL, cwd=td)
             data = {f[:-4]: np.load(os.path.join(td, f)) for f in os.listdir(td) if f.endswith(".npy")}
+        prune_fd(name, data)
         data["meta"] = np.array(json.dumps({"mesh": mspec, "args": a}))
         out = os.path.join(os.path.dirname(os.path.abspath(__file__)), name + ".npz")
         np.savez_compressed(out, **data)
-        print(name, {k: v.shape for k, v in data.items() if k in ("xyz", "edge", "jac_a")}, os.path.getsize(out))
+        print(name, {k: v.shape for k, v in data.items() if k in ("xyz", "edge", "jac_a") or k.startswith("jacfd")},
+              os.path.getsize(out))
 
 
 if __name__ == "__main__":

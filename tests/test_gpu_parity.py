@@ -125,6 +125,39 @@ def test_jacobian_and_reference_sweeps(case):
     assert_bitexact(dm.get(cfd.FIELD_X), d["lusgs_x"], "LU-SGS solution")
 
 
+FD_METHODS = {"central": cfd.JAC_FD_CENTRAL, "forward": cfd.JAC_FD_FORWARD, "backward": cfd.JAC_FD_BACKWARD,
+              "alternate": cfd.JAC_FD_ALTERNATE}
+
+
+def test_jacobian_finite_difference(case):
+    """cfd_jacobian_fd equals Compute_Jacobian_FiniteDifference_Roe bit for bit, every method in the fixture;
+    the FP32 store holds the same blocks rounded."""
+    d, dm, p = case
+    keys = sorted(k for k in d if k.startswith("jacfd_a_"))
+    if not keys:
+        pytest.skip("no finite-difference Jacobian in this fixture")
+    dm.set(cfd.FIELD_Q, d["q_bc"])
+    if "grad" in d:
+        dm.set(cfd.FIELD_GRAD, d["grad"])
+    if "phi" in d:
+        dm.set(cfd.FIELD_PHI, d["phi"])
+    dm.residual(p)  # marks the residual valid; the reference residual is restored below
+    dm.set(cfd.FIELD_RES_CONV, d["res_conv"])
+    dm.set(cfd.FIELD_RES_DISS, d["res_diss"])
+    dm.set(cfd.FIELD_DT, d["dt"])
+    it = d["meta"]["args"]["iter"]
+    for k in keys:
+        method = FD_METHODS[k[len("jacfd_a_"):]]
+        print("finite-difference method", k, flush=True)
+        dm.jacobian_fd(p, method=method, it=it, add_time=True, rhs_mode=0)
+        a, b = dm.jacobian_get()
+        assert_bitexact(a, d[k], "finite-difference Jacobian (%s)" % k)
+        assert_bitexact(b, d["jac_b"], "right-hand side")
+    dm.jacobian_fd(p, method=FD_METHODS[keys[0][len("jacfd_a_"):]], it=it, add_time=True, rhs_mode=0, mixed=True)
+    a32, _ = dm.jacobian_get()
+    assert np.array_equal(np.asarray(a32, np.float32), np.asarray(d[keys[0]], np.float32).reshape(a32.shape))
+
+
 def multicolour_order(n2n_ia, n2n_ja):
     """Greedy colouring of the node graph in index order; rows sorted by (colour, index)."""
     n = len(n2n_ia) - 1
