@@ -80,6 +80,33 @@ extern "C" const char *cfd_last_error(const cfd_ctx *ctx) {
 // ILU(0) merge map of a BSR pattern: for lower entry kk = (i, k) and the t-th
 // upper block of row k, mqq[moff[kk] + t] is the slot of row i in the same
 // column, or -1.
+// Pattern of a block-SELL layout (BlockSell) over rows 0..n-1: row i holds
+// the BSR slots rb(i) .. re(i)-1 in that order, with block columns ja[slot].
+template <class RB, class RE>
+static void build_block_sell(int n, RB rb, RE re, const std::vector<int> &ja, std::vector<int> &off,
+                             std::vector<int> &col, std::vector<int> &src) {
+    const int ns = (n + 63) / 64;
+    off.assign(ns + 1, 0);
+    for (int sl = 0; sl < ns; sl++) {
+        int w = 0;
+        for (int i = 64 * sl; i < std::min(n, 64 * sl + 64); i++) w = std::max(w, re(i) - rb(i));
+        off[sl + 1] = off[sl] + 64 * w;
+    }
+    col.assign(off[ns], 0);
+    src.assign(off[ns], -1);
+    for (int i = 0; i < n; i++) {
+        const int l = i % 64;
+        size_t q = (size_t)off[i / 64] + l;
+        for (int kk = rb(i); kk < re(i); kk++, q += 64) {
+            col[q] = ja[kk];
+            src[q] = kk;
+        }
+    }
+}
+
+static int upload_block_sell(cfd_mesh *m, const std::vector<int> &off, const std::vector<int> &col,
+                             const std::vector<int> &src, BlockSell &bs);
+
 static bool build_merge_map(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau, int N,
                             std::vector<int> &moff, std::vector<int> &mqq, int64_t &total) {
     moff.assign(ia[N], 0);
@@ -159,6 +186,16 @@ static T *upload(cfd_mesh *m, const T *h, size_t n, int *rc) {
         return nullptr;
     }
     return d;
+}
+
+static int upload_block_sell(cfd_mesh *m, const std::vector<int> &off, const std::vector<int> &col,
+                             const std::vector<int> &src, BlockSell &bs) {
+    int rc = 0;
+    bs.nblk = (int64_t)col.size();
+    bs.off = upload(m, off.data(), off.size(), &rc);
+    if (!rc && !col.empty()) bs.col = upload(m, col.data(), col.size(), &rc);
+    if (!rc && !src.empty()) bs.src = upload(m, src.data(), src.size(), &rc);
+    return rc;
 }
 
 extern "C" void cfd_mesh_destroy(cfd_mesh *m) {
@@ -423,6 +460,8 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             for (int i = 0; i < N; i++) cnt[colour[i] + 1]++;
             for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
             for (int i = 0; i < N; i++) mperm[cnt[colour[i]]++] = i;  // ascending index within a colour
+            m->mc_cstart.assign(nc + 1, 0);
+            for (int c = 0; c < nc; c++) m->mc_cstart[c + 1] = cnt[c];  // cnt[c] now ends colour c
             for (int t = 0; t < N; t++) miperm[mperm[t]] = t;
             std::vector<std::pair<int, int>> row;
             for (int t = 0; t < N; t++) {
@@ -505,6 +544,23 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         if (!rc) m->mc_src = upload(m, msrc.data(), msrc.size(), &rc);
         if (!rc) m->mc_moff = upload(m, mmoff.data(), mmoff.size(), &rc);
         if (!rc) m->mc_mqq = upload(m, mmqq.data(), mmqq.size(), &rc);
+        if (!rc) {  // block-SELL patterns of A and of the multicolour factors
+            std::vector<int> off, col, src;
+            const std::vector<int> &hia = m->h_ia, &hiau = m->h_iau;
+            build_block_sell(N, [&](int i) { return hia[i]; }, [&](int i) { return hia[i + 1]; }, m->h_ja, off, col,
+                             src);
+            rc = upload_block_sell(m, off, col, src, m->a_sell);
+            if (!rc) {
+                build_block_sell(N, [&](int t) { return mia[t]; }, [&](int t) { return miau[t]; }, mja, off, col, src);
+                rc = upload_block_sell(m, off, col, src, m->mcl_sell);
+            }
+            if (!rc) {
+                build_block_sell(N, [&](int t) { return miau[t] + 1; }, [&](int t) { return mia[t + 1]; }, mja, off,
+                                 col, src);
+                rc = upload_block_sell(m, off, col, src, m->mcu_sell);
+            }
+            (void)hiau;
+        }
         if (!rc) m->mc_lower.rows = upload(m, mlrows.data(), mlrows.size(), &rc);
         if (!rc) m->mc_upper.rows = upload(m, murows.data(), murows.size(), &rc);
         if (!rc) m->lower.rows = upload(m, lrows.data(), lrows.size(), &rc);
@@ -524,6 +580,10 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             m->sync_free = !(sf && sf[0] == '0');
             const char *gr = getenv("CFD_GRAPHS");
             m->use_graphs = !(gr && gr[0] == '0');
+            // one row per lane fills the GPU only with enough rows; smaller
+            // systems keep five lanes per row (CFD_SELL=0/1 forces either)
+            const char *se = getenv("CFD_SELL");
+            m->use_sell = se ? se[0] == '1' : N >= kSellMinRows;
         }
         if (rc) {
             cfd_mesh_destroy(m);
@@ -595,6 +655,11 @@ extern "C" int64_t cfd_footprint_bytes(int32_t N, int32_t E, int32_t B, int32_t 
     bytes += 8 * (1024 * 65 + 4 * 65 + 6) + 4 * 4;         // reduction scratch, eigenvalue bits, flags
     bytes += 25 * s * nnz * 2 + 25 * s * n;                 // Jacobian, ILU(0) factors, inverse pivots
     bytes += s * 5 * n * (2 * (int64_t)km + 2) + 8 * 5 * n; // Krylov V, Z, w; FP64 true residual
+    // block-SELL copies: A, and the multicolour factors (strict lower + upper
+    // blocks, inverted diagonals); slices padded to their longest row, taken
+    // as <= 25% of the blocks plus 64 slots per slice
+    const int64_t sell = nnz + nnz / 4 + 64 * (n / 64 + 1);
+    bytes += 2 * sell * (4 + 4 + 25 * s) + 25 * s * (n + 64);
     return bytes;
 }
 
@@ -662,6 +727,7 @@ extern "C" int cfd_mesh_reset(cfd_mesh *m, const cfd_params *p) {
     m->q_version++;
     m->jac_valid = false;
     m->prec_valid = false;
+    m->asell_valid = false;
     m->residual_valid = false;
     return CFD_OK;
 }
@@ -1034,6 +1100,7 @@ extern "C" int cfd_jacobian_set(cfd_mesh *m, const double *a, const double *b) {
     m->jac_valid = true;
     m->jac_mixed = false;
     m->prec_valid = false;
+    m->asell_valid = false;
     return CFD_OK;
 }
 

@@ -80,6 +80,23 @@ struct KrylovGraphs {
     std::vector<void *> exec;  // hipGraphExec_t
 };
 
+// Block rows in a sliced ELL layout with the 25 entries of a block spread
+// across lanes: the k-th block of row i (slice s = i/64, lane l = i%64) is
+// slot p = off[s] + 64k + l, and entry e of slot p is value 25(p - l) + 64e + l.
+// Each group of 64 slots is a [25][64] tile, so for every (k, e) the 64 rows
+// of a slice read one contiguous 64-value run.  Row lengths come from the
+// BSR pattern the layout mirrors; padding slots have src -1.
+// Rows from which the block-SELL kernels (one row per lane) are used: 2048
+// wavefronts, two per SIMD of the 256 CUs.
+constexpr int kSellMinRows = 131072;
+
+struct BlockSell {
+    int64_t nblk = 0;     // slots, a multiple of 64
+    int *off = nullptr;   // [nslice + 1]
+    int *col = nullptr;   // [nblk] block column (0 for padding)
+    int *src = nullptr;   // [nblk] BSR slot it is copied from, -1 for padding
+};
+
 struct cfd_mesh {
     cfd_ctx *ctx = nullptr;
     hipStream_t stream = nullptr;
@@ -144,6 +161,15 @@ struct cfd_mesh {
     bool sync_free = true;    // single-launch triangular sweeps (else one launch per level)
     int *ready = nullptr;     // [N] sweep completion epochs
     int *epoch = nullptr;     // [1] current sweep epoch
+    // block-SELL copies for the Krylov loop: A (SpMV), and the multicolour
+    // ILU(0) factors split into strict lower and upper parts plus the
+    // inverted diagonal blocks (one block per row, slot = row)
+    BlockSell a_sell, mcl_sell, mcu_sell;
+    double *asv = nullptr, *lsv = nullptr, *usv = nullptr, *dsv = nullptr;
+    float *asvf = nullptr, *lsvf = nullptr, *usvf = nullptr, *dsvf = nullptr;
+    bool asell_valid = false;    // A's SELL copy matches the current Jacobian
+    bool use_sell = false;       // Krylov loop on the block-SELL copies (else the BSR arrays)
+    std::vector<int> mc_cstart;  // [ncolor + 1] first permuted row of each colour
     bool jac_valid = false, jac_mixed = false;
     bool prec_valid = false;
     int prec_kind = -1;

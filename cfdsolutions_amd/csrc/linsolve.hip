@@ -784,6 +784,56 @@ __global__ __launch_bounds__(kBlock) void k_trsv_mc(int nrows, const int *__rest
     }
 }
 
+// One colour of the multicolour ILU(0) sweeps with the factors in block-SELL
+// layout: rows r0..r1-1 (permuted order, one colour, independent of each
+// other), one row per lane.  Forward: w_i = r_P(i) - sum over the strict lower
+// blocks L_ij w_j; backward: w_i = D_i^-1 (w_i - sum over the strict upper
+// blocks U_ij w_j), also scattered to z in original order.  Same operation
+// order as k_trsv_mc.
+template <class TM, class TV>
+__global__ __launch_bounds__(kBlock) void k_trsv_mc_sell(int r0, int r1, const int *__restrict__ ia,
+                                                         const int *__restrict__ iau, const int *__restrict__ perm,
+                                                         const int *__restrict__ off, const int *__restrict__ col,
+                                                         const TM *__restrict__ val, const TM *__restrict__ dv,
+                                                         const TV *__restrict__ rhs, TV *w, TV *z, int backward) {
+    const int i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+    if (i < r0 || i >= r1) return;
+    const int l = i & 63;
+    const int len = backward ? ia[i + 1] - iau[i] - 1 : iau[i] - ia[i];
+    TV y[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) y[r] = backward ? w[5 * (size_t)i + r] : rhs[5 * (size_t)perm[i] + r];
+    size_t p = (size_t)off[i >> 6] + l;
+    for (int k = 0; k < len; k++, p += 64) {
+        const TV *wj = w + 5 * (size_t)col[p];
+        TV wv[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) wv[c] = wj[c];
+        const TM *A = val + 25 * (p - l) + l;
+#pragma unroll
+        for (int r = 0; r < 5; r++) {
+            TV s = TV(0.0);
+#pragma unroll
+            for (int c = 0; c < 5; c++) s += (TV)A[64 * (5 * r + c)] * wv[c];
+            y[r] = y[r] - s;
+        }
+    }
+    if (!backward) {
+#pragma unroll
+        for (int r = 0; r < 5; r++) w[5 * (size_t)i + r] = y[r];
+        return;
+    }
+    const TM *D = dv + 25 * (size_t)(i - l) + l;
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        TV s = TV(0.0);
+#pragma unroll
+        for (int c = 0; c < 5; c++) s += (TV)D[64 * (5 * r + c)] * y[c];
+        w[5 * (size_t)i + r] = s;
+        z[5 * (size_t)perm[i] + r] = s;
+    }
+}
+
 // z = D^-1 r (block-Jacobi apply)
 template <class TM, class TV>
 __global__ __launch_bounds__(kBlock) void k_bjac_apply(int N, const TM *dinv, const TV *rhs, TV *z) {
@@ -815,6 +865,57 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int N, const int *__restrict__ 
     }
     if (bsub) s = (TY)bsub[t] - s;  // residual form r = b - A x
     y[t] = s;
+}
+
+// Copies BSR-ordered blocks (25 values each) into a block-SELL layout: slot
+// p takes block src[p]; padding slots are left as they are (never read).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_to_sell(size_t nblk, const int *__restrict__ src, const T *__restrict__ v,
+                                                    T *__restrict__ out) {
+    for (size_t p = blockIdx.x * (size_t)kBlock + threadIdx.x; p < nblk; p += (size_t)gridDim.x * kBlock) {
+        const int q = src ? src[p] : (int)p;
+        if (q < 0) continue;
+        const size_t l = p & 63;
+        T *o = out + 25 * (p - l) + l;
+        const T *in = v + 25 * (size_t)q;
+#pragma unroll
+        for (int e = 0; e < 25; e++) o[64 * e] = in[e];
+    }
+}
+
+// y = A x (or y = b - A x) with A in block-SELL layout, one row per lane.  Each
+// row's sum runs over its blocks in BSR order and each block's columns in
+// order, as k_spmv and the oracle add them, so the bits are the same.
+template <class TA, class TX, class TY>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, const int *__restrict__ ia, const int *__restrict__ off,
+                                                      const int *__restrict__ col, const TA *__restrict__ val,
+                                                      const TX *__restrict__ x, TY *__restrict__ y,
+                                                      const double *__restrict__ bsub) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int l = i & 63;
+    const int len = ia[i + 1] - ia[i];
+    size_t p = (size_t)off[i >> 6] + l;
+    TY s[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) s[r] = TY(0.0);
+    for (int k = 0; k < len; k++, p += 64) {
+        const TX *xj = x + 5 * (size_t)col[p];
+        TX xv[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) xv[c] = xj[c];
+        const TA *A = val + 25 * (p - l) + l;
+#pragma unroll
+        for (int r = 0; r < 5; r++)
+#pragma unroll
+            for (int c = 0; c < 5; c++) s[r] += (TY)A[64 * (5 * r + c)] * (TY)xv[c];
+    }
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        TY v = s[r];
+        if (bsub) v = (TY)bsub[5 * (size_t)i + r] - v;  // residual form r = b - A x
+        y[5 * (size_t)i + r] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1154,6 +1255,7 @@ int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed) {
     m->jac_valid = true;
     m->jac_mixed = mixed != 0;
     m->prec_valid = false;
+    m->asell_valid = false;
     return CFD_OK;
 }
 
@@ -1204,10 +1306,36 @@ int launch_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, i
     m->jac_valid = true;
     m->jac_mixed = mixed != 0;
     m->prec_valid = false;
+    m->asell_valid = false;
     return CFD_OK;
 }
 
 namespace {
+
+// Block-SELL value arrays by storage type.
+template <class TM>
+struct SellVals;
+template <>
+struct SellVals<double> {
+    static double *&a(cfd_mesh *m) { return m->asv; }
+    static double *&l(cfd_mesh *m) { return m->lsv; }
+    static double *&u(cfd_mesh *m) { return m->usv; }
+    static double *&d(cfd_mesh *m) { return m->dsv; }
+};
+template <>
+struct SellVals<float> {
+    static float *&a(cfd_mesh *m) { return m->asvf; }
+    static float *&l(cfd_mesh *m) { return m->lsvf; }
+    static float *&u(cfd_mesh *m) { return m->usvf; }
+    static float *&d(cfd_mesh *m) { return m->dsvf; }
+};
+// The BSR Jacobian by storage type.
+template <class TM>
+const TM *bsr_a(cfd_mesh *m);
+template <>
+const double *bsr_a<double>(cfd_mesh *m) { return m->a; }
+template <>
+const float *bsr_a<float>(cfd_mesh *m) { return m->af; }
 
 template <class TM>
 int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
@@ -1219,6 +1347,15 @@ int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
             hipLaunchKernelGGL(k_ilu0_level<TM>, dim3((nr + 47) / 48), dim3(kBlock), 0, m->stream, nr,
                                m->mc_lower.rows + b0, m->mc_ia, m->mc_ja, m->mc_iau, m->mc_moff, m->mc_mqq, lu, dinv,
                                lu, dinv, m->flag);
+        }
+        // block-SELL copies for the sweeps
+        if (m->use_sell) {
+            hipLaunchKernelGGL(k_to_sell<TM>, dim3(grid_cap((size_t)m->mcl_sell.nblk, 8192)), dim3(kBlock), 0, m->stream,
+                               (size_t)m->mcl_sell.nblk, m->mcl_sell.src, lu, SellVals<TM>::l(m));
+            hipLaunchKernelGGL(k_to_sell<TM>, dim3(grid_cap((size_t)m->mcu_sell.nblk, 8192)), dim3(kBlock), 0, m->stream,
+                               (size_t)m->mcu_sell.nblk, m->mcu_sell.src, lu, SellVals<TM>::u(m));
+            hipLaunchKernelGGL(k_to_sell<TM>, dim3(grid_cap((size_t)m->N, 8192)), dim3(kBlock), 0, m->stream,
+                               (size_t)m->N, nullptr, dinv, SellVals<TM>::d(m));
         }
     } else if (kind == CFD_PREC_ILU0) {
         CFD_HIP(m, hipMemcpyAsync(lu, a, sizeof(TM) * 25 * (size_t)m->nnz, hipMemcpyDeviceToDevice, m->stream));
@@ -1236,6 +1373,17 @@ int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
     return CFD_OK;
 }
 
+// y = A x, or y = bsub - A x, with the block-SELL copy of the Jacobian.
+template <class TM, class TX, class TY>
+void spmv(cfd_mesh *m, const TX *x, TY *y, const double *bsub) {
+    if (m->use_sell)
+        hipLaunchKernelGGL((k_spmv_sell<TM, TX, TY>), dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream,
+                           m->N, m->ia, m->a_sell.off, m->a_sell.col, (const TM *)SellVals<TM>::a(m), x, y, bsub);
+    else
+        hipLaunchKernelGGL((k_spmv<TM, TX, TY>), dim3((int)((5 * (size_t)m->N + kBlock - 1) / kBlock)), dim3(kBlock),
+                           0, m->stream, m->N, m->ia, m->ja, bsr_a<TM>(m), x, y, bsub);
+}
+
 template <class TV>
 TV *mc_work(cfd_mesh *m);
 template <>
@@ -1246,7 +1394,7 @@ float *mc_work<float>(cfd_mesh *m) { return m->zpf; }
 template <class TM, class TV>
 void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r, TV *z) {
     const size_t L = 5 * (size_t)m->N;
-    if (kind == CFD_PREC_ILU0_MC) {
+    if (kind == CFD_PREC_ILU0_MC && !m->use_sell) {
         TV *wk = mc_work<TV>(m);
         for (int pass = 0; pass < 2; pass++) {
             const LevelSchedule &ls = pass ? m->mc_upper : m->mc_lower;
@@ -1255,6 +1403,21 @@ void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r
                 const int nwaves = (nr + 11) / 12;
                 hipLaunchKernelGGL((k_trsv_mc<TM, TV>), dim3((nwaves + 3) / 4), dim3(kBlock), 0, m->stream, nr,
                                    ls.rows + b0, m->mc_ia, m->mc_ja, m->mc_iau, m->mc_perm, lu, dinv, r, wk, z, pass);
+            }
+        }
+    } else if (kind == CFD_PREC_ILU0_MC) {
+        // one launch per colour: forward in colour order, backward in reverse
+        TV *wk = mc_work<TV>(m);
+        const TM *dv = SellVals<TM>::d(m);
+        for (int pass = 0; pass < 2; pass++) {
+            const BlockSell &bs = pass ? m->mcu_sell : m->mcl_sell;
+            const TM *val = pass ? SellVals<TM>::u(m) : SellVals<TM>::l(m);
+            for (int t = 0; t < m->ncolor; t++) {
+                const int c = pass ? m->ncolor - 1 - t : t;
+                const int r0 = m->mc_cstart[c], r1 = m->mc_cstart[c + 1];
+                const int g = (r1 - (r0 & ~63) + kBlock - 1) / kBlock;
+                hipLaunchKernelGGL((k_trsv_mc_sell<TM, TV>), dim3(g), dim3(kBlock), 0, m->stream, r0, r1, m->mc_ia,
+                                   m->mc_iau, m->mc_perm, bs.off, bs.col, val, dv, r, wk, z, pass);
             }
         }
     } else if (kind == CFD_PREC_ILU0 && m->sync_free) {
@@ -1329,8 +1492,7 @@ void arnoldi_step(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, TV *V,
         hipLaunchKernelGGL((k_scale_dev<TV, TV>), dim3(gN), dim3(kBlock), 0, m->stream, L, w, vj,
                            m->redout + 2 * (kMaxKrylov + 1));
     apply_prec<TM, TV>(m, m->prec_kind, lu, dinv, vj, zj);
-    hipLaunchKernelGGL((k_spmv<TM, TV, TV>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a, zj, w,
-                       nullptr);
+    spmv<TM, TV, TV>(m, zj, w, nullptr);
     // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
     mdot<TV>(m, j + 1, V, L, w, 0);
     hipLaunchKernelGGL(k_maxpy<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, V, L, m->redout, w);
@@ -1414,8 +1576,7 @@ void arnoldi_step_batch(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, 
         hipLaunchKernelGGL((k_scale_dev_seg<TV, TV>), dim3(gN), dim3(kBlock), 0, m->stream, L, m->ecase, w, vj,
                            m->bred, 2 * (kMaxKrylov + 1));
     apply_prec<TM, TV>(m, m->prec_kind, lu, dinv, vj, zj);
-    hipLaunchKernelGGL((k_spmv<TM, TV, TV>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a, zj, w,
-                       nullptr);
+    spmv<TM, TV, TV>(m, zj, w, nullptr);
     mdot_seg<TV>(m, j + 1, V, L, w, 0);
     hipLaunchKernelGGL(k_maxpy_seg<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, m->ecase, V, L, m->bred, 0, w);
     mdot_seg<TV>(m, j + 1, V, L, w, kMaxKrylov + 1);
@@ -1580,8 +1741,7 @@ int fgmres_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM 
         }
         if (!any) break;
         // true residual for the restart
-        hipLaunchKernelGGL((k_spmv<TM, double, double>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a,
-                           m->x, m->r, m->b);
+        spmv<TM, double, double>(m, m->x, m->r, m->b);
         mdot_seg<double>(m, 1, m->r, 0, m->r, 0);
         CFD_HIP(m, hipMemcpyAsync(m->hbred, m->bred, sizeof(double) * kBOut * K, hipMemcpyDeviceToHost, m->stream));
         CFD_HIP(m, hipStreamSynchronize(m->stream));
@@ -1665,8 +1825,7 @@ int fgmres(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM *lu, c
                            m->redout + 3 * (kMaxKrylov + 1), m->x);
         if (res <= tol) break;
         // true residual for the restart
-        hipLaunchKernelGGL((k_spmv<TM, double, double>), dim3(gS), dim3(kBlock), 0, m->stream, m->N, m->ia, m->ja, a,
-                           m->x, m->r, m->b);
+        spmv<TM, double, double>(m, m->x, m->r, m->b);
         mdot<double>(m, 1, m->r, 0, m->r, 0);
         CFD_HIP(m, hipMemcpyAsync(hr, m->redout, sizeof(double), hipMemcpyDeviceToHost, m->stream));
         CFD_HIP(m, hipStreamSynchronize(m->stream));
@@ -1783,6 +1942,23 @@ int run_linsolve_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const std::vecto
 namespace {
 
 // Jacobian precision check, Krylov workspace and preconditioner for FGMRES.
+// Allocates the block-SELL value arrays of storage type TM: A always, the
+// multicolour factors for that preconditioner.
+template <class TM>
+int ensure_sell(cfd_mesh *m, int precond) {
+    if (!m->use_sell) return CFD_OK;
+    TM *&av = SellVals<TM>::a(m);
+    if (!av) av = (TM *)cfd_dalloc(m, sizeof(TM) * 25 * (size_t)m->a_sell.nblk);
+    if (!av) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (block-SELL Jacobian)");
+    if (precond != CFD_PREC_ILU0_MC) return CFD_OK;
+    TM *&lv = SellVals<TM>::l(m), *&uv = SellVals<TM>::u(m), *&dv = SellVals<TM>::d(m);
+    if (!lv) lv = (TM *)cfd_dalloc(m, sizeof(TM) * 25 * (size_t)std::max<int64_t>(m->mcl_sell.nblk, 64));
+    if (!uv) uv = (TM *)cfd_dalloc(m, sizeof(TM) * 25 * (size_t)std::max<int64_t>(m->mcu_sell.nblk, 64));
+    if (!dv) dv = (TM *)cfd_dalloc(m, sizeof(TM) * 25 * (((size_t)m->N + 63) / 64 * 64));
+    if (!lv || !uv || !dv) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (block-SELL factors)");
+    return CFD_OK;
+}
+
 int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
     if (o->krylov_m < 1 || o->krylov_m > kMaxKrylov)
         return cfd_set_error(m->ctx, CFD_E_ARG, "krylov_m must be in [1, %d]", kMaxKrylov);
@@ -1793,6 +1969,18 @@ int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
         return cfd_set_error(m->ctx, CFD_E_ARG, "Jacobian precision does not match linsolve precision");
     int rc = ensure_krylov(m, o->krylov_m, mixed);
     if (rc) return rc;
+    rc = mixed ? ensure_sell<float>(m, o->precond) : ensure_sell<double>(m, o->precond);
+    if (rc) return rc;
+    if (m->use_sell && !m->asell_valid) {
+        if (mixed)
+            hipLaunchKernelGGL(k_to_sell<float>, dim3(grid_cap((size_t)m->a_sell.nblk, 8192)), dim3(kBlock), 0,
+                               m->stream, (size_t)m->a_sell.nblk, m->a_sell.src, m->af, m->asvf);
+        else
+            hipLaunchKernelGGL(k_to_sell<double>, dim3(grid_cap((size_t)m->a_sell.nblk, 8192)), dim3(kBlock), 0,
+                               m->stream, (size_t)m->a_sell.nblk, m->a_sell.src, m->a, m->asv);
+        CFD_HIP(m, hipGetLastError());
+        m->asell_valid = true;
+    }
     if (!m->prec_valid || m->prec_kind != o->precond || m->prec_mixed != mixed) {
         m->prec_kind = o->precond;
         m->prec_mixed = mixed;

@@ -64,6 +64,56 @@ def test_sync_free_sweeps_match_level_launches(ctx, mixed):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("precond", [cfd.PREC_ILU0_MC, cfd.PREC_ILU0, cfd.PREC_BJACOBI])
+def test_block_sell_matches_bsr(ctx, mixed, precond):
+    """The block-SELL SpMV and multicolour sweeps (one row per lane, used for large systems) give the bits of the
+    BSR kernels (five lanes per row): same FGMRES iterations, residual and solution; and a steady solve on them
+    gives the same state."""
+    print("block-SELL vs BSR, mixed", mixed, "precond", precond, flush=True)
+    m = cmesh.bump_channel(16, 6, 6)
+    p = cfd.make_params(mach=0.5, alpha_deg=1.0, flux="roe", order=1)
+    out = []
+    old = os.environ.get("CFD_SELL")
+    try:
+        for sell in ("0", "1"):
+            os.environ["CFD_SELL"] = sell
+            dm = _jacobian_system(ctx, m, p, True, mixed)
+            st = dm.linsolve(cfd.SOLVER_FGMRES, precond, krylov_m=30, max_restarts=3, rtol=1e-8, mixed=mixed)
+            out.append((st.iterations, st.final_res, dm.get(cfd.FIELD_X)))
+            dm.close()
+    finally:
+        if old is None:
+            del os.environ["CFD_SELL"]
+        else:
+            os.environ["CFD_SELL"] = old
+    assert out[0][0] == out[1][0]
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_block_sell_steady_solve_matches_bsr(ctx):
+    print("block-SELL vs BSR steady solve", flush=True)
+    m = cmesh.bump_channel(12, 4, 4)
+    p = cfd.make_params(mach=0.5, alpha_deg=1.0, flux="roe", order=1)
+    res = []
+    old = os.environ.get("CFD_SELL")
+    try:
+        for sell in ("0", "1"):
+            os.environ["CFD_SELL"] = sell
+            dm = cfd.DeviceMesh(ctx, m, p)
+            r, _ = dm.solve_steady(p, mixed=True, precond=cfd.PREC_ILU0_MC, max_iter=40, tol=1e-10, check=False)
+            res.append((r.iterations, r.krylov_total, dm.get(cfd.FIELD_Q)))
+            dm.close()
+    finally:
+        if old is None:
+            del os.environ["CFD_SELL"]
+        else:
+            os.environ["CFD_SELL"] = old
+    assert res[0][:2] == res[1][:2]
+    assert np.array_equal(res[0][2], res[1][2])
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 def test_graph_replays_match_eager(ctx, mixed):
     """Captured Arnoldi-step graphs, replayed over three Newton steps (new matrix values each time, eager work and
     synchronisations in between), give the bits of the eager launches at every replay."""
