@@ -242,13 +242,15 @@ def solve3_leg(ctx):
             "speedup_mixed": r0.seconds / r1.seconds}
 
 
-def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0.3, 0.85), aoa=(-2.0, 6.0),
-              fluxes=("roe", "hllc"), stream=True):
+def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, batch_max=8, seed=20261202, mach=(0.3, 0.85),
+              aoa=(-2.0, 6.0), fluxes=("roe", "hllc"), stream=True):
     """Steady-solve requests on bump channels of log-uniform size in [nmin, nmax] nodes, bump height U(0.05,
     0.15), Mach U(0.3, 0.85), AoA U(-2, 6) deg, Roe or HLLC (first order), to RMS 1e-10, FGMRES(30) with the
-    multicolour ILU(0).  The plain FP64 GPU path solves them one at a time (meshes resident); the server solves
-    them in mixed precision with continuous batching, first as a burst (capacity) and then as a Poisson stream at
-    80% of that capacity (latency)."""
+    multicolour ILU(0).  The plain FP64 GPU path solves them one at a time (meshes resident).  The server solves
+    them in mixed precision, first with one request per solve (meshes resident, `inflight` at once) as a burst,
+    then with continuous batching (up to batch_max compatible requests per union-mesh solve, a request waiting
+    at most 50 ms for companions while the GPU is busy) as a burst (capacity) and as a Poisson stream at 80% of
+    that capacity (latency)."""
     from cfdsolutions_amd import cfd, mesh
 
     rng = np.random.default_rng(seed)
@@ -279,15 +281,7 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
         d.close()
     print("bench: serve baseline fp64 %d solves in %.2f s" % (n_req, t64), file=sys.stderr, flush=True)
 
-    srv = cfd.Server(ctx, hbm_budget=200 << 30, max_inflight=inflight)
-    for k, m in enumerate(pool):  # structure resident before timing (one solve per mesh)
-        srv.submit(-1 - k, k, m, params(reqs[0]), alpha_deg=reqs[0][2], precond=cfd.PREC_ILU0_MC, mixed=True,
-                   **SOLVE_KW)
-    got = 0
-    while got < n_meshes:
-        got += len(srv.poll(wait=30.0))
-
-    def run(arrivals):
+    def run(srv, arrivals):
         t_start = srv.stats()["clock"]
         done, k, t0 = [], 0, time.perf_counter()
         while len(done) < n_req:
@@ -300,34 +294,51 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, seed=20261202, mach=(0
             wait = 0.0 if k >= n_req else max(0.0, arrivals[k] - (time.perf_counter() - t0))
             got = srv.poll(wait=min(wait, 0.05) if k < n_req else 1.0)
             for c in got:
-                print("bench: served request %d (%.2f s latency)" % (c["id"], c["t_done"] - c["t_submit"]),
-                      file=sys.stderr, flush=True)
+                print("bench: served request %d (%.2f s latency, batch of %d)" % (
+                    c["id"], c["t_done"] - c["t_submit"], c["batch_size"]), file=sys.stderr, flush=True)
             done += got
         t_end = max(c["t_done"] for c in done)
         lat = np.array([c["t_done"] - c["t_submit"] for c in done])
         return done, t_end - t_start, lat
 
-    burst, span_b, lat_b = run(np.zeros(n_req))
+    # one request per solve, meshes resident before timing (one solve per mesh)
+    srv = cfd.Server(ctx, hbm_budget=200 << 30, max_inflight=inflight)
+    for k, m in enumerate(pool):
+        srv.submit(-1 - k, k, m, params(reqs[0]), alpha_deg=reqs[0][2], precond=cfd.PREC_ILU0_MC, mixed=True,
+                   **SOLVE_KW)
+    got = 0
+    while got < n_meshes:
+        got += len(srv.poll(wait=30.0))
+    single, span_1, lat_1 = run(srv, np.zeros(n_req))
+    srv.close()
+    cap1 = n_req / span_1
+    print("bench: serve without batching %.2f solves/s" % cap1, file=sys.stderr, flush=True)
+
+    # continuous batching
+    srv = cfd.Server(ctx, hbm_budget=200 << 30, max_inflight=inflight, batch_max=batch_max, batch_wait_us=50000)
+    burst, span_b, lat_b = run(srv, np.zeros(n_req))
     cap = n_req / span_b
     if stream:
         gaps = rng.exponential(1.0 / (0.8 * cap), n_req)
-        stream, span_s, lat_s = run(np.cumsum(gaps) - gaps[0])
+        stream, span_s, lat_s = run(srv, np.cumsum(gaps) - gaps[0])
     else:
         stream, span_s, lat_s = burst, span_b, lat_b
     st = srv.stats()
     srv.close()
-    bad = [c for c in burst + stream if c["status"] != 0 or not c["result"].converged]
+    bad = [c for c in single + burst + stream if c["status"] != 0 or not c["result"].converged]
     ferr = 0.0
-    for c in stream:
+    for c in single + burst + stream:
         f64 = np.array(res64[c["id"]].force)
         ferr = max(ferr, float(np.max(np.abs(np.array(c["result"].force) - f64)) / np.max(np.abs(f64))))
     base = n_req / t64
     res = {"workload": "%d requests on %d bump channels of %d-%d nodes (log-uniform), 1st-order %s, "
-                       "M %.2f-%.2f, AoA %g..%g, RMS < 1e-10, %d in flight" % (
+                       "M %.2f-%.2f, AoA %g..%g, RMS < 1e-10, %d in flight, batches of up to %d" % (
                            n_req, n_meshes, min(m.n_node for m in pool), max(m.n_node for m in pool),
-                           "/".join(fluxes), mach[0], mach[1], aoa[0], aoa[1], inflight),
+                           "/".join(fluxes), mach[0], mach[1], aoa[0], aoa[1], inflight, batch_max),
            "fp64_sequential_solves_per_s": base,
            "serving_capacity_solves_per_s": cap,
+           "serving_without_batching_solves_per_s": cap1,
+           "mean_batch_size": float(np.mean([c["batch_size"] for c in burst + stream])),
            "serving_vs_fp64": cap / base,
            "stream_rate_per_s": 0.8 * cap,
            "stream_solves_per_s": n_req / span_s,
@@ -405,6 +416,7 @@ def main():
     ap.add_argument("--serve-meshes", type=int, default=6)
     ap.add_argument("--serve-nodes", type=str, default="10000,50000", help="min,max nodes of the serving meshes")
     ap.add_argument("--serve-inflight", type=int, default=12)
+    ap.add_argument("--serve-batch", type=int, default=8, help="requests per union-mesh batch in the server")
     ap.add_argument("--batch-cases", type=int, default=32,
                     help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")
     ap.add_argument("--batch-fp64-sample", type=int, default=32,
@@ -443,7 +455,8 @@ def main():
             out["solve_192x64x64"] = solve3_leg(ctx)
         if "serve" in legs:
             nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
-            out["serving"] = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight)
+            out["serving"] = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight,
+                                       args.serve_batch)
         if "batch" in legs and args.batch_cases:
             out["batched_small_cases"] = batch_leg(ctx, args.batch_cases, args.batch_fp64_sample)
         ctx.close()
@@ -511,7 +524,8 @@ def main():
             solves = solve_leg(ctx)
         if "serve" in legs:
             nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
-            serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight)
+            serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight,
+                                args.serve_batch)
         if "batch" in legs and args.batch_cases:
             batched = batch_leg(ctx, args.batch_cases, args.batch_fp64_sample)
     ctx.close()

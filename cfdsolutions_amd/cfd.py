@@ -74,7 +74,8 @@ class SolveResult(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("max_inflight", ctypes.c_int32), ("lookahead", ctypes.c_int32), ("max_bypass", ctypes.c_int32),
-                ("keep_resident", ctypes.c_int32), ("hbm_budget", ctypes.c_int64)]
+                ("keep_resident", ctypes.c_int32), ("hbm_budget", ctypes.c_int64), ("batch_max", ctypes.c_int32),
+                ("batch_nodes", ctypes.c_int32), ("batch_wait_us", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
@@ -84,8 +85,8 @@ class Request(ctypes.Structure):
 
 class Completion(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int64), ("status", ctypes.c_int32), ("reused", ctypes.c_int32),
-                ("result", SolveResult), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
-                ("t_done", ctypes.c_double), ("footprint", ctypes.c_int64), ("device_bytes", ctypes.c_int64),
+                ("batch_size", ctypes.c_int32), ("pad_", ctypes.c_int32), ("result", SolveResult),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("footprint", ctypes.c_int64), ("device_bytes", ctypes.c_int64),
                 ("message", ctypes.c_char * 160)]
 
 
@@ -333,10 +334,12 @@ class Server:
     submit() queues a steady solve; completions arrive through poll().  The
     arrays of a request are kept alive here until its completion is polled."""
 
-    def __init__(self, ctx, hbm_budget, max_inflight=4, lookahead=16, max_bypass=8, keep_resident=True):
+    def __init__(self, ctx, hbm_budget, max_inflight=4, lookahead=16, max_bypass=8, keep_resident=True, batch_max=0,
+                 batch_nodes=0, batch_wait_us=0):
         self.ctx = ctx
         self.h = ctypes.c_void_p()
-        o = ServerOpts(int(max_inflight), int(lookahead), int(max_bypass), int(bool(keep_resident)), int(hbm_budget))
+        o = ServerOpts(int(max_inflight), int(lookahead), int(max_bypass), int(bool(keep_resident)), int(hbm_budget),
+                       int(batch_max), int(batch_nodes), int(batch_wait_us), 0)
         ctx.check(lib().cfd_server_create(ctx.h, ctypes.byref(o), ctypes.byref(self.h)))
         self._live = {}
         self._descs = {}
@@ -359,7 +362,8 @@ class Server:
         for k in range(n):
             c = buf[k]
             _, q, hist = self._live.pop(int(c.id))
-            out.append(dict(id=int(c.id), status=int(c.status), reused=bool(c.reused), result=c.result,
+            out.append(dict(id=int(c.id), status=int(c.status), reused=bool(c.reused), batch_size=int(c.batch_size),
+                            result=c.result,
                             t_submit=c.t_submit, t_admit=c.t_admit, t_done=c.t_done, footprint=int(c.footprint),
                             device_bytes=int(c.device_bytes), message=c.message.decode(errors="replace"), q=q,
                             hist=hist[: c.result.iterations + 1]))

@@ -135,28 +135,29 @@ int build_segmap(cfd_mesh *m, SegMap &sg, const std::vector<int> &off, const std
     return CFD_OK;
 }
 
-bool same_scheme(const cfd_params &a, const cfd_params &b) {
+}  // namespace
+
+bool cfd_same_scheme(const cfd_params &a, const cfd_params &b) {
     return a.gamma == b.gamma && a.flux == b.flux && a.order == b.order && a.limiter == b.limiter &&
            a.limiter_order == b.limiter_order && a.entropy_fix == b.entropy_fix && a.gradient == b.gradient &&
            a.zpg_iter == b.zpg_iter && a.precision == b.precision && a.venkat_k == b.venkat_k &&
            a.cfl_min == b.cfl_min && a.cfl_max == b.cfl_max && a.cfl_ramp == b.cfl_ramp;
 }
 
-bool same_solve(const cfd_solve_opts &a, const cfd_solve_opts &b) {
+bool cfd_same_solve(const cfd_solve_opts &a, const cfd_solve_opts &b) {
     return a.max_iter == b.max_iter && a.tol == b.tol && a.first_order_iters == b.first_order_iters &&
            a.relaxation == b.relaxation && a.wall_tag == b.wall_tag && a.lin.solver == b.lin.solver &&
            a.lin.precond == b.lin.precond && a.lin.krylov_m == b.lin.krylov_m &&
            a.lin.max_restarts == b.lin.max_restarts && a.lin.rtol == b.lin.rtol && a.lin.mixed == b.lin.mixed;
 }
 
-}  // namespace
-
-extern "C" int cfd_batch_solve(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
-                               const cfd_solve_opts *opts, cfd_solve_result *results) {
+int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
+                         const cfd_solve_opts *opts, cfd_solve_result *results, double *const *q_out,
+                         int64_t *device_bytes) {
     if (!ctx || K < 1 || !meshes || !params || !opts || !results)
         return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: invalid arguments");
     for (int c = 1; c < K; c++)
-        if (!same_scheme(params[0], params[c]) || !same_solve(opts[0], opts[c]))
+        if (!cfd_same_scheme(params[0], params[c]) || !cfd_same_solve(opts[0], opts[c]))
             return cfd_set_error(ctx, CFD_E_ARG,
                                  "cfd_batch_solve: case %d differs from case 0 in scheme or solver options", c);
     const cfd_solve_opts &o = opts[0];
@@ -289,10 +290,15 @@ extern "C" int cfd_batch_solve(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *mes
         rc = cfd_field_set(m, CFD_FIELD_Q, q.data());
     }
     for (int c = 0; c < K; c++) results[c] = cfd_solve_result{};
+    // cases still iterating: the update and the boundary condition skip the
+    // others, so a finished case keeps its last iteration's state and ghosts
     int *d_active = nullptr;
     if (!rc) {
         d_active = (int *)cfd_dalloc(m, sizeof(int) * K);
         if (!d_active) rc = cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of device memory");
+        else if (hipMemcpy(d_active, act_i.data(), sizeof(int) * K, hipMemcpyHostToDevice) != hipSuccess)
+            rc = cfd_set_error(ctx, CFD_E_HIP, "hipMemcpy failed");
+        m->case_active = d_active;
     }
     // backward-Euler Newton iterations in lockstep (cfd_solve_steady per case)
     for (int it = 0; !rc && it < o.max_iter; it++) {
@@ -387,9 +393,30 @@ extern "C" int cfd_batch_solve(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *mes
         r.pressure_integral = v[3];
         r.seconds = secs;
     }
+    // each case's final state, [5][n_c + b_c] with its ghosts after its nodes
+    if (!rc && q_out) {
+        const size_t NT = (size_t)N + B;
+        std::vector<double> u(5 * NT);
+        rc = cfd_field_get(m, CFD_FIELD_Q, u.data());
+        for (int c = 0; c < K && !rc; c++) {
+            if (!q_out[c]) continue;
+            const size_t nc = (size_t)(noff[c + 1] - noff[c]), bc = (size_t)(boff[c + 1] - boff[c]), tc = nc + bc;
+            for (int k = 0; k < 5; k++) {
+                const double *src = u.data() + k * NT;
+                memcpy(q_out[c] + k * tc, src + noff[c], sizeof(double) * nc);
+                memcpy(q_out[c] + k * tc + nc, src + N + boff[c], sizeof(double) * bc);
+            }
+        }
+    }
+    if (device_bytes) *device_bytes = m->bytes;
     if (rc)
         for (int c = 0; c < K; c++)
             if (!results[c].status) results[c].status = rc;
     cfd_mesh_destroy(m);
     return rc;
+}
+
+extern "C" int cfd_batch_solve(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
+                               const cfd_solve_opts *opts, cfd_solve_result *results) {
+    return cfd_batch_solve_into(ctx, K, meshes, params, opts, results, nullptr, nullptr);
 }

@@ -202,6 +202,7 @@ struct cfd_mesh {
     double *case_inf = nullptr;              // device [ncase][8] freestream rho, u, v, w, p, outflow p
     int *bcase = nullptr;                    // device [B] case of each boundary edge
     int *ecase = nullptr;                    // device [N] case of each node
+    int *case_active = nullptr;              // device [ncase] 1 while a case still iterates
     SegMap seg_rms, seg_dot, seg_forces;
     double *bpart = nullptr;                 // device partial sums of segmented reductions
     double *bred = nullptr, *hbred = nullptr;  // per-case results [ncase][kBOut] (device, pinned host)
@@ -266,6 +267,13 @@ int launch_convert_geometry_fp32(cfd_mesh *m);
 int launch_residual_to_fp64(cfd_mesh *m);
 
 int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed);
+// batched solves (batch.hip): cases that may share one union mesh, and the
+// solve itself with each case's final state copied to q_out[c] (may be NULL)
+bool cfd_same_scheme(const cfd_params &a, const cfd_params &b);
+bool cfd_same_solve(const cfd_solve_opts &a, const cfd_solve_opts &b);
+int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
+                         const cfd_solve_opts *opts, cfd_solve_result *results, double *const *q_out,
+                         int64_t *device_bytes);
 int launch_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, int add_time, int rhs_mode,
                        int mixed);
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);

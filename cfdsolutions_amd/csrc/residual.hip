@@ -87,12 +87,16 @@ struct BcArgs {
     // outflow p) and the case of each boundary edge; NULL for one case
     const double *case_inf;
     const int *bcase;
+    // cases still iterating (union of cases); a finished case keeps the
+    // ghost states of its last iteration.  NULL: every edge.
+    const int *case_active;
 };
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_bc(BcArgs<T> a) {
     const int e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= a.B) return;
+    if (a.case_active && !a.case_active[a.bcase[e]]) return;
     const size_t nl = a.bnode[e], nr = (size_t)a.N + e;
     T QL[5], QR[5], QB[5];
 #pragma unroll
@@ -891,7 +895,7 @@ int launch_bc(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, int iter) {
     if (m->B == 0) return CFD_OK;
     BcArgs<T> a{m->N, m->E, m->B, iter, p->zpg_iter, m->bnode, m->btype, b.sa, b.qa, (T)p->gamma, (T)p->inf_rho,
                 (T)p->inf_u, (T)p->inf_v, (T)p->inf_w, (T)p->inf_p, (T)p->outflow_p, m->flag, m->case_inf,
-                m->bcase};
+                m->bcase, m->ncase ? m->case_active : nullptr};
     hipLaunchKernelGGL(k_bc<T>, dim3((m->B + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, a);
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;

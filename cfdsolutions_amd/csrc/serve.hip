@@ -9,9 +9,16 @@
 // resident after a solve and are reused by later requests with the same mesh
 // key; idle meshes are evicted least-recently-used when memory is needed.
 //
+// With batching on, a worker that admits a request also takes the compatible
+// requests in the look-ahead window (same scheme and solver options) and
+// solves them as one union mesh: every kernel launch then covers all of them,
+// and the host waits once per step for the whole batch instead of once per
+// request.  Batched meshes are built for the batch and freed after it.
+//
 // Every kernel on the solve path is deterministic (no atomics in sums,
-// fixed-shape reductions), so a request's result does not depend on what
-// else runs concurrently.
+// fixed-shape reductions), and a batched case uses the reductions it would
+// use alone, so a request's result does not depend on what else runs
+// concurrently or shares its batch.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -31,8 +38,9 @@ using Clock = std::chrono::steady_clock;
 struct Pending {
     cfd_request req;
     double t_submit;
-    int64_t need;   // bytes to reserve if no resident mesh can be reused
-    int bypassed;   // later requests admitted ahead of this one
+    int64_t need;    // bytes to reserve if no resident mesh can be reused
+    int bypassed;    // later requests admitted ahead of this one
+    bool batchable;  // FGMRES solve on a mesh in the batched layout
 };
 
 struct Resident {
@@ -45,6 +53,14 @@ struct Resident {
 };
 
 int krylov_of(const cfd_solve_opts &o) { return o.lin.solver == CFD_SOLVER_FGMRES ? o.lin.krylov_m : 0; }
+
+// Requests admitted together: one request with its mesh slot, or a batch
+// solved as one union mesh holding `reserve` bytes of the budget.
+struct Job {
+    std::vector<Pending> reqs;
+    Resident *slot = nullptr;
+    int64_t reserve = 0;
+};
 }  // namespace
 
 struct cfd_server {
@@ -84,13 +100,80 @@ bool make_room(cfd_server *s, int64_t need) {
     return true;
 }
 
+// Tries to start a batch led by queue entry q: the compatible requests of the
+// look-ahead window, up to batch_max requests and batch_nodes nodes, when
+// there are at least two and their union mesh fits the budget.  Called with
+// the server lock held.
+bool admit_batch(cfd_server *s, size_t q, size_t look, Job &job) {
+    const Pending &p = s->queue[q];
+    const int64_t cap = s->opts.batch_nodes > 0 ? s->opts.batch_nodes : 2000000;
+    std::vector<size_t> take{q};
+    int64_t nn = p.req.mesh->n_node, ne = p.req.mesh->n_edge, nb = p.req.mesh->n_bedge;
+    for (size_t t = q + 1; t < look && (int)take.size() < s->opts.batch_max; t++) {
+        const Pending &o = s->queue[t];
+        if (!o.batchable || !cfd_same_scheme(p.req.params, o.req.params) || !cfd_same_solve(p.req.opts, o.req.opts))
+            continue;
+        if (nn + o.req.mesh->n_node > cap) continue;
+        take.push_back(t);
+        nn += o.req.mesh->n_node;
+        ne += o.req.mesh->n_edge;
+        nb += o.req.mesh->n_bedge;
+    }
+    if (take.size() < 2 || nn > INT32_MAX / 4 || ne > INT32_MAX / 4 || nb > INT32_MAX / 4) return false;
+    const int64_t need = cfd_footprint_bytes((int32_t)nn, (int32_t)ne, (int32_t)nb, krylov_of(p.req.opts),
+                                             p.req.opts.lin.mixed);
+    if (need > s->opts.hbm_budget || !make_room(s, need)) return false;
+    s->in_use += need;
+    s->peak = std::max(s->peak, s->in_use);
+    // every entry passed over ahead of a taken one counts as bypassed once
+    for (size_t k = 0, t = 0; k < take.back(); k++) {
+        if (t < take.size() && take[t] == k) {
+            t++;
+            continue;
+        }
+        s->queue[k].bypassed++;
+    }
+    job.reqs.clear();
+    for (size_t t : take) job.reqs.push_back(s->queue[t]);
+    for (size_t t = take.size(); t-- > 0;) s->queue.erase(s->queue.begin() + (long)take[t]);
+    job.slot = nullptr;
+    job.reserve = need;
+    s->admitted += (int64_t)take.size();
+    return true;
+}
+
 // Picks the first request of the look-ahead window that can start now and
-// reserves its memory.  A request that has been bypassed max_bypass times
-// blocks the window until it is admitted.  Called with the server lock held.
-bool admit(cfd_server *s, Pending &job, Resident *&slot) {
+// reserves its memory (with the requests batched alongside it when batching
+// is on).  A request that has been bypassed max_bypass times blocks the
+// window until it is admitted.  Called with the server lock held.
+// Compatible requests queued in the look-ahead window with the one at q
+// (itself included).
+int companions(const cfd_server *s, size_t q, size_t look) {
+    const Pending &p = s->queue[q];
+    int n = 0;
+    for (size_t t = q; t < look; t++) {
+        const Pending &o = s->queue[t];
+        if (o.batchable && cfd_same_scheme(p.req.params, o.req.params) && cfd_same_solve(p.req.opts, o.req.opts)) n++;
+    }
+    return n;
+}
+
+bool admit(cfd_server *s, Job &job, double &retry_at) {
     const size_t look = std::min<size_t>(s->queue.size(), (size_t)std::max(1, s->opts.lookahead));
+    retry_at = 0.0;
     for (size_t q = 0; q < look; q++) {
         Pending &p = s->queue[q];
+        if (s->opts.batch_max >= 2 && p.batchable) {
+            // batching delay: while the GPU is busy with other solves, a short
+            // batch waits for companions (up to batch_wait_us after arrival)
+            const double until = p.t_submit + 1e-6 * s->opts.batch_wait_us;
+            if (s->running > 0 && s->opts.batch_wait_us > 0 && p.bypassed < s->opts.max_bypass && s->now() < until &&
+                companions(s, q, look) < s->opts.batch_max) {
+                retry_at = retry_at > 0.0 ? std::min(retry_at, until) : until;
+                continue;
+            }
+            if (admit_batch(s, q, look, job)) return true;
+        }
         const int km = krylov_of(p.req.opts);
         Resident *r = nullptr;
         for (auto &cm : s->meshes)
@@ -110,8 +193,9 @@ bool admit(cfd_server *s, Pending &job, Resident *&slot) {
             r->busy = true;
             r->last_use = ++s->tick;
             for (size_t k = 0; k < q; k++) s->queue[k].bypassed++;
-            job = p;
-            slot = r;
+            job.reqs.assign(1, p);
+            job.slot = r;
+            job.reserve = 0;
             s->queue.erase(s->queue.begin() + (long)q);
             s->admitted++;
             return true;
@@ -121,12 +205,50 @@ bool admit(cfd_server *s, Pending &job, Resident *&slot) {
     return false;
 }
 
+// Solves a batch as one union mesh; one completion per request.
+std::vector<cfd_completion> run_batch(cfd_server *s, const Job &job) {
+    const int K = (int)job.reqs.size();
+    std::vector<cfd_mesh_desc> descs(K);
+    std::vector<cfd_params> params(K);
+    std::vector<cfd_solve_opts> opts(K);
+    std::vector<cfd_solve_result> res(K);
+    std::vector<double *> qo(K);
+    for (int c = 0; c < K; c++) {
+        descs[c] = *job.reqs[c].req.mesh;
+        params[c] = job.reqs[c].req.params;
+        opts[c] = job.reqs[c].req.opts;
+        qo[c] = job.reqs[c].req.q_out;
+    }
+    const double t_admit = s->now();
+    int64_t bytes = 0;
+    const int rc = cfd_batch_solve_into(s->ctx, K, descs.data(), params.data(), opts.data(), res.data(), qo.data(),
+                                        &bytes);
+    const double t_done = s->now();
+    std::vector<cfd_completion> out(K);
+    for (int c = 0; c < K; c++) {
+        cfd_completion &o = out[c];
+        o = cfd_completion{};
+        o.id = job.reqs[c].req.id;
+        o.status = rc ? rc : res[c].status;
+        o.batch_size = K;
+        o.result = res[c];
+        o.t_submit = job.reqs[c].t_submit;
+        o.t_admit = t_admit;
+        o.t_done = t_done;
+        o.footprint = job.reserve;
+        o.device_bytes = bytes;
+        if (o.status) snprintf(o.message, sizeof(o.message), "%s", rc ? cfd_last_error(s->ctx) : "solve failed");
+    }
+    return out;
+}
+
 cfd_completion run_one(cfd_server *s, const Pending &p, Resident *slot) {
     cfd_completion c{};
     c.id = p.req.id;
     c.t_submit = p.t_submit;
     c.t_admit = s->now();
     c.footprint = slot->bytes;
+    c.batch_size = 1;
     int rc;
     if (!slot->mesh) {
         rc = cfd_mesh_create(s->ctx, p.req.mesh, &p.req.params, &slot->mesh);
@@ -150,14 +272,30 @@ void worker(cfd_server *s) {
     hipSetDevice(s->ctx->device);
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        Pending job{};
-        Resident *slot = nullptr;
+        Job job;
         bool got = false;
-        while (!s->stop && !(got = admit(s, job, slot))) s->cv_work.wait(lk);
+        double retry_at = 0.0;
+        while (!s->stop && !(got = admit(s, job, retry_at))) {
+            if (retry_at > 0.0)  // a request is waiting for batch companions
+                s->cv_work.wait_for(lk, std::chrono::duration<double>(std::max(1e-4, retry_at - s->now())));
+            else
+                s->cv_work.wait(lk);
+        }
         if (!got) break;
         s->running++;
         lk.unlock();
-        cfd_completion c = run_one(s, job, slot);
+        if (job.reqs.size() > 1) {
+            std::vector<cfd_completion> cs = run_batch(s, job);
+            lk.lock();
+            s->running--;
+            s->in_use -= job.reserve;
+            for (auto &c : cs) s->done.push_back(c);
+            s->cv_done.notify_all();
+            s->cv_work.notify_all();
+            continue;
+        }
+        Resident *slot = job.slot;
+        cfd_completion c = run_one(s, job.reqs[0], slot);
         const int64_t actual = slot->mesh ? cfd_mesh_device_bytes(slot->mesh) : 0;
         lk.lock();
         s->running--;
@@ -189,7 +327,7 @@ void worker(cfd_server *s) {
 }  // namespace
 
 extern "C" int cfd_server_create(cfd_ctx *ctx, const cfd_server_opts *o, cfd_server **out) {
-    if (!ctx || !o || !out || o->max_inflight < 1 || o->max_inflight > 64 || o->hbm_budget <= 0)
+    if (!ctx || !o || !out || o->max_inflight < 1 || o->max_inflight > 64 || o->hbm_budget <= 0 || o->batch_max > 4096)
         return cfd_set_error(ctx, CFD_E_ARG, "cfd_server_create: invalid options");
     cfd_server *s = new (std::nothrow) cfd_server();
     if (!s) return cfd_set_error(ctx, CFD_E_OOM, "cfd_server_create: out of host memory");
@@ -210,8 +348,12 @@ extern "C" int cfd_server_submit(cfd_server *s, const cfd_request *r) {
     if (need > s->opts.hbm_budget)
         return cfd_set_error(s->ctx, CFD_E_OOM, "request %lld needs %lld bytes, budget is %lld", (long long)r->id,
                              (long long)need, (long long)s->opts.hbm_budget);
+    // batchable: FGMRES, and ghosts numbered after the nodes in boundary-edge
+    // order (the layout the union mesh concatenates)
+    bool batchable = r->opts.lin.solver == CFD_SOLVER_FGMRES;
+    for (int32_t b = 0; batchable && b < d->n_bedge; b++) batchable = d->bedge[2 * b + 1] == d->n_node + b;
     std::lock_guard<std::mutex> g(s->mu);
-    s->queue.push_back(Pending{*r, s->now(), need, 0});
+    s->queue.push_back(Pending{*r, s->now(), need, 0, batchable});
     s->cv_work.notify_one();
     return CFD_OK;
 }

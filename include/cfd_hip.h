@@ -274,7 +274,13 @@ int cfd_batch_solve(cfd_ctx *ctx, int32_t ncase, const cfd_mesh_desc *meshes, co
  * one GPU: requests are admitted in arrival order (with bounded look-ahead)
  * when a worker is free and their HBM footprint fits the budget; device
  * meshes stay resident after a solve and serve later requests with the same
- * mesh_key.  Results do not depend on concurrency (deterministic kernels). */
+ * mesh_key.  With batch_max >= 2 the server batches continuously: a worker
+ * that admits a request also takes the compatible requests queued in its
+ * look-ahead window (same scheme and solver options, FGMRES) and solves them
+ * together as one union mesh (as cfd_batch_solve); each batched result is
+ * bit-identical to the same request solved alone, and requests that arrive
+ * meanwhile are admitted by the other workers.  Results do not depend on
+ * concurrency (deterministic kernels). */
 typedef struct cfd_server cfd_server;
 
 typedef struct cfd_server_opts {
@@ -283,6 +289,11 @@ typedef struct cfd_server_opts {
     int32_t max_bypass;     /* admissions ahead of a request before it blocks the window (<= 0: 8) */
     int32_t keep_resident;  /* 1: keep device meshes for reuse by requests with the same mesh_key */
     int64_t hbm_budget;     /* device bytes the server may hold (meshes, workspaces) */
+    int32_t batch_max;      /* >= 2: up to this many requests per union-mesh batch; else one per solve */
+    int32_t batch_nodes;    /* nodes per batch at most (<= 0: 2,000,000) */
+    int32_t batch_wait_us;  /* while other solves run, a batchable request with fewer than batch_max
+                               companions queued waits up to this long for more (0: never waits) */
+    int32_t pad_;
 } cfd_server_opts;
 
 typedef struct cfd_request {
@@ -292,16 +303,18 @@ typedef struct cfd_request {
     cfd_params params;
     cfd_solve_opts opts;
     double *q_out;              /* host [5][N+B] final state, or NULL */
-    double *rms_hist;           /* host [opts.max_iter] residual history, or NULL */
+    double *rms_hist;           /* host [opts.max_iter] residual history, or NULL (not filled when batched) */
 } cfd_request;
 
 typedef struct cfd_completion {
     int64_t id;
     int32_t status;             /* CFD_OK or CFD_E_* */
     int32_t reused;             /* 1 if a resident mesh was reused */
+    int32_t batch_size;         /* requests solved together with this one (1: alone) */
+    int32_t pad_;
     cfd_solve_result result;
     double t_submit, t_admit, t_done;  /* seconds on the server clock */
-    int64_t footprint;          /* bytes reserved for the request's mesh */
+    int64_t footprint;          /* bytes reserved for the request's mesh (its batch's union mesh) */
     int64_t device_bytes;       /* bytes the mesh actually held */
     char message[160];          /* error text when status != CFD_OK */
 } cfd_completion;

@@ -159,7 +159,9 @@ REQS = [  # (mesh key, dims, flux, order, mach, alpha)
 SOLVE = dict(max_iter=200, tol=1e-10, krylov_m=20, max_restarts=4, rtol=1e-3, precond=cfd.PREC_ILU0, mixed=True)
 
 
-def test_server_results_match_alone(ctx):
+@pytest.fixture(scope="module")
+def alone_runs(ctx):
+    """Each request of REQS solved by itself: (iterations, Krylov total, forces, final state)."""
     meshes = {k: cmesh.bump_channel(*d) for k, d, *_ in REQS}
     alone = []
     for k, d, flux, order, mach, alpha in REQS:
@@ -168,6 +170,11 @@ def test_server_results_match_alone(ctx):
         r, _ = dm.solve_steady(p, alpha_deg=alpha, **SOLVE)
         alone.append((r.iterations, r.krylov_total, list(r.force), dm.get(cfd.FIELD_Q)))
         dm.close()
+    return meshes, alone
+
+
+def test_server_results_match_alone(ctx, alone_runs):
+    meshes, alone = alone_runs
     srv = cfd.Server(ctx, hbm_budget=8 << 30, max_inflight=3)
     for rid, (k, d, flux, order, mach, alpha) in enumerate(REQS):
         p = cfd.make_params(mach=mach, alpha_deg=alpha, flux=flux, order=order)
@@ -196,6 +203,33 @@ def test_server_results_match_alone(ctx):
         assert list(c["result"].force) == force
         assert np.array_equal(c["q"], q), "request %d differs from its alone run" % c["id"]
         assert c["t_submit"] <= c["t_admit"] <= c["t_done"]
+
+
+def test_server_batches_match_alone(ctx, alone_runs):
+    """Continuous batching: with one worker busy on the first request, the queued compatible requests are solved
+    together as union meshes, and every result equals the same request solved alone, bit for bit."""
+    meshes, alone = alone_runs
+    srv = cfd.Server(ctx, hbm_budget=8 << 30, max_inflight=1, batch_max=4)
+    for rid, (k, d, flux, order, mach, alpha) in enumerate(REQS):
+        p = cfd.make_params(mach=mach, alpha_deg=alpha, flux=flux, order=order)
+        srv.submit(rid, k, meshes[k], p, want_state=True, alpha_deg=alpha, **SOLVE)
+    done = []
+    while len(done) < len(REQS):
+        got = srv.poll(wait=60.0)
+        assert got, "server made no progress in 60 s"
+        done += got
+    st = srv.stats()
+    srv.close()
+    assert max(c["batch_size"] for c in done) >= 2
+    assert st["admitted"] == len(REQS) and st["peak_bytes"] <= 8 << 30
+    for c in done:
+        it, kry, force, q = alone[c["id"]]
+        assert c["status"] == 0, c["message"]
+        assert c["result"].converged
+        assert c["result"].iterations == it and c["result"].krylov_total == kry
+        assert list(c["result"].force) == force
+        assert np.array_equal(c["q"], q), "request %d differs from its alone run" % c["id"]
+        assert 0 < c["device_bytes"] <= c["footprint"], (c["device_bytes"], c["footprint"])
 
 
 def test_server_admission_respects_budget(ctx):
