@@ -656,3 +656,122 @@ extern "C" int cfdm_get_arrays(const cfdm_mesh *m, double *xyz, int32_t *tet, in
     cp(n2n_ja, m->n2nJA);
     return CFDM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Restart and solution files (conservative variables)
+// ---------------------------------------------------------------------------
+// Restart_Writer, Euler3D/src/RestartIO.cpp:45-109: iteration, node count,
+// ghost count and variable count (int32), then the five variables of every
+// node and ghost, interleaved per node (native byte order, as fwrite).
+extern "C" int cfdm_restart_write(const char *path, int32_t iteration, int32_t n_node, int32_t n_ghost,
+                                  const double *q) {
+    if (!path || !q || n_node < 0 || n_ghost < 0) return fail(CFDM_E_ARG, "cfdm_restart_write: invalid arguments");
+    const size_t nt = (size_t)n_node + (size_t)n_ghost;
+    std::vector<double> buf(5 * nt);
+    for (size_t i = 0; i < nt; i++)
+        for (int k = 0; k < 5; k++) buf[5 * i + k] = q[k * nt + i];
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail(CFDM_E_IO, "cfdm_restart_write: cannot write %s", path);
+    const int32_t head[4] = {iteration, n_node, n_ghost, 5};
+    const bool ok = fwrite(head, sizeof(int32_t), 4, fp) == 4 && fwrite(buf.data(), sizeof(double), buf.size(), fp) ==
+                                                                      buf.size();
+    if (fclose(fp) != 0 || !ok) return fail(CFDM_E_IO, "cfdm_restart_write: write to %s failed", path);
+    return CFDM_OK;
+}
+
+// Restart_Reader, Euler3D/src/RestartIO.cpp:112-179: the counts must match
+// the mesh (the reference stops otherwise); q receives [5][n_node + n_ghost].
+extern "C" int cfdm_restart_read(const char *path, int32_t n_node, int32_t n_ghost, int32_t *iteration, double *q) {
+    if (!path || !q || n_node < 0 || n_ghost < 0) return fail(CFDM_E_ARG, "cfdm_restart_read: invalid arguments");
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail(CFDM_E_IO, "cfdm_restart_read: cannot open %s", path);
+    int32_t head[4] = {0, 0, 0, 0};
+    const size_t nt = (size_t)n_node + (size_t)n_ghost;
+    std::vector<double> buf(5 * nt);
+    int rc = CFDM_OK;
+    if (fread(head, sizeof(int32_t), 4, fp) != 4)
+        rc = fail(CFDM_E_IO, "cfdm_restart_read: %s is truncated", path);
+    else if (head[1] != n_node)
+        rc = fail(CFDM_E_ARG, "cfdm_restart_read: %s holds %d nodes, the mesh %d", path, head[1], n_node);
+    else if (head[2] != n_ghost)
+        rc = fail(CFDM_E_ARG, "cfdm_restart_read: %s holds %d ghosts, the mesh %d", path, head[2], n_ghost);
+    else if (head[3] != 5)
+        rc = fail(CFDM_E_ARG, "cfdm_restart_read: %s holds %d variables, expected 5", path, head[3]);
+    else if (fread(buf.data(), sizeof(double), buf.size(), fp) != buf.size())
+        rc = fail(CFDM_E_IO, "cfdm_restart_read: %s is truncated", path);
+    fclose(fp);
+    if (rc) return rc;
+    for (size_t i = 0; i < nt; i++)
+        for (int k = 0; k < 5; k++) q[k * nt + i] = buf[5 * i + k];
+    if (iteration) *iteration = head[0];
+    return CFDM_OK;
+}
+
+// VTK_Writer, Euler3D/src/MeshIO.cpp:219-399 (conservative variables, GENERIC
+// non-dimensionalisation): legacy ASCII unstructured grid with the boundary
+// triangles (their surface tag as BoundaryIds) and the tetrahedra as cells,
+// and density, velocity components, pressure (Get_Pressure), temperature
+// (Get_Temperature), Mach (Get_Mach, Euler3D/src/Material.cpp:701-1100) and
+// the velocity vector as point data.  q: [5][ldq], nodes first.
+extern "C" int cfdm_vtk_write(const cfdm_mesh *m, const char *path, const double *q, int32_t ldq, double gamma,
+                              double r_gas) {
+    if (!m || !path || !q || ldq < m->nNode) return fail(CFDM_E_ARG, "cfdm_vtk_write: invalid arguments");
+    FILE *fp = fopen(path, "w");
+    if (!fp) return fail(CFDM_E_IO, "cfdm_vtk_write: cannot write %s", path);
+    const int N = m->nNode;
+    const std::vector<int> &tri = m->ready ? m->ftri : m->tri;  // triangles as the solver holds them
+    fprintf(fp, "# vtk DataFile Version 2.0\n");
+    fprintf(fp, "Solver solution data\n");
+    fprintf(fp, "ASCII \n");
+    fprintf(fp, "DATASET UNSTRUCTURED_GRID\n");
+    fprintf(fp, "POINTS %d double\n", N);
+    for (int i = 0; i < N; i++)
+        fprintf(fp, "%f %f %f\n", m->xyz[3 * (size_t)i], m->xyz[3 * (size_t)i + 1], m->xyz[3 * (size_t)i + 2]);
+    const int ncell = m->nTri + m->nTet;
+    fprintf(fp, "CELLS %d %d\n", ncell, 4 * m->nTri + 5 * m->nTet);
+    for (int c = 0; c < m->nTri; c++)
+        fprintf(fp, "3 %d %d %d\n", tri[3 * (size_t)c], tri[3 * (size_t)c + 1], tri[3 * (size_t)c + 2]);
+    for (int c = 0; c < m->nTet; c++)
+        fprintf(fp, "4 %d %d %d %d\n", m->tet[4 * (size_t)c], m->tet[4 * (size_t)c + 1], m->tet[4 * (size_t)c + 2],
+                m->tet[4 * (size_t)c + 3]);
+    fprintf(fp, "CELL_TYPES %d\n", ncell);
+    for (int c = 0; c < m->nTri; c++) fprintf(fp, "5\n");
+    for (int c = 0; c < m->nTet; c++) fprintf(fp, "10\n");
+    fprintf(fp, "CELL_DATA %d\n", ncell);
+    fprintf(fp, "SCALARS BoundaryIds Int 1\n");
+    fprintf(fp, "LOOKUP_TABLE default\n");
+    for (int c = 0; c < m->nTri; c++) fprintf(fp, "%d\n", m->triTag[c]);
+    for (int c = 0; c < m->nTet; c++) fprintf(fp, "0\n");
+    fprintf(fp, "POINT_DATA %d\n", N);
+    const double *Q1 = q, *Q2 = q + ldq, *Q3 = q + 2 * (size_t)ldq, *Q4 = q + 3 * (size_t)ldq,
+                 *Q5 = q + 4 * (size_t)ldq;
+    const char *names[4] = {"Density", "X_Velocity", "Y_Velocity", "Z_Velocity"};
+    for (int f = 0; f < 4; f++) {
+        fprintf(fp, "SCALARS %s double 1\n", names[f]);
+        fprintf(fp, "LOOKUP_TABLE default\n");
+        for (int i = 0; i < N; i++) {
+            const double v = f == 0 ? Q1[i] : (f == 1 ? Q2[i] : (f == 2 ? Q3[i] : Q4[i])) / Q1[i];
+            fprintf(fp, "%22.15e\n", v);
+        }
+    }
+    for (int f = 0; f < 3; f++) {
+        fprintf(fp, "SCALARS %s double 1\n", f == 0 ? "Pressure" : (f == 1 ? "Temperature" : "Mach"));
+        fprintf(fp, "LOOKUP_TABLE default\n");
+        for (int i = 0; i < N; i++) {
+            double v;
+            if (f < 2) {
+                const double p = (gamma - 1.0) * (Q5[i] - 0.5 * (Q2[i] * Q2[i] + Q3[i] * Q3[i] + Q4[i] * Q4[i]) / Q1[i]);
+                v = f == 0 ? p : p / (r_gas * Q1[i]);
+            } else {
+                const double q2 = (Q2[i] * Q2[i] + Q3[i] * Q3[i] + Q4[i] * Q4[i]) / (Q1[i] * Q1[i]);
+                const double p = (gamma - 1.0) * (Q5[i] - 0.5 * Q1[i] * q2);
+                v = sqrt(q2 * Q1[i] / (gamma * p));
+            }
+            fprintf(fp, "%22.15e\n", v);
+        }
+    }
+    fprintf(fp, "VECTORS Velocity double \n");
+    for (int i = 0; i < N; i++) fprintf(fp, "%22.15e \t %22.15e \t %22.15e\n", Q2[i] / Q1[i], Q3[i] / Q1[i], Q4[i] / Q1[i]);
+    if (fclose(fp) != 0) return fail(CFDM_E_IO, "cfdm_vtk_write: write to %s failed", path);
+    return CFDM_OK;
+}

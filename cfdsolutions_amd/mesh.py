@@ -116,3 +116,35 @@ def bump_channel(nx, ny, nz, bump_h=0.1, **kw):
 def jittered_cube(n, jitter=0.2, seed=20261202, **kw):
     """Unit cube n^3 hexes, interior nodes jittered by U(-jitter, jitter)*h (residual microbenchmark)."""
     return box(n, n, n, 1.0, 1.0, 1.0, bump_h=0.0, jitter=jitter, seed=seed, **kw)
+
+
+def write_restart(path, q, n_node, n_ghost, iteration=0):
+    """Restart file of a conservative state q [5][n_node + n_ghost] (Restart_Writer layout)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    if q.size != 5 * (n_node + n_ghost):
+        raise ValueError("state has %d values, expected %d" % (q.size, 5 * (n_node + n_ghost)))
+    _check(_lib().cfdm_restart_write(str(path).encode(), int(iteration), int(n_node), int(n_ghost), _p(q)))
+
+
+def read_restart(path, n_node, n_ghost):
+    """Reads a restart file; returns (iteration, q [5][n_node + n_ghost])."""
+    q = np.zeros((5, n_node + n_ghost))
+    it = ctypes.c_int32(0)
+    _check(_lib().cfdm_restart_read(str(path).encode(), int(n_node), int(n_ghost), ctypes.byref(it), _p(q)))
+    return int(it.value), q
+
+
+def write_vtk(ugrid_path, vtk_path, q, gamma=1.4, r_gas=None):
+    """VTK solution file (VTK_Writer layout) of the mesh in a UGRID file and a conservative state q [5][>= N]."""
+    lib = _lib()
+    h = ctypes.c_void_p()
+    _check(lib.cfdm_read_ugrid(str(ugrid_path).encode(), ctypes.byref(h)))
+    try:
+        _check(lib.cfdm_preprocess(h))
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        r = 1.0 / gamma if r_gas is None else r_gas
+        _check(lib.cfdm_vtk_write(h, str(vtk_path).encode(), _p(q), int(q.shape[1]), ctypes.c_double(gamma),
+                                  ctypes.c_double(r)))
+    finally:
+        lib.cfdm_free(h)
+

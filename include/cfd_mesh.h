@@ -97,6 +97,20 @@ int cfdm_get_arrays(const cfdm_mesh *m, double *xyz, int32_t *tet, int32_t *tri,
                     int32_t *bedge, double *bedge_area, int32_t *bedge_tag,
                     double *vol, int32_t *n2n_ia, int32_t *n2n_ja);
 
+/* Restart_Writer (Euler3D/src/RestartIO.cpp:45-109), conservative variables:
+ * int32 iteration, node count, ghost count, variable count (5), then the 5
+ * variables of each of the n_node + n_ghost nodes interleaved per node.
+ * q is [5][n_node + n_ghost] (as cfd_field_get(CFD_FIELD_Q)). */
+int cfdm_restart_write(const char *path, int32_t iteration, int32_t n_node, int32_t n_ghost, const double *q);
+/* Restart_Reader (Euler3D/src/RestartIO.cpp:112-179): fails with CFDM_E_ARG if
+ * the file's node or ghost count differs from the mesh's. */
+int cfdm_restart_read(const char *path, int32_t n_node, int32_t n_ghost, int32_t *iteration, double *q);
+/* VTK_Writer (Euler3D/src/MeshIO.cpp:219-399), conservative variables: legacy
+ * ASCII unstructured grid of the boundary triangles and tetrahedra with
+ * BoundaryIds, density, velocity, pressure, temperature and Mach.  q is
+ * [5][ldq] with the nodes first; gamma and r_gas of the non-dimensionalisation. */
+int cfdm_vtk_write(const cfdm_mesh *m, const char *path, const double *q, int32_t ldq, double gamma, double r_gas);
+
 const char *cfdm_last_error(void);
 
 #ifdef __cplusplus

@@ -22,6 +22,7 @@
 #include "MC.h"
 #include "Material.h"
 #include "MeshIO.h"
+#include "RestartIO.h"
 #include "Roe_Fluxes.h"
 #include "Solver.h"
 #include "SolverParameters.h"
@@ -220,6 +221,10 @@ int main(int argc, char **argv) {
 
     Apply_Boundary_Condition(iter);
     dump_d("q_bc", gather5(Q1, Q2, Q3, Q4, Q5, N + NB), N + NB);
+    // restart and solution files of this state (Euler3D/src/RestartIO.cpp:45-109, MeshIO.cpp:219-399)
+    RestartIteration = iter;
+    Restart_Writer((g_out + "/restart.srt").c_str(), 0);
+    VTK_Writer((g_out + "/solution.vtk").c_str(), 0);
     if (SolverOrder == SOLVER_ORDER_SECOND) {
         Compute_Least_Square_Gradient(0);
         std::vector<double> g(15 * (size_t)N);

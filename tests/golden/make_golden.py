@@ -85,6 +85,10 @@ def main():
                     str(a["lorder"])]
             subprocess.run(args, check=True, stdout=subprocess.DEVNULL, cwd=td)
             data = {f[:-4]: np.load(os.path.join(td, f)) for f in os.listdir(td) if f.endswith(".npy")}
+            # the reference's own restart and VTK files of the boundary-condition state, as bytes
+            for f, key in (("restart.srt", "restart_file"), ("solution.vtk", "vtk_file")):
+                with open(os.path.join(td, f), "rb") as fh:
+                    data[key] = np.frombuffer(fh.read(), dtype=np.uint8)
         prune_fd(name, data)
         data["meta"] = np.array(json.dumps({"mesh": mspec, "args": a}))
         out = os.path.join(os.path.dirname(os.path.abspath(__file__)), name + ".npz")
