@@ -7,6 +7,7 @@
 // are bit-identical to the reference without atomics or colouring; each edge
 // flux is evaluated by both of its nodes.
 #include <cfloat>
+#include <cstdlib>
 #include <cmath>
 
 #include "cfd_internal.h"
@@ -117,6 +118,31 @@ __global__ __launch_bounds__(kBlock) void k_bc(BcArgs<T> a) {
 #pragma unroll
     for (int k = 0; k < 5; k++) o[k] = QB[k];
 }
+
+// ---------------------------------------------------------------------------
+// Direct global->LDS loads (gfx950 global_load_lds): each lane's W bytes go
+// to row r of a wave's staging slot at column lane.  Neighbour walks use them
+// to bring incidence k+1 into LDS while incidence k is processed; no VGPR
+// holds an in-flight load, so nothing waits on the prefetch early.  The
+// compiler does not order these loads against LDS reads: a walk waits
+// vmcnt(0) before reading its slot and lgkmcnt(0) before refilling it.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) void glb_void;
+
+// cache-policy bits of the staging loads: streaming data (indices, areas)
+// is not kept in the caches, node records are
+constexpr int kPolNT = 2;
+
+template <int POL>
+__device__ __forceinline__ void dma16(const void *g, void *l) {
+    __builtin_amdgcn_global_load_lds((glb_void *)g, (lds_void *)l, 16, 0, POL);
+}
+template <int POL>
+__device__ __forceinline__ void dma4(const void *g, void *l) {
+    __builtin_amdgcn_global_load_lds((glb_void *)g, (lds_void *)l, 4, 0, POL);
+}
+
 
 // ---------------------------------------------------------------------------
 // Gradients
@@ -241,6 +267,75 @@ __device__ __forceinline__ bool near_vacuum(const T *Q, T P) {
 // a final face state of the node is near vacuum (the only case in which the
 // Roe pressure correction can act).  Neither pass depends on the order of the
 // neighbours.
+// One neighbour's bound on the limiter of node c (Compute_Limiter_Venkatakrishnan,
+// Limiters.cpp:185-267; Barth-Jespersen, Limiters.cpp:116-180): qj and
+// (xj, yj, zj) are the neighbour's state and coordinates.
+template <class T, int KIND>  // 1 Venkatakrishnan, 2 Barth-Jespersen
+__device__ __forceinline__ void limiter_nbr(const T *qj, T xj, T yj, T zj, const T *qc, T xc, T yc, T zc, const T *gr,
+                                            const T *qmin, const T *qmax, T eps2, T frac, T *ph) {
+    const T Rx = T(0.5) * (xj - xc), Ry = T(0.5) * (yj - yc), Rz = T(0.5) * (zj - zc);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const T qf = qc[k] + frac * (qj[k] - qc[k]) + (T(1.0) - frac) * (gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz);
+        const T dM = qf - qc[k];
+        if (KIND == 1) {
+            if (dM > T(0.0)) {
+                const T dP = qmax[k] - qc[k];
+                T t = (dP * dP + eps2) + T(2.0) * (dM * dP);
+                t = t / (dP * dP + T(2.0) * dM * dM + dM * dP + eps2);
+                ph[k] = tmin(ph[k], t);
+            } else if (dM < T(0.0)) {
+                const T dP = qmin[k] - qc[k];
+                T t = (dP * dP + eps2) + T(2.0) * (dM * dP);
+                t = t / (dP * dP + T(2.0) * dM * dM + dM * dP + eps2);
+                ph[k] = tmin(ph[k], t);
+            } else {
+                ph[k] = tmin(ph[k], T(1.0));
+            }
+        } else {
+            if (dM > T(0.0))
+                ph[k] = tmin(ph[k], tmin(T(1.0), (qmax[k] - qc[k]) / dM));
+            else if (dM < T(0.0))
+                ph[k] = tmin(ph[k], tmin(T(1.0), (qmin[k] - qc[k]) / dM));
+            else
+                ph[k] = tmin(ph[k], T(1.0));
+        }
+        ph[k] = tmax(T(0.0), ph[k]);
+    }
+}
+
+// Pressure of the face state towards one neighbour with the limiter ph
+// (Compute_Limiter_PressureCorrection, Limiters.cpp:273-304): a negative
+// pressure zeroes the limiter; a near-vacuum one marks the node as risky.
+template <class T>
+__device__ __forceinline__ void pressure_nbr(const T *qj, T xj, T yj, T zj, const T *qc, T xc, T yc, T zc,
+                                             const T *gr, T frac, T g, T *ph, bool &zeroed, bool &risky) {
+    const T Rx = T(0.5) * (xj - xc), Ry = T(0.5) * (yj - yc), Rz = T(0.5) * (zj - zc);
+    T Q[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        Q[k] = qc[k] + ph[k] * (frac * (qj[k] - qc[k]) + (T(1.0) - frac) * (gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz));
+    const T P = (g - T(1.0)) * (Q[4] - (T(0.5) / Q[0]) * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]));
+    if (P < T(0.0)) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) ph[k] = T(0.0);
+        zeroed = true;
+    } else if (near_vacuum(Q, P)) {
+        risky = true;
+    }
+}
+
+// After every neighbour: if a face zeroed the limiter, every face state is
+// the node state, whose own pressure decides the risk flag.
+template <class T>
+__device__ __forceinline__ bool pressure_final(const T *qc, T g, bool zeroed, bool risky) {
+    if (zeroed) {
+        const T P = (g - T(1.0)) * (qc[4] - (T(0.5) / qc[0]) * (qc[1] * qc[1] + qc[2] * qc[2] + qc[3] * qc[3]));
+        risky = near_vacuum(qc, P);
+    }
+    return risky;
+}
+
 template <class T, int KIND>  // 1 Venkatakrishnan, 2 Barth-Jespersen
 __device__ __forceinline__ bool limiter_passes(int N, int deg, size_t base, const int *__restrict__ nbr,
                                                const T *__restrict__ qa, const T *qc, T xc, T yc, T zc, const T *gr,
@@ -253,61 +348,16 @@ __device__ __forceinline__ bool limiter_passes(int N, int deg, size_t base, cons
         const int jr = nbr[base + (size_t)n * kSlice];
         if (jr >= N) continue;
         const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
-        const T Rx = T(0.5) * (rj[5] - xc), Ry = T(0.5) * (rj[6] - yc), Rz = T(0.5) * (rj[7] - zc);
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            const T qf = qc[k] + frac * (rj[k] - qc[k]) + (T(1.0) - frac) * (gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz);
-            const T dM = qf - qc[k];
-            if (KIND == 1) {
-                if (dM > T(0.0)) {
-                    const T dP = qmax[k] - qc[k];
-                    T t = (dP * dP + eps2) + T(2.0) * (dM * dP);
-                    t = t / (dP * dP + T(2.0) * dM * dM + dM * dP + eps2);
-                    ph[k] = tmin(ph[k], t);
-                } else if (dM < T(0.0)) {
-                    const T dP = qmin[k] - qc[k];
-                    T t = (dP * dP + eps2) + T(2.0) * (dM * dP);
-                    t = t / (dP * dP + T(2.0) * dM * dM + dM * dP + eps2);
-                    ph[k] = tmin(ph[k], t);
-                } else {
-                    ph[k] = tmin(ph[k], T(1.0));
-                }
-            } else {
-                if (dM > T(0.0))
-                    ph[k] = tmin(ph[k], tmin(T(1.0), (qmax[k] - qc[k]) / dM));
-                else if (dM < T(0.0))
-                    ph[k] = tmin(ph[k], tmin(T(1.0), (qmin[k] - qc[k]) / dM));
-                else
-                    ph[k] = tmin(ph[k], T(1.0));
-            }
-            ph[k] = tmax(T(0.0), ph[k]);
-        }
+        limiter_nbr<T, KIND>(rj, rj[5], rj[6], rj[7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph);
     }
-    // Compute_Limiter_PressureCorrection, Limiters.cpp:273-304
     bool zeroed = false, risky = false;
     for (int n = 0; n < deg; n++) {
         const int jr = nbr[base + (size_t)n * kSlice];
         if (jr >= N) continue;
         const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
-        const T Rx = T(0.5) * (rj[5] - xc), Ry = T(0.5) * (rj[6] - yc), Rz = T(0.5) * (rj[7] - zc);
-        T Q[5];
-#pragma unroll
-        for (int k = 0; k < 5; k++)
-            Q[k] = qc[k] + ph[k] * (frac * (rj[k] - qc[k]) + (T(1.0) - frac) * (gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz));
-        const T P = (g - T(1.0)) * (Q[4] - (T(0.5) / Q[0]) * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]));
-        if (P < T(0.0)) {
-#pragma unroll
-            for (int k = 0; k < 5; k++) ph[k] = T(0.0);
-            zeroed = true;
-        } else if (near_vacuum(Q, P)) {
-            risky = true;
-        }
+        pressure_nbr<T>(rj, rj[5], rj[6], rj[7], qc, xc, yc, zc, gr, frac, g, ph, zeroed, risky);
     }
-    if (zeroed) {  // every face state is now the node state
-        const T P = (g - T(1.0)) * (qc[4] - (T(0.5) / qc[0]) * (qc[1] * qc[1] + qc[2] * qc[2] + qc[3] * qc[3]));
-        risky = near_vacuum(qc, P);
-    }
-    return risky;
+    return pressure_final<T>(qc, g, zeroed, risky);
 }
 
 template <class T, int KIND>
@@ -354,7 +404,82 @@ __global__ __launch_bounds__(kBlock) void k_limiter(int N, const int *__restrict
 // incidences: the edge-ordered Green-Gauss sums and the neighbour extrema are
 // gathered together, the gradient stays in registers for the limiter passes.
 // Bit-identical to k_gg_grad followed by k_limiter.
-template <class T, int KIND>
+// Staging slot of a walk over node records: the qa record (q and
+// coordinates) of the next neighbour, its area vector when AREA, and the
+// index of the incidence after it.
+template <class T>
+struct QaStage {
+    static constexpr int P = 16 / (int)sizeof(T);  // values per 16-byte chunk
+    static constexpr int QC = 8 / P;               // chunks of a qa record (8 values)
+    static constexpr int AW = (int)sizeof(T) / 4;  // dwords per area component
+    static constexpr int AUX = 1 + 3 * AW;
+};
+
+template <class T, bool AREA>
+__device__ __forceinline__ void stage_qa(const T *qa, const T *isx, const T *isy, const T *isz, const int *nbr,
+                                         int jr, size_t idx, bool has_next, uint4 (*slot)[64], uint32_t (*aux)[64]) {
+    using S = QaStage<T>;
+    const int j = jr >= 0 ? jr : ~jr;  // ghosts N + b have qa rows too
+    const char *q = reinterpret_cast<const char *>(qa + kQA * (size_t)j);
+#pragma unroll
+    for (int c = 0; c < S::QC; c++) dma16<0>(q + 16 * c, &slot[c][0]);
+    if (AREA) {
+        const char *ax = reinterpret_cast<const char *>(isx + idx), *ay = reinterpret_cast<const char *>(isy + idx),
+                   *az = reinterpret_cast<const char *>(isz + idx);
+#pragma unroll
+        for (int w = 0; w < S::AW; w++) {
+            dma4<kPolNT>(ax + 4 * w, &aux[1 + w][0]);
+            dma4<kPolNT>(ay + 4 * w, &aux[1 + S::AW + w][0]);
+            dma4<kPolNT>(az + 4 * w, &aux[1 + 2 * S::AW + w][0]);
+        }
+    }
+    if (has_next) dma4<kPolNT>(nbr + idx + kSlice, &aux[0][0]);
+}
+
+template <class T>
+__device__ __forceinline__ T aux_word(const uint32_t (*aux)[64], int row, int lane) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t v = (uint64_t)aux[row][lane] | ((uint64_t)aux[row + 1][lane] << 32);
+        return __builtin_bit_cast(T, v);
+    } else {
+        return __builtin_bit_cast(T, aux[row][lane]);
+    }
+}
+
+// Walks one node's incidences (SELL entries base, base + 64, ...) in order
+// with the next neighbour's record staged in LDS; body(jr, r, s0, s1, s2)
+// receives the neighbour's record r[8] (and the area vector when AREA).
+template <class T, bool AREA, class F>
+__device__ __forceinline__ void staged_walk(const T *qa, const T *isx, const T *isy, const T *isz,
+                                            const int *nbr, size_t base, int deg, uint4 (*slot)[64],
+                                            uint32_t (*aux)[64], int lane, F body) {
+    using S = QaStage<T>;
+    int jr = 0;
+    if (deg > 0) {
+        jr = ld_stream(nbr + base);
+        stage_qa<T, AREA>(qa, isx, isy, isz, nbr, jr, base, deg > 1, slot, aux);
+    }
+    size_t idx = base;
+    for (int k = 0; k < deg; k++, idx += kSlice) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        T r[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) r[e] = reinterpret_cast<const T *>(&slot[e / S::P][lane])[e % S::P];
+        T s0 = T(0.0), s1 = T(0.0), s2 = T(0.0);
+        if (AREA) {
+            s0 = aux_word<T>(aux, 1, lane);
+            s1 = aux_word<T>(aux, 1 + S::AW, lane);
+            s2 = aux_word<T>(aux, 1 + 2 * S::AW, lane);
+        }
+        const int jr_next = (int)aux[0][lane];
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        if (k + 1 < deg) stage_qa<T, AREA>(qa, isx, isy, isz, nbr, jr_next, idx + kSlice, k + 2 < deg, slot, aux);
+        body(jr, r, s0, s1, s2);
+        jr = jr_next;
+    }
+}
+
+template <class T, int KIND, bool STAGE>
 __global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restrict__ ip, const int *__restrict__ sptr,
                                                        const int *__restrict__ nbr, const T *__restrict__ isx,
                                                        const T *__restrict__ isy, const T *__restrict__ isz,
@@ -373,6 +498,72 @@ __global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restr
     for (int k = 0; k < 5; k++) qmin[k] = qmax[k] = qc[k];
     const int deg = ip[c + 1] - ip[c];
     const size_t base = (size_t)sptr[c / kSlice] + (c % kSlice);
+    if constexpr (STAGE) {
+        // the same three walks with every neighbour record staged in LDS
+        __shared__ uint4 stage[kBlock / 64][QaStage<T>::QC][64];
+        __shared__ uint32_t stage_aux[kBlock / 64][QaStage<T>::AUX][64];
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+        uint4(*slot)[64] = stage[wv];
+        uint32_t(*aux)[64] = stage_aux[wv];
+        const int lane = threadIdx.x % 64;
+        staged_walk<T, true>(qa, isx, isy, isz, nbr, base, deg, slot, aux, lane,
+                             [&](int jr, const T *r, T s0, T s1, T s2) {
+            if (jr >= N) {
+#pragma unroll
+                for (int f = 0; f < 5; f++) {
+                    gs[3 * f + 0] += qc[f] * s0;
+                    gs[3 * f + 1] += qc[f] * s1;
+                    gs[3 * f + 2] += qc[f] * s2;
+                }
+                return;
+            }
+            const bool iL = jr >= 0;
+#pragma unroll
+            for (int f = 0; f < 5; f++) {
+                const T qj = r[f];
+                qmin[f] = tmin(qmin[f], qj);
+                qmax[f] = tmax(qmax[f], qj);
+                const T qf = T(0.5) * (qc[f] + qj);
+                if (iL) {
+                    gs[3 * f + 0] += qf * s0;
+                    gs[3 * f + 1] += qf * s1;
+                    gs[3 * f + 2] += qf * s2;
+                } else {
+                    gs[3 * f + 0] -= qf * s0;
+                    gs[3 * f + 1] -= qf * s1;
+                    gs[3 * f + 2] -= qf * s2;
+                }
+            }
+        });
+        const T v = vol[c];
+        T gr[15];
+        T *rb = qb + kQB * (size_t)c;
+#pragma unroll
+        for (int f = 0; f < 15; f++) {
+            gr[f] = gs[f] / v;
+            rb[f] = gr[f];
+        }
+        T ph[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) ph[k] = T(1.0);
+        T eps2 = k3;
+        eps2 = T(6.0) * eps2 * v / T(M_PI);
+        staged_walk<T, false>(qa, isx, isy, isz, nbr, base, deg, slot, aux, lane,
+                              [&](int jr, const T *r, T, T, T) {
+            if (jr >= N) return;
+            limiter_nbr<T, KIND>(r, r[5], r[6], r[7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph);
+        });
+        bool zeroed = false, risky = false;
+        staged_walk<T, false>(qa, isx, isy, isz, nbr, base, deg, slot, aux, lane,
+                              [&](int jr, const T *r, T, T, T) {
+            if (jr >= N) return;
+            pressure_nbr<T>(r, r[5], r[6], r[7], qc, xc, yc, zc, gr, frac, g, ph, zeroed, risky);
+        });
+        if (pressure_final<T>(qc, g, zeroed, risky)) atomicAdd(risk, 1);
+#pragma unroll
+        for (int k = 0; k < 5; k++) rb[kPHI + k] = ph[k];
+        return;
+    }
     size_t idx = base;
     for (int k = 0; k < deg; k++, idx += kSlice) {
         const int jr = nbr[idx];
@@ -507,9 +698,6 @@ struct ResArgs {
 // holds an in-flight load, so nothing in the loop waits on the prefetch
 // before the next incidence begins.  Each wave owns one staging slot; a
 // wave-wide load of width W to row r puts lane l's bytes at row r, column l.
-typedef __attribute__((address_space(3))) void lds_void;
-typedef __attribute__((address_space(1))) void glb_void;
-
 template <class T, int ORDER>
 struct NbrStage {
     static constexpr int P = 16 / (int)sizeof(T);       // values per 16-byte chunk
@@ -526,19 +714,6 @@ struct NbrRec {
     T q[NbrStage<T, ORDER>::NQ];
     T b[ORDER == 2 ? 20 : 1];
 };
-
-// cache-policy bits of the staging loads: streaming data (indices, areas)
-// is not kept in the caches, node records are
-constexpr int kPolNT = 2;
-
-template <int POL>
-__device__ __forceinline__ void dma16(const void *g, void *l) {
-    __builtin_amdgcn_global_load_lds((glb_void *)g, (lds_void *)l, 16, 0, POL);
-}
-template <int POL>
-__device__ __forceinline__ void dma4(const void *g, void *l) {
-    __builtin_amdgcn_global_load_lds((glb_void *)g, (lds_void *)l, 4, 0, POL);
-}
 
 // Stages incidence `idx` with neighbour jr: the neighbour's record, the area
 // vector and, if has_next, the neighbour index of the incidence after it.
@@ -957,12 +1132,19 @@ int launch_gradient_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p
     const T k3 = (T)pow(p->venkat_k, 3.0);
     const T g = (T)p->gamma;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, m->stream, 1, 0, m->risk);
-    if (p->limiter == CFD_LIM_VENKAT)
-        hipLaunchKernelGGL((k_gg_limiter<T, 1>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->sptr,
-                           m->nbr, b.isx, b.isy, b.isz, b.vol, b.qa, b.qb, g, frac, k3, m->risk);
-    else
-        hipLaunchKernelGGL((k_gg_limiter<T, 2>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, m->sptr,
-                           m->nbr, b.isx, b.isy, b.isz, b.vol, b.qa, b.qb, g, frac, k3, m->risk);
+    // neighbour records staged in LDS (CFD_STAGE_GL=0: direct gathers)
+    static const bool stage = !(getenv("CFD_STAGE_GL") && getenv("CFD_STAGE_GL")[0] == '0');
+#define CFD_GL_LAUNCH(KIND, STAGE)                                                                                 \
+    hipLaunchKernelGGL((k_gg_limiter<T, KIND, STAGE>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr,     \
+                       m->sptr, m->nbr, b.isx, b.isy, b.isz, b.vol, b.qa, b.qb, g, frac, k3, m->risk)
+    if (p->limiter == CFD_LIM_VENKAT) {
+        if (stage) CFD_GL_LAUNCH(1, true);
+        else CFD_GL_LAUNCH(1, false);
+    } else {
+        if (stage) CFD_GL_LAUNCH(2, true);
+        else CFD_GL_LAUNCH(2, false);
+    }
+#undef CFD_GL_LAUNCH
     CFD_HIP(m, hipGetLastError());
     return launch_roe_correction<T>(m, b, p, nb, g, frac);
 }
