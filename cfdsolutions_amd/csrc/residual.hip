@@ -278,28 +278,20 @@ __device__ __forceinline__ void limiter_nbr(const T *qj, T xj, T yj, T zj, const
     for (int k = 0; k < 5; k++) {
         const T qf = qc[k] + frac * (qj[k] - qc[k]) + (T(1.0) - frac) * (gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz);
         const T dM = qf - qc[k];
+        // The three cases (dM > 0: towards the maximum, dM < 0: towards the
+        // minimum, otherwise no bound) as selects, so the five variables'
+        // divisions are independent straight-line code; the value chosen
+        // is the one the branch would compute.
+        const bool up = dM > T(0.0), bound = up || dM < T(0.0);
+        const T dP = up ? qmax[k] - qc[k] : qmin[k] - qc[k];
+        T t;
         if (KIND == 1) {
-            if (dM > T(0.0)) {
-                const T dP = qmax[k] - qc[k];
-                T t = (dP * dP + eps2) + T(2.0) * (dM * dP);
-                t = t / (dP * dP + T(2.0) * dM * dM + dM * dP + eps2);
-                ph[k] = tmin(ph[k], t);
-            } else if (dM < T(0.0)) {
-                const T dP = qmin[k] - qc[k];
-                T t = (dP * dP + eps2) + T(2.0) * (dM * dP);
-                t = t / (dP * dP + T(2.0) * dM * dM + dM * dP + eps2);
-                ph[k] = tmin(ph[k], t);
-            } else {
-                ph[k] = tmin(ph[k], T(1.0));
-            }
+            t = (dP * dP + eps2) + T(2.0) * (dM * dP);
+            t = t / (dP * dP + T(2.0) * dM * dM + dM * dP + eps2);
         } else {
-            if (dM > T(0.0))
-                ph[k] = tmin(ph[k], tmin(T(1.0), (qmax[k] - qc[k]) / dM));
-            else if (dM < T(0.0))
-                ph[k] = tmin(ph[k], tmin(T(1.0), (qmin[k] - qc[k]) / dM));
-            else
-                ph[k] = tmin(ph[k], T(1.0));
+            t = tmin(T(1.0), dP / dM);
         }
+        ph[k] = tmin(ph[k], bound ? t : T(1.0));
         ph[k] = tmax(T(0.0), ph[k]);
     }
 }
@@ -508,31 +500,20 @@ __global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restr
         const int lane = threadIdx.x % 64;
         staged_walk<T, true>(qa, isx, isy, isz, nbr, base, deg, slot, aux, lane,
                              [&](int jr, const T *r, T s0, T s1, T s2) {
-            if (jr >= N) {
-#pragma unroll
-                for (int f = 0; f < 5; f++) {
-                    gs[3 * f + 0] += qc[f] * s0;
-                    gs[3 * f + 1] += qc[f] * s1;
-                    gs[3 * f + 2] += qc[f] * s2;
-                }
-                return;
-            }
-            const bool iL = jr >= 0;
+            // branch-free: a ghost adds the node's own value; an interior
+            // neighbour adds (as L) or subtracts (as R) the face average, and
+            // a - b equals a + (-b) exactly
+            const bool ghost = jr >= N, neg = jr < 0;
 #pragma unroll
             for (int f = 0; f < 5; f++) {
                 const T qj = r[f];
-                qmin[f] = tmin(qmin[f], qj);
-                qmax[f] = tmax(qmax[f], qj);
-                const T qf = T(0.5) * (qc[f] + qj);
-                if (iL) {
-                    gs[3 * f + 0] += qf * s0;
-                    gs[3 * f + 1] += qf * s1;
-                    gs[3 * f + 2] += qf * s2;
-                } else {
-                    gs[3 * f + 0] -= qf * s0;
-                    gs[3 * f + 1] -= qf * s1;
-                    gs[3 * f + 2] -= qf * s2;
-                }
+                qmin[f] = ghost ? qmin[f] : tmin(qmin[f], qj);
+                qmax[f] = ghost ? qmax[f] : tmax(qmax[f], qj);
+                const T qf = ghost ? qc[f] : T(0.5) * (qc[f] + qj);
+                const T a0 = qf * s0, a1 = qf * s1, a2 = qf * s2;
+                gs[3 * f + 0] += neg ? -a0 : a0;
+                gs[3 * f + 1] += neg ? -a1 : a1;
+                gs[3 * f + 2] += neg ? -a2 : a2;
             }
         });
         const T v = vol[c];
@@ -568,32 +549,19 @@ __global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restr
     for (int k = 0; k < deg; k++, idx += kSlice) {
         const int jr = nbr[idx];
         const T s0 = ld_stream(isx + idx), s1 = ld_stream(isy + idx), s2 = ld_stream(isz + idx);
-        if (jr >= N) {
+        // branch-free as in the staged walk (ghost rows of qa exist)
+        const bool ghost = jr >= N, neg = jr < 0;
+        const T *rj = qa + kQA * (size_t)(neg ? ~jr : jr);
 #pragma unroll
-            for (int f = 0; f < 5; f++) {
-                gs[3 * f + 0] += qc[f] * s0;
-                gs[3 * f + 1] += qc[f] * s1;
-                gs[3 * f + 2] += qc[f] * s2;
-            }
-        } else {
-            const bool iL = jr >= 0;
-            const T *rj = qa + kQA * (size_t)(iL ? jr : ~jr);
-#pragma unroll
-            for (int f = 0; f < 5; f++) {
-                const T qj = rj[f];
-                qmin[f] = tmin(qmin[f], qj);
-                qmax[f] = tmax(qmax[f], qj);
-                const T qf = T(0.5) * (qc[f] + qj);
-                if (iL) {
-                    gs[3 * f + 0] += qf * s0;
-                    gs[3 * f + 1] += qf * s1;
-                    gs[3 * f + 2] += qf * s2;
-                } else {
-                    gs[3 * f + 0] -= qf * s0;
-                    gs[3 * f + 1] -= qf * s1;
-                    gs[3 * f + 2] -= qf * s2;
-                }
-            }
+        for (int f = 0; f < 5; f++) {
+            const T qj = rj[f];
+            qmin[f] = ghost ? qmin[f] : tmin(qmin[f], qj);
+            qmax[f] = ghost ? qmax[f] : tmax(qmax[f], qj);
+            const T qf = ghost ? qc[f] : T(0.5) * (qc[f] + qj);
+            const T a0 = qf * s0, a1 = qf * s1, a2 = qf * s2;
+            gs[3 * f + 0] += neg ? -a0 : a0;
+            gs[3 * f + 1] += neg ? -a1 : a1;
+            gs[3 * f + 2] += neg ? -a2 : a2;
         }
     }
     const T v = vol[c];
@@ -1132,8 +1100,11 @@ int launch_gradient_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p
     const T k3 = (T)pow(p->venkat_k, 3.0);
     const T g = (T)p->gamma;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, m->stream, 1, 0, m->risk);
-    // neighbour records staged in LDS (CFD_STAGE_GL=0: direct gathers)
-    static const bool stage = !(getenv("CFD_STAGE_GL") && getenv("CFD_STAGE_GL")[0] == '0');
+    // neighbour records staged in LDS for FP32, direct gathers for FP64 (the
+    // faster of the two for each precision on the 160^3 cube; CFD_STAGE_GL=0/1
+    // forces either)
+    static const char *const env = getenv("CFD_STAGE_GL");
+    const bool stage = env ? env[0] == '1' : sizeof(T) == 4;
 #define CFD_GL_LAUNCH(KIND, STAGE)                                                                                 \
     hipLaunchKernelGGL((k_gg_limiter<T, KIND, STAGE>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr,     \
                        m->sptr, m->nbr, b.isx, b.isy, b.isz, b.vol, b.qa, b.qb, g, frac, k3, m->risk)
