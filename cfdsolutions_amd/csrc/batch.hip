@@ -153,7 +153,7 @@ bool cfd_same_solve(const cfd_solve_opts &a, const cfd_solve_opts &b) {
 
 int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
                          const cfd_solve_opts *opts, cfd_solve_result *results, double *const *q_out,
-                         int64_t *device_bytes) {
+                         int64_t *device_bytes, bool allow_graphs) {
     if (!ctx || K < 1 || !meshes || !params || !opts || !results)
         return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: invalid arguments");
     for (int c = 1; c < K; c++)
@@ -211,6 +211,7 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
     cfd_mesh *m = nullptr;
     int rc = cfd_mesh_create_cases(ctx, &ud, &params[0], node_case.data(), &m);
     if (rc) return rc;
+    if (!allow_graphs) m->use_graphs = false;
     // batch bookkeeping
     m->ncase = K;
     m->case_node = noff;
@@ -418,5 +419,5 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
 
 extern "C" int cfd_batch_solve(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
                                const cfd_solve_opts *opts, cfd_solve_result *results) {
-    return cfd_batch_solve_into(ctx, K, meshes, params, opts, results, nullptr, nullptr);
+    return cfd_batch_solve_into(ctx, K, meshes, params, opts, results, nullptr, nullptr, true);
 }

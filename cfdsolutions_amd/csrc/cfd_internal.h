@@ -268,12 +268,14 @@ int launch_residual_to_fp64(cfd_mesh *m);
 
 int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed);
 // batched solves (batch.hip): cases that may share one union mesh, and the
-// solve itself with each case's final state copied to q_out[c] (may be NULL)
+// solve itself with each case's final state copied to q_out[c] (may be NULL);
+// allow_graphs false launches the Arnoldi steps directly (callers whose other
+// threads use the runtime meanwhile: stream capture is not safe beside them)
 bool cfd_same_scheme(const cfd_params &a, const cfd_params &b);
 bool cfd_same_solve(const cfd_solve_opts &a, const cfd_solve_opts &b);
 int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
                          const cfd_solve_opts *opts, cfd_solve_result *results, double *const *q_out,
-                         int64_t *device_bytes);
+                         int64_t *device_bytes, bool allow_graphs);
 int launch_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, int add_time, int rhs_mode,
                        int mixed);
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);

@@ -221,8 +221,10 @@ std::vector<cfd_completion> run_batch(cfd_server *s, const Job &job) {
     }
     const double t_admit = s->now();
     int64_t bytes = 0;
+    // direct launches, as for served meshes: other workers use the runtime
+    // while this batch runs, and stream capture is not safe beside them
     const int rc = cfd_batch_solve_into(s->ctx, K, descs.data(), params.data(), opts.data(), res.data(), qo.data(),
-                                        &bytes);
+                                        &bytes, false);
     const double t_done = s->now();
     std::vector<cfd_completion> out(K);
     for (int c = 0; c < K; c++) {
