@@ -75,7 +75,8 @@ class SolveResult(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("max_inflight", ctypes.c_int32), ("lookahead", ctypes.c_int32), ("max_bypass", ctypes.c_int32),
                 ("keep_resident", ctypes.c_int32), ("hbm_budget", ctypes.c_int64), ("batch_max", ctypes.c_int32),
-                ("batch_nodes", ctypes.c_int32), ("batch_wait_us", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("batch_nodes", ctypes.c_int32), ("batch_wait_us", ctypes.c_int32),
+                ("batch_case_nodes", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
@@ -335,11 +336,11 @@ class Server:
     arrays of a request are kept alive here until its completion is polled."""
 
     def __init__(self, ctx, hbm_budget, max_inflight=4, lookahead=16, max_bypass=8, keep_resident=True, batch_max=0,
-                 batch_nodes=0, batch_wait_us=0):
+                 batch_nodes=0, batch_wait_us=0, batch_case_nodes=0):
         self.ctx = ctx
         self.h = ctypes.c_void_p()
         o = ServerOpts(int(max_inflight), int(lookahead), int(max_bypass), int(bool(keep_resident)), int(hbm_budget),
-                       int(batch_max), int(batch_nodes), int(batch_wait_us), 0)
+                       int(batch_max), int(batch_nodes), int(batch_wait_us), int(batch_case_nodes))
         ctx.check(lib().cfd_server_create(ctx.h, ctypes.byref(o), ctypes.byref(self.h)))
         self._live = {}
         self._descs = {}

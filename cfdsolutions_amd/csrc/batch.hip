@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -151,74 +152,88 @@ bool cfd_same_solve(const cfd_solve_opts &a, const cfd_solve_opts &b) {
            a.lin.max_restarts == b.lin.max_restarts && a.lin.rtol == b.lin.rtol && a.lin.mixed == b.lin.mixed;
 }
 
-int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
-                         const cfd_solve_opts *opts, cfd_solve_result *results, double *const *q_out,
-                         int64_t *device_bytes, bool allow_graphs) {
-    if (!ctx || K < 1 || !meshes || !params || !opts || !results)
-        return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: invalid arguments");
-    for (int c = 1; c < K; c++)
-        if (!cfd_same_scheme(params[0], params[c]) || !cfd_same_solve(opts[0], opts[c]))
-            return cfd_set_error(ctx, CFD_E_ARG,
-                                 "cfd_batch_solve: case %d differs from case 0 in scheme or solver options", c);
-    const cfd_solve_opts &o = opts[0];
-    if (o.lin.solver != CFD_SOLVER_FGMRES)
-        return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: batched solves use FGMRES");
-    const auto t0 = std::chrono::steady_clock::now();
-    // union mesh
-    std::vector<int> noff(K + 1, 0), eoff(K + 1, 0), boff(K + 1, 0);
+namespace {
+
+// One union mesh over a subset of the cases of a batch.
+struct Union {
+    cfd_mesh *m = nullptr;
+    std::vector<int> cases;       // batch index of each union case
+    std::vector<int> noff, boff;  // node and boundary-edge offsets of each union case
+    int *d_active = nullptr;      // device [K] 1 while a union case still iterates
+};
+
+// Builds the union mesh of the batch cases `cases`.  Each case starts from
+// q_init[c] ([5][n_c + b_c], nodes then ghosts) when that is not empty, else
+// from its freestream.
+int build_union(cfd_ctx *ctx, const std::vector<int> &cases, const cfd_mesh_desc *meshes, const cfd_params *params,
+                const std::vector<std::vector<double>> &q_init, bool allow_graphs, Union &u) {
+    const int K = (int)cases.size();
+    auto tph = std::chrono::steady_clock::now();
+    auto phase = [&](const char *what) {
+        if (!cfd_progress()) return;
+        const auto t = std::chrono::steady_clock::now();
+        fprintf(stderr, "cfd_batch_solve: %-28s %8.3f s\n", what, std::chrono::duration<double>(t - tph).count());
+        tph = t;
+    };
+    u = Union{};
+    u.cases = cases;
+    std::vector<int> &noff = u.noff, &boff = u.boff, eoff(K + 1, 0);
+    noff.assign(K + 1, 0);
+    boff.assign(K + 1, 0);
     for (int c = 0; c < K; c++) {
-        const cfd_mesh_desc &d = meshes[c];
-        if (d.n_node <= 0 || d.n_edge < 0 || d.n_bedge < 0)
-            return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: case %d has an invalid mesh", c);
+        const cfd_mesh_desc &d = meshes[cases[c]];
         noff[c + 1] = noff[c] + d.n_node;
         eoff[c + 1] = eoff[c] + d.n_edge;
         boff[c + 1] = boff[c] + d.n_bedge;
-        if ((int64_t)noff[c + 1] + boff[c + 1] > INT32_MAX / 2 || 2LL * eoff[c + 1] + boff[c + 1] > INT32_MAX / 2)
-            return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: union mesh too large for int32 indices");
     }
     const int N = noff[K], E = eoff[K], B = boff[K];
     std::vector<int> edge(2 * (size_t)E), bedge(2 * (size_t)B), btype(B), btag(B), ia(N + 1, 0), ja(2 * (size_t)E),
         node_case(N), bcase(B);
     std::vector<double> earea(3 * (size_t)E), barea(3 * (size_t)B), xyz(3 * (size_t)N), vol(N);
-    for (int c = 0; c < K; c++) {
-        const cfd_mesh_desc &d = meshes[c];
-        const int no = noff[c], eo = eoff[c], bo = boff[c];
-        for (int e = 0; e < d.n_edge; e++) {
-            edge[2 * (size_t)(eo + e)] = d.edge[2 * e] + no;
-            edge[2 * (size_t)(eo + e) + 1] = d.edge[2 * e + 1] + no;
-        }
-        memcpy(&earea[3 * (size_t)eo], d.edge_area, sizeof(double) * 3 * d.n_edge);
-        for (int b = 0; b < d.n_bedge; b++) {
-            if (d.bedge[2 * b + 1] != d.n_node + b)
-                return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: case %d boundary edge %d ghost index", c, b);
-            bedge[2 * (size_t)(bo + b)] = d.bedge[2 * b] + no;
-            bedge[2 * (size_t)(bo + b) + 1] = N + bo + b;
-            btype[bo + b] = d.bedge_type[b];
-            btag[bo + b] = d.bedge_tag ? d.bedge_tag[b] : 0;
-            bcase[bo + b] = c;
-        }
-        memcpy(&barea[3 * (size_t)bo], d.bedge_area, sizeof(double) * 3 * d.n_bedge);
-        memcpy(&xyz[3 * (size_t)no], d.xyz, sizeof(double) * 3 * d.n_node);
-        memcpy(&vol[no], d.vol, sizeof(double) * d.n_node);
-        for (int i = 0; i < d.n_node; i++) {
-            ia[no + i + 1] = ia[no] + d.n2n_ia[i + 1];
-            node_case[no + i] = c;
-        }
-        for (int k = 0; k < d.n2n_ia[d.n_node]; k++) ja[(size_t)ia[no] + k] = d.n2n_ja[k] + no;
-    }
+    // each case fills its own ranges; its n2n lists start at entry 2 * eoff[c]
+    cfd_parallel_for(
+        K,
+        [&](int64_t c0, int64_t c1) {
+            for (int c = (int)c0; c < (int)c1; c++) {
+                const cfd_mesh_desc &d = meshes[cases[c]];
+                const int no = noff[c], eo = eoff[c], bo = boff[c];
+                for (int e = 0; e < d.n_edge; e++) {
+                    edge[2 * (size_t)(eo + e)] = d.edge[2 * e] + no;
+                    edge[2 * (size_t)(eo + e) + 1] = d.edge[2 * e + 1] + no;
+                }
+                memcpy(&earea[3 * (size_t)eo], d.edge_area, sizeof(double) * 3 * d.n_edge);
+                for (int b = 0; b < d.n_bedge; b++) {
+                    bedge[2 * (size_t)(bo + b)] = d.bedge[2 * b] + no;
+                    bedge[2 * (size_t)(bo + b) + 1] = N + bo + b;
+                    btype[bo + b] = d.bedge_type[b];
+                    btag[bo + b] = d.bedge_tag ? d.bedge_tag[b] : 0;
+                    bcase[bo + b] = c;
+                }
+                memcpy(&barea[3 * (size_t)bo], d.bedge_area, sizeof(double) * 3 * d.n_bedge);
+                memcpy(&xyz[3 * (size_t)no], d.xyz, sizeof(double) * 3 * d.n_node);
+                memcpy(&vol[no], d.vol, sizeof(double) * d.n_node);
+                for (int i = 0; i < d.n_node; i++) {
+                    ia[no + i + 1] = 2 * eo + d.n2n_ia[i + 1];
+                    node_case[no + i] = c;
+                }
+                for (int k = 0; k < d.n2n_ia[d.n_node]; k++) ja[2 * (size_t)eo + k] = d.n2n_ja[k] + no;
+            }
+        },
+        1);
     cfd_mesh_desc ud{N, E, B, edge.data(), earea.data(), bedge.data(), barea.data(), btype.data(), btag.data(),
                      xyz.data(), vol.data(), ia.data(), ja.data()};
-    cfd_mesh *m = nullptr;
-    int rc = cfd_mesh_create_cases(ctx, &ud, &params[0], node_case.data(), &m);
+    phase("union assembly");
+    int rc = cfd_mesh_create_cases(ctx, &ud, &params[cases[0]], node_case.data(), &u.m);
     if (rc) return rc;
+    phase("union mesh");
+    cfd_mesh *m = u.m;
     if (!allow_graphs) m->use_graphs = false;
-    // batch bookkeeping
     m->ncase = K;
     m->case_node = noff;
     m->case_bedge = boff;
     std::vector<double> inf((size_t)K * 8, 0.0);
     for (int c = 0; c < K; c++) {
-        const cfd_params &p = params[c];
+        const cfd_params &p = params[cases[c]];
         double *f = &inf[(size_t)c * 8];
         f[0] = p.inf_rho;
         f[1] = p.inf_u;
@@ -232,77 +247,214 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
         const int nc = noff[c + 1] - noff[c], bc = boff[c + 1] - boff[c];
         zoff[c] = noff[c];
         zlen[c] = nc;
-        nb[c] = std::min(grid_for(nc), 1024);                                              // launch_rms
+        nb[c] = std::min(grid_for(nc), 1024);  // launch_rms
         eoffv[c] = 5 * noff[c];
         elen[c] = 5 * nc;
         nbd[c] = (int)std::max<size_t>(1, std::min<size_t>((5 * (size_t)nc + kBlock - 1) / kBlock, kDotParts));  // mdot
         bo[c] = boff[c];
         blen[c] = bc;
-        nbf[c] = bc ? std::min(grid_for(bc), 256) : 1;                                     // launch_forces
+        nbf[c] = bc ? std::min(grid_for(bc), 256) : 1;  // launch_forces
     }
-    std::vector<char> active(K, 1), solve(K, 1);
-    std::vector<int> iters, act_i(K, 1);
-    std::vector<double> fres;
-    cfd_params p = params[0];
-    p.precision = CFD_FP64;
-    if (!rc) {
-        m->case_inf = (double *)cfd_dalloc(m, sizeof(double) * inf.size());
-        m->bcase = (int *)cfd_dalloc(m, sizeof(int) * std::max(1, B));
-        m->ecase = (int *)cfd_dalloc(m, sizeof(int) * N);
-        m->bred = (double *)cfd_dalloc(m, sizeof(double) * kBOut * K);
-        m->bcoef = (double *)cfd_dalloc(m, sizeof(double) * (kMaxKrylov + 1) * K);
-        m->bkdim = (int *)cfd_dalloc(m, sizeof(int) * K);
-        if (!m->case_inf || !m->bcase || !m->ecase || !m->bred || !m->bcoef || !m->bkdim)
-            rc = cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of device memory");
-    }
-    if (!rc && (hipHostMalloc((void **)&m->hbred, sizeof(double) * kBOut * K, hipHostMallocDefault) != hipSuccess ||
-                hipHostMalloc((void **)&m->hbcoef, sizeof(double) * (kMaxKrylov + 1) * K, hipHostMallocDefault) !=
-                    hipSuccess ||
-                hipHostMalloc((void **)&m->hbkdim, sizeof(int) * K, hipHostMallocDefault) != hipSuccess))
-        rc = cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of pinned host memory");
-    if (!rc && hipMemcpy(m->case_inf, inf.data(), sizeof(double) * inf.size(), hipMemcpyHostToDevice) != hipSuccess)
-        rc = cfd_set_error(ctx, CFD_E_HIP, "hipMemcpy failed");
-    if (!rc && B && hipMemcpy(m->bcase, bcase.data(), sizeof(int) * B, hipMemcpyHostToDevice) != hipSuccess)
-        rc = cfd_set_error(ctx, CFD_E_HIP, "hipMemcpy failed");
-    if (!rc && hipMemcpy(m->ecase, node_case.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess)
-        rc = cfd_set_error(ctx, CFD_E_HIP, "hipMemcpy failed");
-    if (!rc) rc = build_segmap(m, m->seg_rms, zoff, zlen, nb);
+    m->case_inf = (double *)cfd_dalloc(m, sizeof(double) * inf.size());
+    m->bcase = (int *)cfd_dalloc(m, sizeof(int) * std::max(1, B));
+    m->ecase = (int *)cfd_dalloc(m, sizeof(int) * N);
+    m->bred = (double *)cfd_dalloc(m, sizeof(double) * kBOut * K);
+    m->bcoef = (double *)cfd_dalloc(m, sizeof(double) * (kMaxKrylov + 1) * K);
+    m->bkdim = (int *)cfd_dalloc(m, sizeof(int) * K);
+    u.d_active = (int *)cfd_dalloc(m, sizeof(int) * K);
+    if (!m->case_inf || !m->bcase || !m->ecase || !m->bred || !m->bcoef || !m->bkdim || !u.d_active)
+        return cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of device memory");
+    if (hipHostMalloc((void **)&m->hbred, sizeof(double) * kBOut * K, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&m->hbcoef, sizeof(double) * (kMaxKrylov + 1) * K, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&m->hbkdim, sizeof(int) * K, hipHostMallocDefault) != hipSuccess)
+        return cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of pinned host memory");
+    const std::vector<int> ones(K, 1);
+    if (hipMemcpy(m->case_inf, inf.data(), sizeof(double) * inf.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        (B && hipMemcpy(m->bcase, bcase.data(), sizeof(int) * B, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(m->ecase, node_case.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(u.d_active, ones.data(), sizeof(int) * K, hipMemcpyHostToDevice) != hipSuccess)
+        return cfd_set_error(ctx, CFD_E_HIP, "hipMemcpy failed");
+    m->case_active = u.d_active;
+    rc = build_segmap(m, m->seg_rms, zoff, zlen, nb);
     if (!rc) rc = build_segmap(m, m->seg_dot, eoffv, elen, nbd);
     if (!rc) rc = build_segmap(m, m->seg_forces, bo, blen, nbf);
-    if (!rc) {
-        const size_t np = std::max({(size_t)m->seg_dot.nblocks * (kMaxKrylov + 1), (size_t)m->seg_rms.nblocks * 5,
-                                    (size_t)m->seg_forces.nblocks * 4});
-        m->bpart = (double *)cfd_dalloc(m, sizeof(double) * np);
-        if (!m->bpart) rc = cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of device memory");
-    }
-    // initial state: each case at its own freestream (nodes and ghosts)
-    if (!rc) {
-        std::vector<double> q(5 * (size_t)(N + B));
-        auto put = [&](size_t i, int c) {
-            const cfd_params &pc = params[c];
+    if (rc) return rc;
+    const size_t np = std::max({(size_t)m->seg_dot.nblocks * (kMaxKrylov + 1), (size_t)m->seg_rms.nblocks * 5,
+                                (size_t)m->seg_forces.nblocks * 4});
+    m->bpart = (double *)cfd_dalloc(m, sizeof(double) * np);
+    if (!m->bpart) return cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of device memory");
+    // initial state of every case, nodes and ghosts
+    std::vector<double> q(5 * (size_t)(N + B));
+    for (int c = 0; c < K; c++) {
+        const size_t nc = (size_t)(noff[c + 1] - noff[c]), bc = (size_t)(boff[c + 1] - boff[c]), tc = nc + bc;
+        const std::vector<double> &qi = q_init[cases[c]];
+        for (int k = 0; k < 5; k++) {
+            double *dn = q.data() + (size_t)k * (N + B) + noff[c], *dg = q.data() + (size_t)k * (N + B) + N + boff[c];
+            if (!qi.empty()) {
+                memcpy(dn, qi.data() + k * tc, sizeof(double) * nc);
+                memcpy(dg, qi.data() + k * tc + nc, sizeof(double) * bc);
+                continue;
+            }
+            const cfd_params &pc = params[cases[c]];
             double et = pc.inf_u * pc.inf_u + pc.inf_v * pc.inf_v + pc.inf_w * pc.inf_w;
             et = pc.inf_p / ((pc.gamma - 1.0) * pc.inf_rho) + 0.5 * et;
             const double v[5] = {pc.inf_rho, pc.inf_rho * pc.inf_u, pc.inf_rho * pc.inf_v, pc.inf_rho * pc.inf_w,
                                  pc.inf_rho * et};
-            for (int k = 0; k < 5; k++) q[(size_t)k * (N + B) + i] = v[k];
-        };
-        for (int i = 0; i < N; i++) put(i, node_case[i]);
-        for (int b = 0; b < B; b++) put((size_t)N + b, bcase[b]);
-        rc = cfd_field_set(m, CFD_FIELD_Q, q.data());
+            std::fill(dn, dn + nc, v[k]);
+            std::fill(dg, dg + bc, v[k]);
+        }
     }
+    rc = cfd_field_set(m, CFD_FIELD_Q, q.data());
+    phase("segments and state");
+    return rc;
+}
+
+// Final results of the union cases in `take` (union indices): forces, time
+// and, where asked or where keep[c] is set, the state [5][n_c + b_c] of each.
+int harvest(Union &u, const std::vector<int> &take, const cfd_solve_opts *opts, const cfd_params &p, double secs,
+            cfd_solve_result *results, double *const *q_out, const std::vector<char> &keep,
+            std::vector<std::vector<double>> &q_keep) {
+    cfd_mesh *m = u.m;
+    const int K = (int)u.cases.size();
+    hipLaunchKernelGGL(k_forces_seg, dim3(m->seg_forces.nblocks), dim3(kBlock), 0, m->stream, m->seg_forces.blk,
+                       m->seg_forces.first, m->seg_forces.off, m->seg_forces.len, m->bnode, m->btype, m->btag,
+                       opts[0].wall_tag, m->d.sa, m->d.qa, p.gamma, m->bpart);
+    hipLaunchKernelGGL(k_reduce_seg<4>, dim3(K), dim3(kBlock), 0, m->stream, m->seg_forces.first, m->bpart, m->bred);
+    if (hipMemcpyAsync(m->hbred, m->bred, sizeof(double) * kBOut * K, hipMemcpyDeviceToHost, m->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+        return cfd_set_error(m->ctx, CFD_E_HIP, "cfd_batch_solve: reading forces failed");
+    bool need_q = false;
+    for (int lc : take) need_q = need_q || (q_out && q_out[u.cases[lc]]);
+    for (int lc = 0; lc < K; lc++) need_q = need_q || keep[u.cases[lc]];
+    for (int lc : take) {
+        const int c = u.cases[lc];
+        cfd_solve_result &r = results[c];
+        const double *v = m->hbred + (size_t)lc * kBOut;
+        const double a = opts[c].alpha_deg * M_PI / 180.0;
+        r.force[0] = v[0];
+        r.force[1] = v[1];
+        r.force[2] = v[2];
+        r.drag = v[0] * cos(a) + v[1] * sin(a);
+        r.lift = -v[0] * sin(a) + v[1] * cos(a);
+        r.pressure_integral = v[3];
+        r.seconds = secs;
+    }
+    if (!need_q) return CFD_OK;
+    const int N = m->N, B = m->B;
+    const size_t NT = (size_t)N + B;
+    std::vector<double> all(5 * NT);
+    const int rc = cfd_field_get(m, CFD_FIELD_Q, all.data());
+    if (rc) return rc;
+    auto extract = [&](int lc, double *dst) {
+        const size_t nc = (size_t)(u.noff[lc + 1] - u.noff[lc]), bc = (size_t)(u.boff[lc + 1] - u.boff[lc]),
+                     tc = nc + bc;
+        for (int k = 0; k < 5; k++) {
+            const double *src = all.data() + k * NT;
+            memcpy(dst + k * tc, src + u.noff[lc], sizeof(double) * nc);
+            memcpy(dst + k * tc + nc, src + N + u.boff[lc], sizeof(double) * bc);
+        }
+    };
+    for (int lc : take)
+        if (q_out && q_out[u.cases[lc]]) extract(lc, q_out[u.cases[lc]]);
+    for (int lc = 0; lc < K; lc++) {
+        const int c = u.cases[lc];
+        if (!keep[c]) continue;
+        q_keep[c].resize(5 * (size_t)(u.noff[lc + 1] - u.noff[lc] + u.boff[lc + 1] - u.boff[lc]));
+        extract(lc, q_keep[c].data());
+    }
+    return CFD_OK;
+}
+
+}  // namespace
+
+int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
+                         const cfd_solve_opts *opts, cfd_solve_result *results, double *const *q_out,
+                         int64_t *device_bytes, bool allow_graphs, cfd_case_done_fn done, void *done_arg) {
+    if (!ctx || K < 1 || !meshes || !params || !opts || !results)
+        return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: invalid arguments");
+    for (int c = 1; c < K; c++)
+        if (!cfd_same_scheme(params[0], params[c]) || !cfd_same_solve(opts[0], opts[c]))
+            return cfd_set_error(ctx, CFD_E_ARG,
+                                 "cfd_batch_solve: case %d differs from case 0 in scheme or solver options", c);
+    const cfd_solve_opts &o = opts[0];
+    if (o.lin.solver != CFD_SOLVER_FGMRES)
+        return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: batched solves use FGMRES");
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    int64_t tn = 0, te = 0, tb = 0;
+    for (int c = 0; c < K; c++) {
+        const cfd_mesh_desc &d = meshes[c];
+        if (d.n_node <= 0 || d.n_edge < 0 || d.n_bedge < 0 || !d.n2n_ia || !d.n2n_ja || d.n2n_ia[0] != 0 ||
+            d.n2n_ia[d.n_node] != 2 * d.n_edge)
+            return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: case %d has an invalid mesh", c);
+        for (int b = 0; b < d.n_bedge; b++)
+            if (d.bedge[2 * b + 1] != d.n_node + b)
+                return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: case %d boundary edge %d ghost index", c, b);
+        tn += d.n_node;
+        te += d.n_edge;
+        tb += d.n_bedge;
+        if (tn + tb > INT32_MAX / 2 || 2 * te + tb > INT32_MAX / 2)
+            return cfd_set_error(ctx, CFD_E_ARG, "cfd_batch_solve: union mesh too large for int32 indices");
+    }
+    // Compaction: once the cases still iterating hold at most half of the
+    // union's nodes, their states move to a new union of just those cases
+    // and the finished ones are harvested.  A case's arithmetic does not
+    // depend on the other cases of its union, so its iterates are unchanged.
+    // CFD_BATCH_COMPACT=0 never compacts, =1 compacts at any union size.
+    const char *ce = getenv("CFD_BATCH_COMPACT");
+    const int64_t compact_min = ce ? (ce[0] == '0' ? INT64_MAX : 0) : 2 * (int64_t)kSellMinRows;
     for (int c = 0; c < K; c++) results[c] = cfd_solve_result{};
-    // cases still iterating: the update and the boundary condition skip the
-    // others, so a finished case keeps its last iteration's state and ghosts
-    int *d_active = nullptr;
-    if (!rc) {
-        d_active = (int *)cfd_dalloc(m, sizeof(int) * K);
-        if (!d_active) rc = cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of device memory");
-        else if (hipMemcpy(d_active, act_i.data(), sizeof(int) * K, hipMemcpyHostToDevice) != hipSuccess)
-            rc = cfd_set_error(ctx, CFD_E_HIP, "hipMemcpy failed");
-        m->case_active = d_active;
-    }
+    std::vector<std::vector<double>> q_keep(K);
+    std::vector<char> active(K, 1), keep(K, 0);
+    std::vector<int> all(K);
+    for (int c = 0; c < K; c++) all[c] = c;
+    Union u;
+    int rc = build_union(ctx, all, meshes, params, q_keep, allow_graphs, u);
+    if (device_bytes) *device_bytes = u.m ? u.m->bytes : 0;
+    std::vector<char> solve;
+    std::vector<int> iters, act_i;
+    std::vector<double> fres;
+    cfd_params p = params[0];
+    p.precision = CFD_FP64;
     // backward-Euler Newton iterations in lockstep (cfd_solve_steady per case)
     for (int it = 0; !rc && it < o.max_iter; it++) {
+        if (it > 0 && u.m->N >= compact_min) {
+            // the cases that finished at the last residual check hold their
+            // final state (nodes and ghosts) from here on
+            int64_t live = 0;
+            std::vector<int> take, take_l, rest;
+            for (int lc = 0; lc < (int)u.cases.size(); lc++) {
+                const int c = u.cases[lc];
+                if (active[c]) {
+                    live += u.noff[lc + 1] - u.noff[lc];
+                    rest.push_back(c);
+                } else {
+                    take.push_back(c);
+                    take_l.push_back(lc);
+                }
+            }
+            if (2 * live <= u.m->N) {
+                for (int c : rest) keep[c] = 1;
+                rc = harvest(u, take_l, opts, p, elapsed(), results, q_out, keep, q_keep);
+                if (rc) break;
+                cfd_mesh_destroy(u.m);
+                u.m = nullptr;
+                for (int c : take)
+                    if (done) done(done_arg, c);
+                rc = build_union(ctx, rest, meshes, params, q_keep, allow_graphs, u);
+                for (int c : rest) {
+                    keep[c] = 0;
+                    std::vector<double>().swap(q_keep[c]);
+                }
+                if (rc) break;
+                if (cfd_progress())
+                    fprintf(stderr, "cfd_batch_solve: iteration %d, union compacted to %d cases (%d nodes), %.2f s\n",
+                            it, (int)rest.size(), u.m->N, elapsed());
+            }
+        }
+        cfd_mesh *m = u.m;
+        const int Ku = (int)u.cases.size();
         cfd_params pi = p;
         if (pi.order == 2 && it < o.first_order_iters) pi.order = 1;
         rc = cfd_bc(m, &pi, it);
@@ -318,19 +470,20 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
         if (rc) break;
         hipLaunchKernelGGL(k_rms_seg, dim3(m->seg_rms.nblocks), dim3(kBlock), 0, m->stream, m->seg_rms.blk,
                            m->seg_rms.first, m->seg_rms.off, m->seg_rms.len, m->d.res, m->bpart);
-        hipLaunchKernelGGL(k_reduce_seg<5>, dim3(K), dim3(kBlock), 0, m->stream, m->seg_rms.first, m->bpart, m->bred);
-        if (hipMemcpyAsync(m->hbred, m->bred, sizeof(double) * kBOut * K, hipMemcpyDeviceToHost, m->stream) !=
+        hipLaunchKernelGGL(k_reduce_seg<5>, dim3(Ku), dim3(kBlock), 0, m->stream, m->seg_rms.first, m->bpart, m->bred);
+        if (hipMemcpyAsync(m->hbred, m->bred, sizeof(double) * kBOut * Ku, hipMemcpyDeviceToHost, m->stream) !=
                 hipSuccess ||
             hipStreamSynchronize(m->stream) != hipSuccess) {
             rc = cfd_set_error(ctx, CFD_E_HIP, "cfd_batch_solve: reading residual norms failed");
             break;
         }
-        bool any = false;
-        for (int c = 0; c < K; c++) {
+        int64_t live_nodes = 0;
+        for (int lc = 0; lc < Ku; lc++) {
+            const int c = u.cases[lc];
             if (!active[c]) continue;
-            const double *sv = m->hbred + (size_t)c * kBOut;
+            const double *sv = m->hbred + (size_t)lc * kBOut;
             const double t = sv[0] + sv[1] + sv[2] + sv[3] + sv[4];
-            const double rms = sqrt(t / (5.0 * (double)(noff[c + 1] - noff[c])));
+            const double rms = sqrt(t / (5.0 * (double)(u.noff[lc + 1] - u.noff[lc])));
             cfd_solve_result &r = results[c];
             if (it == 0) r.first_rms = rms;
             r.final_rms = rms;
@@ -342,82 +495,65 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
                 r.converged = 1;
                 active[c] = 0;
             }
-            any = any || active[c];
+            if (active[c]) live_nodes += u.noff[lc + 1] - u.noff[lc];
         }
-        if (!any) break;
+        if (!live_nodes) break;
+        const int Kn = Ku;
         rc = cfd_delta_t(m, &pi, it, nullptr);
         if (!rc) rc = cfd_jacobian(m, &pi, 1, o.lin.mixed);
         if (rc) break;
-        for (int c = 0; c < K; c++) solve[c] = active[c];
+        solve.assign(Kn, 0);
+        for (int lc = 0; lc < Kn; lc++) solve[lc] = active[u.cases[lc]];
         rc = run_linsolve_batch(m, &o.lin, solve, iters, fres);
         if (rc) break;
-        for (int c = 0; c < K; c++) {
-            act_i[c] = active[c];
-            if (active[c]) results[c].krylov_total += iters[c];
+        act_i.assign(Kn, 0);
+        int n_active = 0, kmax = 0;
+        for (int lc = 0; lc < Kn; lc++) {
+            const int c = u.cases[lc];
+            act_i[lc] = active[c];
+            if (active[c]) {
+                results[c].krylov_total += iters[lc];
+                n_active++;
+                kmax = std::max(kmax, iters[lc]);
+            }
         }
-        if (hipMemcpyAsync(d_active, act_i.data(), sizeof(int) * K, hipMemcpyHostToDevice, m->stream) != hipSuccess) {
+        if (hipMemcpyAsync(u.d_active, act_i.data(), sizeof(int) * Kn, hipMemcpyHostToDevice, m->stream) !=
+            hipSuccess) {
             rc = cfd_set_error(ctx, CFD_E_HIP, "hipMemcpyAsync failed");
             break;
         }
-        hipLaunchKernelGGL(k_update_seg, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, N,
-                           o.relaxation, m->order, m->ecase, d_active, m->x, m->d.qa);
+        hipLaunchKernelGGL(k_update_seg, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N,
+                           o.relaxation, m->order, m->ecase, u.d_active, m->x, m->d.qa);
         if (hipStreamSynchronize(m->stream) != hipSuccess) {  // act_i is a host buffer of the copy above
             rc = cfd_set_error(ctx, CFD_E_HIP, "cfd_batch_solve: update failed");
             break;
         }
         m->q_version++;
-        for (int c = 0; c < K; c++)
-            if (active[c]) results[c].iterations = it + 1;
+        for (int lc = 0; lc < Kn; lc++)
+            if (active[u.cases[lc]]) results[u.cases[lc]].iterations = it + 1;
+        if (cfd_progress())
+            fprintf(stderr, "cfd_batch_solve: iteration %d, %d of %d cases active, krylov %d, %.2f s\n", it,
+                    n_active, K, kmax, elapsed());
     }
-    // forces of every case
-    if (!rc) {
-        hipLaunchKernelGGL(k_forces_seg, dim3(m->seg_forces.nblocks), dim3(kBlock), 0, m->stream, m->seg_forces.blk,
-                           m->seg_forces.first, m->seg_forces.off, m->seg_forces.len, m->bnode, m->btype, m->btag,
-                           o.wall_tag, m->d.sa, m->d.qa, p.gamma, m->bpart);
-        hipLaunchKernelGGL(k_reduce_seg<4>, dim3(K), dim3(kBlock), 0, m->stream, m->seg_forces.first, m->bpart,
-                           m->bred);
-        if (hipMemcpyAsync(m->hbred, m->bred, sizeof(double) * kBOut * K, hipMemcpyDeviceToHost, m->stream) !=
-                hipSuccess ||
-            hipStreamSynchronize(m->stream) != hipSuccess)
-            rc = cfd_set_error(ctx, CFD_E_HIP, "cfd_batch_solve: reading forces failed");
+    // every case of the last union: forces, time and state
+    if (!rc && u.m) {
+        std::vector<int> take_l(u.cases.size());
+        for (size_t lc = 0; lc < u.cases.size(); lc++) take_l[lc] = (int)lc;
+        std::fill(keep.begin(), keep.end(), 0);
+        rc = harvest(u, take_l, opts, p, elapsed(), results, q_out, keep, q_keep);
     }
-    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    for (int c = 0; c < K && !rc; c++) {
-        cfd_solve_result &r = results[c];
-        const double *v = m->hbred + (size_t)c * kBOut;
-        const double a = opts[c].alpha_deg * M_PI / 180.0;
-        r.force[0] = v[0];
-        r.force[1] = v[1];
-        r.force[2] = v[2];
-        r.drag = v[0] * cos(a) + v[1] * sin(a);
-        r.lift = -v[0] * sin(a) + v[1] * cos(a);
-        r.pressure_integral = v[3];
-        r.seconds = secs;
-    }
-    // each case's final state, [5][n_c + b_c] with its ghosts after its nodes
-    if (!rc && q_out) {
-        const size_t NT = (size_t)N + B;
-        std::vector<double> u(5 * NT);
-        rc = cfd_field_get(m, CFD_FIELD_Q, u.data());
-        for (int c = 0; c < K && !rc; c++) {
-            if (!q_out[c]) continue;
-            const size_t nc = (size_t)(noff[c + 1] - noff[c]), bc = (size_t)(boff[c + 1] - boff[c]), tc = nc + bc;
-            for (int k = 0; k < 5; k++) {
-                const double *src = u.data() + k * NT;
-                memcpy(q_out[c] + k * tc, src + noff[c], sizeof(double) * nc);
-                memcpy(q_out[c] + k * tc + nc, src + N + boff[c], sizeof(double) * bc);
-            }
-        }
-    }
-    if (device_bytes) *device_bytes = m->bytes;
-    if (rc)
+    if (u.m) cfd_mesh_destroy(u.m);
+    if (rc) {
         for (int c = 0; c < K; c++)
             if (!results[c].status) results[c].status = rc;
-    cfd_mesh_destroy(m);
-    return rc;
+        return rc;
+    }
+    if (done)
+        for (int c : u.cases) done(done_arg, c);
+    return CFD_OK;
 }
 
 extern "C" int cfd_batch_solve(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
                                const cfd_solve_opts *opts, cfd_solve_result *results) {
-    return cfd_batch_solve_into(ctx, K, meshes, params, opts, results, nullptr, nullptr, true);
+    return cfd_batch_solve_into(ctx, K, meshes, params, opts, results, nullptr, nullptr, true, nullptr, nullptr);
 }

@@ -77,9 +77,6 @@ extern "C" const char *cfd_last_error(const cfd_ctx *ctx) {
 // ---------------------------------------------------------------------------
 // mesh creation
 // ---------------------------------------------------------------------------
-// ILU(0) merge map of a BSR pattern: for lower entry kk = (i, k) and the t-th
-// upper block of row k, mqq[moff[kk] + t] is the slot of row i in the same
-// column, or -1.
 // Pattern of a block-SELL layout (BlockSell) over rows 0..n-1: row i holds
 // the BSR slots rb(i) .. re(i)-1 in that order, with block columns ja[slot].
 template <class RB, class RE>
@@ -94,19 +91,23 @@ static void build_block_sell(int n, RB rb, RE re, const std::vector<int> &ja, st
     }
     col.assign(off[ns], 0);
     src.assign(off[ns], -1);
-    for (int i = 0; i < n; i++) {
-        const int l = i % 64;
-        size_t q = (size_t)off[i / 64] + l;
-        for (int kk = rb(i); kk < re(i); kk++, q += 64) {
-            col[q] = ja[kk];
-            src[q] = kk;
+    cfd_parallel_for(n, [&](int64_t i0, int64_t i1) {
+        for (int i = (int)i0; i < (int)i1; i++) {
+            size_t q = (size_t)off[i / 64] + i % 64;
+            for (int kk = rb(i); kk < re(i); kk++, q += 64) {
+                col[q] = ja[kk];
+                src[q] = kk;
+            }
         }
-    }
+    });
 }
 
 static int upload_block_sell(cfd_mesh *m, const std::vector<int> &off, const std::vector<int> &col,
                              const std::vector<int> &src, BlockSell &bs);
 
+// ILU(0) merge map of a BSR pattern: for lower entry kk = (i, k) and the t-th
+// upper block of row k, mqq[moff[kk] + t] is the slot of row i in the same
+// column, or -1.
 static bool build_merge_map(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau, int N,
                             std::vector<int> &moff, std::vector<int> &mqq, int64_t &total) {
     moff.assign(ia[N], 0);
@@ -119,16 +120,18 @@ static bool build_merge_map(const std::vector<int> &ia, const std::vector<int> &
             if (tot > INT32_MAX) return false;
         }
     mqq.assign((size_t)tot, -1);
-    for (int i = 0; i < N; i++)
-        for (int kk = ia[i]; kk < iau[i]; kk++) {
-            const int k = ja[kk];
-            int qq = kk + 1;
-            const int qe = ia[i + 1];
-            for (int pp = iau[k] + 1, t2 = 0; pp < ia[k + 1]; pp++, t2++) {
-                while (qq < qe && ja[qq] < ja[pp]) qq++;
-                if (qq < qe && ja[qq] == ja[pp]) mqq[(size_t)moff[kk] + t2] = qq;
+    cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
+        for (int i = (int)i0; i < (int)i1; i++)
+            for (int kk = ia[i]; kk < iau[i]; kk++) {
+                const int k = ja[kk];
+                int qq = kk + 1;
+                const int qe = ia[i + 1];
+                for (int pp = iau[k] + 1, t2 = 0; pp < ia[k + 1]; pp++, t2++) {
+                    while (qq < qe && ja[qq] < ja[pp]) qq++;
+                    if (qq < qe && ja[qq] == ja[pp]) mqq[(size_t)moff[kk] + t2] = qq;
+                }
             }
-        }
+    });
     total = tot;
     if (mqq.empty()) mqq.push_back(-1);  // keeps the device array non-empty
     return true;
@@ -273,6 +276,13 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
     m->B = B;
     m->NT = N + B;
     int rc = CFD_OK;
+    auto tph = std::chrono::steady_clock::now();
+    auto phase = [&](const char *what) {
+        if (!cfd_progress()) return;
+        const auto t = std::chrono::steady_clock::now();
+        fprintf(stderr, "cfd_mesh_create: %-28s %8.3f s\n", what, std::chrono::duration<double>(t - tph).count());
+        tph = t;
+    };
     try {
         if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
             delete m;
@@ -308,26 +318,47 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
                 }
             };
             std::vector<Key> key(N);
-            for (int i = 0; i < N; i++) {
-                uint64_t c = 0;
-                uint32_t q3[3];
-                const double *lo = &blo[3 * (size_t)(node_case ? node_case[i] : 0)];
-                const double *hi = &bhi[3 * (size_t)(node_case ? node_case[i] : 0)];
-                for (int k = 0; k < 3; k++) {
-                    const double span = hi[k] - lo[k];
-                    const double f = span > 0 ? (d->xyz[3 * (size_t)i + k] - lo[k]) / span : 0.0;
-                    q3[k] = (uint32_t)std::min(2097151.0, std::max(0.0, f * 2097151.0));
+            cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
+                for (int i = (int)i0; i < (int)i1; i++) {
+                    uint64_t c = 0;
+                    uint32_t q3[3];
+                    const double *lo = &blo[3 * (size_t)(node_case ? node_case[i] : 0)];
+                    const double *hi = &bhi[3 * (size_t)(node_case ? node_case[i] : 0)];
+                    for (int k = 0; k < 3; k++) {
+                        const double span = hi[k] - lo[k];
+                        const double f = span > 0 ? (d->xyz[3 * (size_t)i + k] - lo[k]) / span : 0.0;
+                        q3[k] = (uint32_t)std::min(2097151.0, std::max(0.0, f * 2097151.0));
+                    }
+                    for (int bit = 20; bit >= 0; bit--)
+                        for (int k = 0; k < 3; k++) c = (c << 1) | ((q3[k] >> bit) & 1u);
+                    key[i] = {node_case ? node_case[i] : 0, c, i};
                 }
-                for (int bit = 20; bit >= 0; bit--)
-                    for (int k = 0; k < 3; k++) c = (c << 1) | ((q3[k] >> bit) & 1u);
-                key[i] = {node_case ? node_case[i] : 0, c, i};
+            });
+            // the keys order by case first: when the cases occupy ascending
+            // node ranges (unions), each range sorts on its own
+            std::vector<int> cut{0};
+            bool ranges = node_case != nullptr;
+            for (int i = 1; ranges && i < N; i++) {
+                if (node_case[i] < node_case[i - 1]) ranges = false;
+                else if (node_case[i] != node_case[i - 1]) cut.push_back(i);
             }
-            std::sort(key.begin(), key.end());
+            cut.push_back(N);
+            if (ranges && cut.size() > 2) {
+                cfd_parallel_for(
+                    (int64_t)cut.size() - 1,
+                    [&](int64_t a, int64_t b) {
+                        for (int64_t r = a; r < b; r++) std::sort(key.begin() + cut[r], key.begin() + cut[r + 1]);
+                    },
+                    1);
+            } else {
+                std::sort(key.begin(), key.end());
+            }
             for (int t = 0; t < N; t++) {
                 order[t] = key[t].node;
                 iperm[key[t].node] = t;
             }
         }
+        phase("morton order");
         m->h_order = order;
         // node incidence lists (internal numbering) in edge order: interior
         // edges by index, then boundary edges -- the order in which the
@@ -351,6 +382,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             }
             for (int b = 0; b < B; b++) inc[pos[iperm[d->bedge[2 * b]]]++] = make_int2(N + b, E + b);
         }
+        phase("incidence lists");
         // SELL-64 copy of the incidence lists with the area vector of each
         // incidence (see cfd_mesh::nbr); padding entries are never read.
         const int nsl = (N + kSlice - 1) / kSlice;
@@ -371,26 +403,29 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         m->nsell = (size_t)sptr[nsl];
         std::vector<int> nbr(m->nsell, 0);
         std::vector<double> isx(m->nsell, 0.0), isy(m->nsell, 0.0), isz(m->nsell, 0.0);
-        for (int i = 0; i < N; i++) {
-            size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
-            for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) {
-                const int2 en = inc[k];
-                const double *S;
-                if (en.x >= N) {
-                    nbr[idx] = en.x;
-                    S = d->bedge_area + 3 * (size_t)(en.x - N);
-                } else if (en.y >= 0) {
-                    nbr[idx] = en.x;
-                    S = d->edge_area + 3 * (size_t)en.y;
-                } else {
-                    nbr[idx] = ~en.x;
-                    S = d->edge_area + 3 * (size_t)(~en.y);
+        cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
+            for (int i = (int)i0; i < (int)i1; i++) {
+                size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
+                for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) {
+                    const int2 en = inc[k];
+                    const double *S;
+                    if (en.x >= N) {
+                        nbr[idx] = en.x;
+                        S = d->bedge_area + 3 * (size_t)(en.x - N);
+                    } else if (en.y >= 0) {
+                        nbr[idx] = en.x;
+                        S = d->edge_area + 3 * (size_t)en.y;
+                    } else {
+                        nbr[idx] = ~en.x;
+                        S = d->edge_area + 3 * (size_t)(~en.y);
+                    }
+                    isx[idx] = S[0];
+                    isy[idx] = S[1];
+                    isz[idx] = S[2];
                 }
-                isx[idx] = S[0];
-                isy[idx] = S[1];
-                isz[idx] = S[2];
             }
-        }
+        });
+        phase("sell incidences");
         // BSR pattern, Create_CRS_SolverBlockMatrix (Euler3D/src/Solver.cpp:612-675):
         // row i holds i and its neighbours in ascending column order.
         m->h_ia.assign(N + 1, 0);
@@ -398,46 +433,63 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         m->nnz = m->h_ia[N];
         m->h_ja.resize(m->nnz);
         m->h_iau.resize(N);
-        for (int i = 0; i < N; i++) {
-            int k = m->h_ia[i];
-            m->h_ja[k++] = i;
-            for (int q = d->n2n_ia[i]; q < d->n2n_ia[i + 1]; q++) m->h_ja[k++] = d->n2n_ja[q];
-            std::sort(m->h_ja.begin() + m->h_ia[i], m->h_ja.begin() + m->h_ia[i + 1]);
-            for (int q = m->h_ia[i]; q < m->h_ia[i + 1]; q++)
-                if (m->h_ja[q] == i) m->h_iau[i] = q;
-        }
+        cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
+            for (int i = (int)i0; i < (int)i1; i++) {
+                int k = m->h_ia[i];
+                m->h_ja[k++] = i;
+                for (int q = d->n2n_ia[i]; q < d->n2n_ia[i + 1]; q++) m->h_ja[k++] = d->n2n_ja[q];
+                std::sort(m->h_ja.begin() + m->h_ia[i], m->h_ja.begin() + m->h_ia[i + 1]);
+                for (int q = m->h_ia[i]; q < m->h_ia[i + 1]; q++)
+                    if (m->h_ja[q] == i) m->h_iau[i] = q;
+            }
+        });
+        phase("bsr pattern");
         // BSR block of the off-diagonal (row of the node, column of the neighbour)
         std::vector<int> off(inc.size(), -1);
-        for (int ii = 0; ii < N; ii++) {
-            const int i = order[ii];
-            for (int k = ip[ii]; k < ip[ii + 1]; k++) {
-                if (inc[k].x >= N) continue;
-                const int j = order[inc[k].x];
-                auto b0 = m->h_ja.begin() + m->h_ia[i], b1 = m->h_ja.begin() + m->h_ia[i + 1];
-                auto it = std::lower_bound(b0, b1, j);
-                if (it == b1 || *it != j) {
-                    cfd_mesh_destroy(m);
-                    return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: edge (%d,%d) missing from n2n lists", i, j);
+        std::vector<int> missing(N, -1);  // per node: a neighbour missing from its n2n list
+        cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
+            for (int ii = (int)i0; ii < (int)i1; ii++) {
+                const int i = order[ii];
+                for (int k = ip[ii]; k < ip[ii + 1]; k++) {
+                    if (inc[k].x >= N) continue;
+                    const int j = order[inc[k].x];
+                    auto b0 = m->h_ja.begin() + m->h_ia[i], b1 = m->h_ja.begin() + m->h_ia[i + 1];
+                    auto it = std::lower_bound(b0, b1, j);
+                    if (it == b1 || *it != j) {
+                        missing[ii] = j;
+                        break;
+                    }
+                    off[k] = (int)(it - m->h_ja.begin());
                 }
-                off[k] = (int)(it - m->h_ja.begin());
             }
-        }
+        });
+        for (int ii = 0; ii < N; ii++)
+            if (missing[ii] >= 0) {
+                const int i = order[ii], j = missing[ii];
+                cfd_mesh_destroy(m);
+                return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: edge (%d,%d) missing from n2n lists", i, j);
+            }
         std::vector<int> soff(m->nsell, -1);
-        for (int i = 0; i < N; i++) {
-            size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
-            for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) soff[idx] = off[k];
-        }
+        cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
+            for (int i = (int)i0; i < (int)i1; i++) {
+                size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
+                for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) soff[idx] = off[k];
+            }
+        });
+        phase("off-diagonal slots");
         // ILU(0) merge map (see cfd_mesh::ilu_mqq)
         std::vector<int> moff, mqq;
         if (!build_merge_map(m->h_ia, m->h_ja, m->h_iau, N, moff, mqq, m->ilu_nmap)) {
             cfd_mesh_destroy(m);
             return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: ILU(0) merge map too large");
         }
+        phase("merge map");
         std::vector<int> lrows, urows;
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, false, m->lower.ptr, lrows, m->lower.maxwidth);
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, true, m->upper.ptr, urows, m->upper.maxwidth);
         m->lower.nlevels = (int)m->lower.ptr.size() - 1;
         m->upper.nlevels = (int)m->upper.ptr.size() - 1;
+        phase("levels");
         // multicolour ordering for ILU(0): greedy colouring of the node graph in
         // reference order; rows sorted by (colour, index)
         std::vector<int> mperm(N), mia(N + 1, 0), mja(m->nnz), miau(N), msrc(m->nnz), mmoff, mmqq, mlrows, murows;
@@ -463,21 +515,23 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             m->mc_cstart.assign(nc + 1, 0);
             for (int c = 0; c < nc; c++) m->mc_cstart[c + 1] = cnt[c];  // cnt[c] now ends colour c
             for (int t = 0; t < N; t++) miperm[mperm[t]] = t;
-            std::vector<std::pair<int, int>> row;
-            for (int t = 0; t < N; t++) {
-                const int oi = mperm[t];
-                row.clear();
-                for (int k = m->h_ia[oi]; k < m->h_ia[oi + 1]; k++) row.push_back({miperm[m->h_ja[k]], k});
-                std::sort(row.begin(), row.end());
-                int k2 = mia[t];
-                for (auto &e : row) {
-                    if (e.first == t) miau[t] = k2;
-                    mja[k2] = e.first;
-                    msrc[k2] = e.second;
-                    k2++;
+            for (int t = 0; t < N; t++) mia[t + 1] = mia[t] + (m->h_ia[mperm[t] + 1] - m->h_ia[mperm[t]]);
+            cfd_parallel_for(N, [&](int64_t t0, int64_t t1) {
+                std::vector<std::pair<int, int>> row;
+                for (int t = (int)t0; t < (int)t1; t++) {
+                    const int oi = mperm[t];
+                    row.clear();
+                    for (int k = m->h_ia[oi]; k < m->h_ia[oi + 1]; k++) row.push_back({miperm[m->h_ja[k]], k});
+                    std::sort(row.begin(), row.end());
+                    int k2 = mia[t];
+                    for (auto &e : row) {
+                        if (e.first == t) miau[t] = k2;
+                        mja[k2] = e.first;
+                        msrc[k2] = e.second;
+                        k2++;
+                    }
                 }
-                mia[t + 1] = k2;
-            }
+            });
             int64_t tot = 0;
             if (!build_merge_map(mia, mja, miau, N, mmoff, mmqq, tot)) {
                 cfd_mesh_destroy(m);
@@ -488,6 +542,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             m->mc_lower.nlevels = (int)m->mc_lower.ptr.size() - 1;
             m->mc_upper.nlevels = (int)m->mc_upper.ptr.size() - 1;
         }
+        phase("multicolour");
         // neighbour lists (reference order within each list), internal numbering
         std::vector<int> nia(N + 1, 0), nja((size_t)d->n2n_ia[N]);
         for (int ii = 0; ii < N; ii++) {
@@ -501,13 +556,15 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         std::vector<double> qa((size_t)kQA * m->NT, 0.0), sa((size_t)4 * B, 0.0), vol(N);
         double q5[5];
         freestream_q(p, q5);
-        for (int i = 0; i < m->NT; i++) {
-            for (int k = 0; k < 5; k++) qa[(size_t)kQA * i + k] = q5[k];
-            if (i < N) {
-                for (int k = 0; k < 3; k++) qa[(size_t)kQA * i + 5 + k] = d->xyz[3 * (size_t)order[i] + k];
-                vol[i] = d->vol[order[i]];
+        cfd_parallel_for(m->NT, [&](int64_t i0, int64_t i1) {
+            for (int i = (int)i0; i < (int)i1; i++) {
+                for (int k = 0; k < 5; k++) qa[(size_t)kQA * i + k] = q5[k];
+                if (i < N) {
+                    for (int k = 0; k < 3; k++) qa[(size_t)kQA * i + 5 + k] = d->xyz[3 * (size_t)order[i] + k];
+                    vol[i] = d->vol[order[i]];
+                }
             }
-        }
+        });
         for (int b = 0; b < B; b++)
             for (int k = 0; k < 3; k++) sa[4 * (size_t)b + k] = d->bedge_area[3 * (size_t)b + k];
         std::vector<int> bnode(B), btag(B, 0);
@@ -516,6 +573,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             if (d->bedge_tag) btag[b] = d->bedge_tag[b];
         }
 
+        phase("node records");
         m->order = upload(m, order.data(), order.size(), &rc);
         if (!rc) m->inc_ptr = upload(m, ip.data(), ip.size(), &rc);
         if (!rc) m->sptr = upload(m, sptr.data(), sptr.size(), &rc);
@@ -544,6 +602,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         if (!rc) m->mc_src = upload(m, msrc.data(), msrc.size(), &rc);
         if (!rc) m->mc_moff = upload(m, mmoff.data(), mmoff.size(), &rc);
         if (!rc) m->mc_mqq = upload(m, mmqq.data(), mmqq.size(), &rc);
+        phase("uploads");
         if (!rc) {  // block-SELL patterns of A and of the multicolour factors
             std::vector<int> off, col, src;
             const std::vector<int> &hia = m->h_ia, &hiau = m->h_iau;
@@ -561,6 +620,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             }
             (void)hiau;
         }
+        phase("block-sell");
         if (!rc) m->mc_lower.rows = upload(m, mlrows.data(), mlrows.size(), &rc);
         if (!rc) m->mc_upper.rows = upload(m, murows.data(), murows.size(), &rc);
         if (!rc) m->lower.rows = upload(m, lrows.data(), lrows.size(), &rc);
@@ -585,6 +645,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             const char *se = getenv("CFD_SELL");
             m->use_sell = se ? se[0] == '1' : N >= kSellMinRows;
         }
+        phase("levels upload");
         if (rc) {
             cfd_mesh_destroy(m);
             return rc;
@@ -615,6 +676,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         hipMemsetAsync(m->mark, 0, sizeof(int) * (size_t)N, m->stream);
         hipMemsetAsync(m->flag, 0, sizeof(int) * 4, m->stream);
         hipMemsetAsync(m->d.res, 0, sizeof(double) * kRES * (size_t)N, m->stream);
+        phase("workspaces");
         rc = launch_lsq_coef(m);
         if (!rc && hipStreamSynchronize(m->stream) != hipSuccess) rc = cfd_set_error(ctx, CFD_E_HIP, "LSQ setup failed");
         if (rc) {
@@ -625,6 +687,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         cfd_mesh_destroy(m);
         return cfd_set_error(ctx, CFD_E_OOM, "cfd_mesh_create: host out of memory");
     }
+    phase("lsq");
     *out = m;
     return CFD_OK;
 }
@@ -1182,6 +1245,10 @@ extern "C" int cfd_solve_steady(cfd_mesh *m, const cfd_params *p0, const cfd_sol
         rc = cfd_update(m, o->relaxation);
         if (rc) break;
         res->iterations = it + 1;
+        if (cfd_progress())
+            fprintf(stderr, "cfd_solve_steady: iteration %d rms %.3e krylov %d (%d in all) %.2f s\n", it, rms,
+                    st.iterations, res->krylov_total,
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     }
     res->status = rc;
     if (!rc) {

@@ -3,9 +3,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cfd_hip.h"
@@ -89,6 +92,34 @@ struct KrylovGraphs {
 // Rows from which the block-SELL kernels (one row per lane) are used: 2048
 // wavefronts, two per SIMD of the 256 CUs.
 constexpr int kSellMinRows = 131072;
+
+// Runs fn(lo, hi) over contiguous chunks of [0, n), each of at least `grain`
+// items, on up to 16 host threads; used by the host-side mesh set-up, whose
+// loops write disjoint outputs, so results do not depend on the thread count.
+template <class F>
+void cfd_parallel_for(int64_t n, F fn, int64_t grain = 1 << 14) {
+    const int64_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nt = (int)std::max<int64_t>(1, std::min(hw, n / std::max<int64_t>(1, grain)));
+    if (nt <= 1) {
+        fn((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (int t = 1; t < nt; t++) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+    fn((int64_t)0, n / nt);
+    for (auto &x : th) x.join();
+}
+
+// CFD_PROGRESS=1: the Newton loops of cfd_solve_steady and cfd_batch_solve
+// print one line per iteration to stderr (long solves stay observable).
+inline bool cfd_progress() {
+    static const bool on = [] {
+        const char *e = getenv("CFD_PROGRESS");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
 
 struct BlockSell {
     int64_t nblk = 0;     // slots, a multiple of 64
@@ -270,12 +301,17 @@ int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed);
 // batched solves (batch.hip): cases that may share one union mesh, and the
 // solve itself with each case's final state copied to q_out[c] (may be NULL);
 // allow_graphs false launches the Arnoldi steps directly (callers whose other
-// threads use the runtime meanwhile: stream capture is not safe beside them)
+// threads use the runtime meanwhile: stream capture is not safe beside them).
+// done(done_arg, c), when given, is called once per case as soon as its
+// result and state are final (finished cases are harvested whenever the union
+// is compacted, the rest at the end); on an error return the cases not yet
+// reported carry the error status and are not reported.
 bool cfd_same_scheme(const cfd_params &a, const cfd_params &b);
 bool cfd_same_solve(const cfd_solve_opts &a, const cfd_solve_opts &b);
+typedef void (*cfd_case_done_fn)(void *arg, int c);
 int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, const cfd_params *params,
                          const cfd_solve_opts *opts, cfd_solve_result *results, double *const *q_out,
-                         int64_t *device_bytes, bool allow_graphs);
+                         int64_t *device_bytes, bool allow_graphs, cfd_case_done_fn done, void *done_arg);
 int launch_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, int add_time, int rhs_mode,
                        int mixed);
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);

@@ -40,7 +40,7 @@ struct Pending {
     double t_submit;
     int64_t need;    // bytes to reserve if no resident mesh can be reused
     int bypassed;    // later requests admitted ahead of this one
-    bool batchable;  // FGMRES solve on a mesh in the batched layout
+    bool batchable;  // FGMRES solve on a small mesh in the batched layout
 };
 
 struct Resident {
@@ -205,13 +205,50 @@ bool admit(cfd_server *s, Job &job, double &retry_at) {
     return false;
 }
 
-// Solves a batch as one union mesh; one completion per request.
+// A batch in flight: its requests' results and which have been posted.
+struct BatchRun {
+    cfd_server *s;
+    const Job *job;
+    std::vector<cfd_solve_result> res;
+    std::vector<char> posted;
+    double t_admit;
+    int64_t bytes = 0;
+};
+
+cfd_completion batch_completion(const BatchRun &b, int c, int status) {
+    cfd_completion o{};
+    o.id = b.job->reqs[c].req.id;
+    o.status = status;
+    o.batch_size = (int32_t)b.job->reqs.size();
+    o.result = b.res[c];
+    o.t_submit = b.job->reqs[c].t_submit;
+    o.t_admit = b.t_admit;
+    o.t_done = b.s->now();
+    o.footprint = b.job->reserve;
+    o.device_bytes = b.bytes;
+    return o;
+}
+
+// Called by the batch solve when a request's result is final: posts its
+// completion at once, while the rest of the batch keeps iterating.
+void post_batch_case(void *arg, int c) {
+    BatchRun &b = *(BatchRun *)arg;
+    cfd_completion o = batch_completion(b, c, b.res[c].status);
+    if (o.status) snprintf(o.message, sizeof(o.message), "solve failed");
+    std::lock_guard<std::mutex> lk(b.s->mu);
+    b.s->done.push_back(o);
+    b.posted[c] = 1;
+    b.s->cv_done.notify_all();
+}
+
+// Solves a batch as one union mesh; each request's completion is posted as
+// soon as its case is final.  Returns the completions not yet posted (after
+// an error: every request not already reported).
 std::vector<cfd_completion> run_batch(cfd_server *s, const Job &job) {
     const int K = (int)job.reqs.size();
     std::vector<cfd_mesh_desc> descs(K);
     std::vector<cfd_params> params(K);
     std::vector<cfd_solve_opts> opts(K);
-    std::vector<cfd_solve_result> res(K);
     std::vector<double *> qo(K);
     for (int c = 0; c < K; c++) {
         descs[c] = *job.reqs[c].req.mesh;
@@ -219,27 +256,18 @@ std::vector<cfd_completion> run_batch(cfd_server *s, const Job &job) {
         opts[c] = job.reqs[c].req.opts;
         qo[c] = job.reqs[c].req.q_out;
     }
-    const double t_admit = s->now();
-    int64_t bytes = 0;
+    BatchRun b{s, &job, std::vector<cfd_solve_result>(K), std::vector<char>(K, 0), s->now()};
     // direct launches, as for served meshes: other workers use the runtime
     // while this batch runs, and stream capture is not safe beside them
-    const int rc = cfd_batch_solve_into(s->ctx, K, descs.data(), params.data(), opts.data(), res.data(), qo.data(),
-                                        &bytes, false);
-    const double t_done = s->now();
-    std::vector<cfd_completion> out(K);
+    const int rc = cfd_batch_solve_into(s->ctx, K, descs.data(), params.data(), opts.data(), b.res.data(), qo.data(),
+                                        &b.bytes, false, post_batch_case, &b);
+    std::vector<cfd_completion> out;
+    std::lock_guard<std::mutex> lk(s->mu);  // posted[] is written under the lock
     for (int c = 0; c < K; c++) {
-        cfd_completion &o = out[c];
-        o = cfd_completion{};
-        o.id = job.reqs[c].req.id;
-        o.status = rc ? rc : res[c].status;
-        o.batch_size = K;
-        o.result = res[c];
-        o.t_submit = job.reqs[c].t_submit;
-        o.t_admit = t_admit;
-        o.t_done = t_done;
-        o.footprint = job.reserve;
-        o.device_bytes = bytes;
+        if (b.posted[c]) continue;
+        cfd_completion o = batch_completion(b, c, rc ? rc : b.res[c].status);
         if (o.status) snprintf(o.message, sizeof(o.message), "%s", rc ? cfd_last_error(s->ctx) : "solve failed");
+        out.push_back(o);
     }
     return out;
 }
@@ -352,7 +380,8 @@ extern "C" int cfd_server_submit(cfd_server *s, const cfd_request *r) {
                              (long long)need, (long long)s->opts.hbm_budget);
     // batchable: FGMRES, and ghosts numbered after the nodes in boundary-edge
     // order (the layout the union mesh concatenates)
-    bool batchable = r->opts.lin.solver == CFD_SOLVER_FGMRES;
+    const int32_t case_cap = s->opts.batch_case_nodes > 0 ? s->opts.batch_case_nodes : kSellMinRows;
+    bool batchable = r->opts.lin.solver == CFD_SOLVER_FGMRES && d->n_node <= case_cap;
     for (int32_t b = 0; batchable && b < d->n_bedge; b++) batchable = d->bedge[2 * b + 1] == d->n_node + b;
     std::lock_guard<std::mutex> g(s->mu);
     s->queue.push_back(Pending{*r, s->now(), need, 0, batchable});

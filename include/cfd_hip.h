@@ -293,7 +293,8 @@ typedef struct cfd_server_opts {
     int32_t batch_nodes;    /* nodes per batch at most (<= 0: 2,000,000) */
     int32_t batch_wait_us;  /* while other solves run, a batchable request with fewer than batch_max
                                companions queued waits up to this long for more (0: never waits) */
-    int32_t pad_;
+    int32_t batch_case_nodes; /* only requests on meshes of at most this many nodes are batched
+                                 (<= 0: 131,072); a larger mesh fills the GPU on its own */
 } cfd_server_opts;
 
 typedef struct cfd_request {

@@ -205,21 +205,44 @@ def test_server_results_match_alone(ctx, alone_runs):
         assert c["t_submit"] <= c["t_admit"] <= c["t_done"]
 
 
-def test_server_batches_match_alone(ctx, alone_runs):
+class _env:
+    """Sets environment variables for the duration of a with block (read by the library at each call)."""
+
+    def __init__(self, **kv):
+        self.kv, self.old = kv, {}
+
+    def __enter__(self):
+        for k, v in self.kv.items():
+            self.old[k] = os.environ.get(k)
+            os.environ[k] = v
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("compact", ["0", "1"])
+def test_server_batches_match_alone(ctx, alone_runs, compact):
     """Continuous batching: with one worker busy on the first request, the queued compatible requests are solved
-    together as union meshes, and every result equals the same request solved alone, bit for bit."""
+    together as union meshes, and every result equals the same request solved alone, bit for bit -- also when
+    each batch is compacted to its unfinished cases as soon as they hold half of its nodes or less
+    (CFD_BATCH_COMPACT=1), the finished ones being posted at that point."""
     meshes, alone = alone_runs
-    srv = cfd.Server(ctx, hbm_budget=8 << 30, max_inflight=1, batch_max=4)
-    for rid, (k, d, flux, order, mach, alpha) in enumerate(REQS):
-        p = cfd.make_params(mach=mach, alpha_deg=alpha, flux=flux, order=order)
-        srv.submit(rid, k, meshes[k], p, want_state=True, alpha_deg=alpha, **SOLVE)
-    done = []
-    while len(done) < len(REQS):
-        got = srv.poll(wait=60.0)
-        assert got, "server made no progress in 60 s"
-        done += got
-    st = srv.stats()
-    srv.close()
+    with _env(CFD_BATCH_COMPACT=compact):
+        srv = cfd.Server(ctx, hbm_budget=8 << 30, max_inflight=1, batch_max=4)
+        for rid, (k, d, flux, order, mach, alpha) in enumerate(REQS):
+            p = cfd.make_params(mach=mach, alpha_deg=alpha, flux=flux, order=order)
+            srv.submit(rid, k, meshes[k], p, want_state=True, alpha_deg=alpha, **SOLVE)
+        done = []
+        while len(done) < len(REQS):
+            got = srv.poll(wait=60.0)
+            assert got, "server made no progress in 60 s"
+            done += got
+        st = srv.stats()
+        srv.close()
     assert max(c["batch_size"] for c in done) >= 2
     assert st["admitted"] == len(REQS) and st["peak_bytes"] <= 8 << 30
     for c in done:
@@ -287,6 +310,35 @@ def test_batch_solve_matches_alone(ctx, precond):
         assert r1.first_rms == r0.first_rms and r1.final_rms == r0.final_rms
         assert list(r1.force) == list(r0.force)
         assert r1.lift == r0.lift and r1.drag == r0.drag
+
+
+def test_batch_solve_compaction_matches_alone(ctx):
+    """CFD_BATCH_COMPACT=1: whenever the cases still iterating hold half of the union's nodes or less, they move to
+    a new union of their own.  Every case still reproduces its alone solve bit for bit; the finished cases are
+    harvested at the compaction, so their reported times are shorter than the batch's."""
+    cases = [((20, 6, 6), 0.08, 0.35, 0.0), ((16, 6, 4), 0.12, 0.45, 2.0), ((14, 5, 5), 0.1, 0.5, -1.0),
+             ((12, 4, 4), 0.1, 0.75, 3.0), ((12, 4, 4), 0.06, 0.7, 1.0)]
+    meshes = [cmesh.bump_channel(*d, bump_h=h) for d, h, _, _ in cases]
+    params = [cfd.make_params(mach=mach, alpha_deg=a, flux="roe", order=1) for _, _, mach, a in cases]
+    kw = dict(SOLVE)
+    kw["precond"] = cfd.PREC_ILU0_MC
+    alone = []
+    for m, p, (_, _, _, a) in zip(meshes, params, cases):
+        dm = cfd.DeviceMesh(ctx, m, p)
+        r, _ = dm.solve_steady(p, alpha_deg=a, **kw)
+        alone.append(r)
+        dm.close()
+    with _env(CFD_BATCH_COMPACT="1"):
+        rc, batch = cfd.batch_solve(ctx, meshes, params, alphas=[a for *_, a in cases], **kw)
+    assert rc == 0, lib_error(ctx)
+    for r0, r1 in zip(alone, batch):
+        assert r1.converged and r1.status == 0
+        assert (r1.iterations, r1.krylov_total) == (r0.iterations, r0.krylov_total)
+        assert r1.first_rms == r0.first_rms and r1.final_rms == r0.final_rms
+        assert list(r1.force) == list(r0.force)
+        assert r1.lift == r0.lift and r1.drag == r0.drag
+    assert len({r.iterations for r in alone}) > 1
+    assert min(r.seconds for r in batch) < max(r.seconds for r in batch), "no compaction took place"
 
 
 def lib_error(ctx):
