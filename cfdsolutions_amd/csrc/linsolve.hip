@@ -757,13 +757,31 @@ __global__ __launch_bounds__(kBlock) void k_trsv_mc(int nrows, const int *__rest
         const int kb = backward ? iau[i] + 1 : ia[i];
         const int ke = backward ? ia[i + 1] : iau[i];
         y = backward ? w[5 * (size_t)i + r] : rhs[5 * (size_t)perm[i] + r];
-        for (int kk = kb; kk < ke; kk++) {
-            const TM *A = lu + 25 * (size_t)kk + 5 * r;
-            const TV *wj = w + 5 * (size_t)ja[kk];
-            TV s = TV(0.0);
+        // four blocks at a time: their loads are all issued before the
+        // products are subtracted, in block order as one at a time
+        for (int k0 = kb; k0 < ke; k0 += 4) {
+            int jj[4];
 #pragma unroll
-            for (int c = 0; c < 5; c++) s += (TV)A[c] * wj[c];
-            y = y - s;
+            for (int u = 0; u < 4; u++) jj[u] = k0 + u < ke ? ja[k0 + u] : -1;
+            TM av[4][5];
+            TV wv[4][5];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (jj[u] >= 0) {
+#pragma unroll
+                    for (int c = 0; c < 5; c++) {
+                        av[u][c] = lu[25 * (size_t)(k0 + u) + 5 * r + c];
+                        wv[u][c] = w[5 * (size_t)jj[u] + c];
+                    }
+                }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (jj[u] >= 0) {
+                    TV s = TV(0.0);
+#pragma unroll
+                    for (int c = 0; c < 5; c++) s += (TV)av[u][c] * wv[u][c];
+                    y = y - s;
+                }
         }
     }
     if (!backward) {
@@ -857,11 +875,29 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int N, const int *__restrict__ 
     if (t >= 5 * N) return;
     const int i = t / 5, r = t - 5 * i;
     TY s = TY(0.0);
-    for (int kk = ia[i]; kk < ia[i + 1]; kk++) {
-        const TA *A = a + 25 * (size_t)kk + 5 * r;
-        const TX *xj = x + 5 * (size_t)ja[kk];
+    // four blocks at a time (loads first, then the products in order)
+    const int ke = ia[i + 1];
+    for (int k0 = ia[i]; k0 < ke; k0 += 4) {
+        int jj[4];
 #pragma unroll
-        for (int c = 0; c < 5; c++) s += (TY)A[c] * (TY)xj[c];
+        for (int u = 0; u < 4; u++) jj[u] = k0 + u < ke ? ja[k0 + u] : -1;
+        TA av[4][5];
+        TX xv[4][5];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (jj[u] >= 0) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) {
+                    av[u][c] = a[25 * (size_t)(k0 + u) + 5 * r + c];
+                    xv[u][c] = x[5 * (size_t)jj[u] + c];
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (jj[u] >= 0) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) s += (TY)av[u][c] * (TY)xv[u][c];
+            }
     }
     if (bsub) s = (TY)bsub[t] - s;  // residual form r = b - A x
     y[t] = s;
