@@ -412,10 +412,11 @@ def main():
     ap.add_argument("--no-solves", action="store_true", help="skip the steady-solve and serving legs")
     ap.add_argument("--legs", type=str, default="residual,solve,serve,batch",
                     help="comma list of legs to run: residual, solve, serve, batch, solve3 (192x64x64 solve)")
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-meshes", type=int, default=6)
     ap.add_argument("--serve-nodes", type=str, default="10000,50000", help="min,max nodes of the serving meshes")
-    ap.add_argument("--serve-inflight", type=int, default=12)
+    ap.add_argument("--serve-inflight", type=int, default=4,
+                    help="server worker threads; 4 matches the HIP runtime's 4 hardware queues per process")
     ap.add_argument("--serve-batch", type=int, default=8, help="requests per union-mesh batch in the server")
     ap.add_argument("--batch-cases", type=int, default=32,
                     help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")

@@ -101,15 +101,24 @@ bool make_room(cfd_server *s, int64_t need) {
 }
 
 // Tries to start a batch led by queue entry q: the compatible requests of the
-// look-ahead window, up to batch_max requests and batch_nodes nodes, when
+// look-ahead window, up to batch_target requests and batch_nodes nodes, when
 // there are at least two and their union mesh fits the budget.  Called with
 // the server lock held.
+// Requests per batch while requests outnumber the idle workers: enough for
+// the queue to fit the idle workers, at most batch_max (1: run alone).
+int batch_target(const cfd_server *s) {
+    const int64_t idle = std::max(1, s->opts.max_inflight - s->running);
+    const int64_t want = ((int64_t)s->queue.size() + idle - 1) / idle;
+    return (int)std::min<int64_t>(want, s->opts.batch_max);
+}
+
 bool admit_batch(cfd_server *s, size_t q, size_t look, Job &job) {
     const Pending &p = s->queue[q];
     const int64_t cap = s->opts.batch_nodes > 0 ? s->opts.batch_nodes : 2000000;
     std::vector<size_t> take{q};
     int64_t nn = p.req.mesh->n_node, ne = p.req.mesh->n_edge, nb = p.req.mesh->n_bedge;
-    for (size_t t = q + 1; t < look && (int)take.size() < s->opts.batch_max; t++) {
+    const int target = batch_target(s);
+    for (size_t t = q + 1; t < look && (int)take.size() < target; t++) {
         const Pending &o = s->queue[t];
         if (!o.batchable || !cfd_same_scheme(p.req.params, o.req.params) || !cfd_same_solve(p.req.opts, o.req.opts))
             continue;
@@ -163,12 +172,13 @@ bool admit(cfd_server *s, Job &job, double &retry_at) {
     retry_at = 0.0;
     for (size_t q = 0; q < look; q++) {
         Pending &p = s->queue[q];
-        if (s->opts.batch_max >= 2 && p.batchable) {
+        const int target = batch_target(s);
+        if (s->opts.batch_max >= 2 && p.batchable && target >= 2) {
             // batching delay: while the GPU is busy with other solves, a short
             // batch waits for companions (up to batch_wait_us after arrival)
             const double until = p.t_submit + 1e-6 * s->opts.batch_wait_us;
             if (s->running > 0 && s->opts.batch_wait_us > 0 && p.bypassed < s->opts.max_bypass && s->now() < until &&
-                companions(s, q, look) < s->opts.batch_max) {
+                companions(s, q, look) < target) {
                 retry_at = retry_at > 0.0 ? std::min(retry_at, until) : until;
                 continue;
             }

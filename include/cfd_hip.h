@@ -274,13 +274,16 @@ int cfd_batch_solve(cfd_ctx *ctx, int32_t ncase, const cfd_mesh_desc *meshes, co
  * one GPU: requests are admitted in arrival order (with bounded look-ahead)
  * when a worker is free and their HBM footprint fits the budget; device
  * meshes stay resident after a solve and serve later requests with the same
- * mesh_key.  With batch_max >= 2 the server batches continuously: a worker
- * that admits a request also takes the compatible requests queued in its
- * look-ahead window (same scheme and solver options, FGMRES) and solves them
- * together as one union mesh (as cfd_batch_solve); each batched result is
- * bit-identical to the same request solved alone, and requests that arrive
- * meanwhile are admitted by the other workers.  Results do not depend on
- * concurrency (deterministic kernels). */
+ * mesh_key.  With batch_max >= 2 the server batches continuously: while
+ * queued requests outnumber the idle workers, a worker that admits a request
+ * also takes compatible requests queued in its look-ahead window (same scheme
+ * and solver options, FGMRES, meshes of at most batch_case_nodes nodes), as
+ * many as needed for the queue to fit the idle workers and at most batch_max,
+ * and solves them together as one union mesh (as cfd_batch_solve).  Each
+ * batched result is bit-identical to the same request solved alone and is
+ * posted as soon as it is final; requests that arrive meanwhile are admitted
+ * by the other workers.  Results do not depend on concurrency (deterministic
+ * kernels). */
 typedef struct cfd_server cfd_server;
 
 typedef struct cfd_server_opts {
@@ -291,8 +294,8 @@ typedef struct cfd_server_opts {
     int64_t hbm_budget;     /* device bytes the server may hold (meshes, workspaces) */
     int32_t batch_max;      /* >= 2: up to this many requests per union-mesh batch; else one per solve */
     int32_t batch_nodes;    /* nodes per batch at most (<= 0: 2,000,000) */
-    int32_t batch_wait_us;  /* while other solves run, a batchable request with fewer than batch_max
-                               companions queued waits up to this long for more (0: never waits) */
+    int32_t batch_wait_us;  /* while other solves run, a batchable request with fewer companions queued
+                               than its batch would take waits up to this long for more (0: never waits) */
     int32_t batch_case_nodes; /* only requests on meshes of at most this many nodes are batched
                                  (<= 0: 131,072); a larger mesh fills the GPU on its own */
 } cfd_server_opts;
