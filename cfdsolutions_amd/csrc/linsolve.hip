@@ -1092,7 +1092,27 @@ __device__ __forceinline__ void block_dots(size_t o, size_t i0, size_t n, size_t
         double acc[8];
 #pragma unroll
         for (int kk = 0; kk < 8; kk++) acc[kk] = 0.0;
-        for (size_t i = i0; i < n; i += stride) {
+        // four strides at a time: all their loads are issued before the
+        // products are added, in the same order as one stride at a time
+        size_t i = i0;
+        for (; i + 3 * stride < n; i += 4 * stride) {
+            TV wv[4], vv[4][8];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                wv[u] = w[o + i + u * stride];
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++)
+                    if (k0 + kk < nv) vv[u][kk] = V[ld * (k0 + kk) + o + i + u * stride];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const double wi = (double)wv[u];
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++)
+                    if (k0 + kk < nv) acc[kk] += (double)vv[u][kk] * wi;
+            }
+        }
+        for (; i < n; i += stride) {
             const double wi = (double)w[o + i];
 #pragma unroll
             for (int kk = 0; kk < 8; kk++)
