@@ -1083,7 +1083,7 @@ __global__ __launch_bounds__(kBlock) void k_lusgs_init(int N, const double *b, d
 // products are accumulated in register chunks of 8 vectors (w is re-read per
 // chunk): each (thread, vector) sum visits the same elements in the same order
 // and each vector goes through the same block tree, whatever the chunking.
-template <class TV>
+template <class TV, bool DEEP = false>
 __device__ __forceinline__ void block_dots(size_t o, size_t i0, size_t n, size_t stride, int nv,
                                            const TV *__restrict__ V, size_t ld, const TV *__restrict__ w,
                                            double *__restrict__ part) {
@@ -1092,10 +1092,12 @@ __device__ __forceinline__ void block_dots(size_t o, size_t i0, size_t n, size_t
         double acc[8];
 #pragma unroll
         for (int kk = 0; kk < 8; kk++) acc[kk] = 0.0;
-        // four strides at a time: all their loads are issued before the
-        // products are added, in the same order as one stride at a time
+        // DEEP: four strides at a time, all their loads issued before the
+        // products are added in the same order as one stride at a time (for
+        // threads with many strides to walk; it costs registers, so short
+        // walks keep the single-stride loop and its occupancy)
         size_t i = i0;
-        for (; i + 3 * stride < n; i += 4 * stride) {
+        for (; DEEP && i + 3 * stride < n; i += 4 * stride) {
             TV wv[4], vv[4][8];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -1134,10 +1136,10 @@ __device__ __forceinline__ void block_dots(size_t o, size_t i0, size_t n, size_t
     }
 }
 
-template <class TV>
+template <class TV, bool DEEP>
 __global__ __launch_bounds__(kBlock) void k_mdot(size_t n, int nv, const TV *__restrict__ V, size_t ld,
                                                  const TV *__restrict__ w, double *__restrict__ part) {
-    block_dots<TV>(0, blockIdx.x * (size_t)kBlock + threadIdx.x, n, (size_t)gridDim.x * kBlock, nv, V, ld, w,
+    block_dots<TV, DEEP>(0, blockIdx.x * (size_t)kBlock + threadIdx.x, n, (size_t)gridDim.x * kBlock, nv, V, ld, w,
                    part + (size_t)blockIdx.x * (kMaxKrylov + 1));
 }
 
@@ -1506,7 +1508,10 @@ template <class TV>
 void mdot(cfd_mesh *m, int nv, const TV *V, size_t ld, const TV *w, int off) {
     const size_t L = 5 * (size_t)m->N;
     const int np = grid_cap(L, kDotParts);
-    hipLaunchKernelGGL(k_mdot<TV>, dim3(np), dim3(kBlock), 0, m->stream, L, nv, V, ld, w, m->red);
+    if (L >= 4 * (size_t)np * kBlock)  // each thread walks four strides or more
+        hipLaunchKernelGGL((k_mdot<TV, true>), dim3(np), dim3(kBlock), 0, m->stream, L, nv, V, ld, w, m->red);
+    else
+        hipLaunchKernelGGL((k_mdot<TV, false>), dim3(np), dim3(kBlock), 0, m->stream, L, nv, V, ld, w, m->red);
     hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(64), 0, m->stream, np, nv, m->red, m->redout + off);
 }
 
