@@ -142,18 +142,91 @@ def time_variant(dm, ev, p, steps, warmup, second_order=True, fused=True):
     return wall, g / steps, l / steps, r / steps
 
 
-def cpu_baseline(m, p, q):
+def cpu_baseline(m, p, q, build="parity"):
+    """One G+L+R2 Roe evaluation by the oracle on one core: build "parity" (-O2 -ffp-contract=off, the checker) or
+    "speed" (the same source at -O3 -march=x86-64-v3)."""
     from oracle import oracle as orc
 
-    om = orc.Mesh(m.xyz, m.edge, m.edge_area, m.bedge, m.bedge_area, m.bedge_type, m.vol, m.n2n_ia, m.n2n_ja,
-                  m.bedge_tag)
-    op = orc.params({"rho": p.inf_rho, "u": p.inf_u, "v": p.inf_v, "w": p.inf_w, "p": p.inf_p,
-                     "outflow_p": p.outflow_p}, flux=orc.ROE, order=2, limiter=orc.LIM_VENKAT, venkat_k=p.venkat_k)
-    t0 = time.perf_counter()
-    g = orc.gg_gradient(om, q)
-    phi = orc.limiter(om, op, q, g)
-    rc, rd, _, _ = orc.residual(om, op, q, g, phi)
-    return time.perf_counter() - t0, rc, rd
+    old = orc.use_build(build)
+    try:
+        om = orc.Mesh(m.xyz, m.edge, m.edge_area, m.bedge, m.bedge_area, m.bedge_type, m.vol, m.n2n_ia, m.n2n_ja,
+                      m.bedge_tag)
+        op = orc.params({"rho": p.inf_rho, "u": p.inf_u, "v": p.inf_v, "w": p.inf_w, "p": p.inf_p,
+                         "outflow_p": p.outflow_p}, flux=orc.ROE, order=2, limiter=orc.LIM_VENKAT,
+                        venkat_k=p.venkat_k)
+        t0 = time.perf_counter()
+        g = orc.gg_gradient(om, q)
+        phi = orc.limiter(om, op, q, g)
+        rc, rd, _, _ = orc.residual(om, op, q, g, phi)
+        return time.perf_counter() - t0, rc, rd
+    finally:
+        orc.use_build(old)
+
+
+def cpu_solve(m, p, build="speed"):
+    """The 48x16x16 steady solve by the oracle on one core (same algorithm as the GPU FP64 path: backward Euler,
+    approximate Roe Jacobian, FGMRES(30) + natural-order ILU(0))."""
+    from oracle import oracle as orc
+
+    old = orc.use_build(build)
+    try:
+        om = orc.Mesh(m.xyz, m.edge, m.edge_area, m.bedge, m.bedge_area, m.bedge_type, m.vol, m.n2n_ia, m.n2n_ja,
+                      m.bedge_tag)
+        op = orc.params({"rho": p.inf_rho, "u": p.inf_u, "v": p.inf_v, "w": p.inf_w, "p": p.inf_p,
+                         "outflow_p": p.outflow_p}, flux=orc.ROE, order=1, limiter=orc.LIM_NONE,
+                        venkat_k=p.venkat_k, cfl_min=p.cfl_min, cfl_max=p.cfl_max, cfl_ramp=p.cfl_ramp)
+        et = p.inf_p / ((p.gamma - 1) * p.inf_rho) + 0.5 * (p.inf_u ** 2 + p.inf_v ** 2 + p.inf_w ** 2)
+        q = np.zeros((5, om.n_total))
+        q[:] = np.array([p.inf_rho, p.inf_rho * p.inf_u, p.inf_rho * p.inf_v, p.inf_rho * p.inf_w,
+                         p.inf_rho * et])[:, None]
+        t0 = time.perf_counter()
+        _, r, _, _, _ = orc.solve_steady(om, op, q, m.bedge_tag, max_iter=SOLVE_KW["max_iter"], tol=SOLVE_KW["tol"],
+                                         krylov_m=SOLVE_KW["krylov_m"], max_restarts=SOLVE_KW["max_restarts"],
+                                         krylov_rtol=SOLVE_KW["rtol"], precond=2, gradient=orc.GRAD_LSQ)
+        return time.perf_counter() - t0, r
+    finally:
+        orc.use_build(old)
+
+
+def measured_copy_gbs(mib=1024, reps=20):
+    """Device-to-device copy bandwidth of one MIB MiB buffer, (read + write bytes) / time, through the HIP runtime
+    (ctypes), timed with HIP events around `reps` copies."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+    hip.hipFree.argtypes = [vp]
+    hip.hipMemset.argtypes = [vp, ctypes.c_int, sz]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    n = mib << 20
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+        return None
+    try:
+        hip.hipMemset(a, 0, n)
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        for _ in range(3):
+            hip.hipMemcpyAsync(b, a, n, 3, None)  # hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, n, 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        return 2.0 * n * reps / (ms.value * 1e-3) / 1e9
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
 
 
 def measured_traffic(kernel):
@@ -179,7 +252,7 @@ def bump_dims(n_nodes):
 SOLVE_KW = dict(max_iter=400, tol=1e-10, krylov_m=30, max_restarts=4, rtol=1e-3)
 
 
-def solve_leg(ctx):
+def solve_leg(ctx, cpu=True):
     """Bump channel 48x16x16 (14,161 nodes), first-order Roe, M 0.5, AoA 0, backward Euler, CFL 1 -> 1000,
     FGMRES(30) + ILU(0), to RMS 1e-10: plain FP64 vs mixed precision, each alone on the GPU."""
     from cfdsolutions_amd import cfd, mesh
@@ -209,6 +282,18 @@ def solve_leg(ctx):
         print("bench: solve 48x16x16 %-22s %.3f s (%d newton / %d krylov), force err %.2e" % (
             name, r.seconds, r.iterations, r.krylov_total, res[name]["force_rel_err_vs_fp64"]),
             file=sys.stderr, flush=True)
+    if cpu:
+        secs, rc = cpu_solve(m, p)
+        res["cpu_oracle"] = {"seconds": secs, "solves_per_s": 1.0 / secs, "cores": 1, "kind": "port",
+                             "build": "oracle.c -O3 -march=x86-64-v3", "newton": rc.iterations,
+                             "krylov": rc.krylov_total, "converged": bool(rc.converged),
+                             "force_rel_err_vs_gpu_fp64": float(np.max(np.abs(np.array(rc.force[:]) -
+                                                                              np.array(r0.force))) /
+                                                                np.max(np.abs(r0.force))),
+                             "gpu_fp64_speedup": secs / r0.seconds,
+                             "gpu_mixed_multicolour_speedup": secs / out["mixed_multicolour_ilu"][0].seconds}
+        print("bench: solve 48x16x16 cpu oracle (1 core)        %.3f s (%d newton / %d krylov)" % (
+            secs, rc.iterations, rc.krylov_total), file=sys.stderr, flush=True)
     return res
 
 
@@ -410,6 +495,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--only", action="store_true", help="run only the headline variant (for profiling)")
     ap.add_argument("--no-solves", action="store_true", help="skip the steady-solve and serving legs")
+    ap.add_argument("--no-cpu-solve", action="store_true", help="skip the oracle's CPU timing of the 48x16x16 solve")
     ap.add_argument("--legs", type=str, default="residual,solve,serve,batch",
                     help="comma list of legs to run: residual, solve, serve, batch, solve3 (192x64x64 solve)")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -451,7 +537,7 @@ def main():
         ctx = cfd.Context(0)
         out = {"metric": "converged solves/s & p99 latency (mixed vs FP64 GPU); residual-eval GB/s", "legs": sorted(legs)}
         if "solve" in legs:
-            out["solves"] = solve_leg(ctx)
+            out["solves"] = solve_leg(ctx, cpu=not args.no_cpu_solve)
         if "solve3" in legs:
             out["solve_192x64x64"] = solve3_leg(ctx)
         if "serve" in legs:
@@ -513,16 +599,22 @@ def main():
         gc, gd = dm.get(cfd.FIELD_RES_CONV), dm.get(cfd.FIELD_RES_DISS)
         secs, rc, rd = cpu_baseline(m, pv, q)
         rel = max(np.max(np.abs(gc - rc)) / np.max(np.abs(rc)), np.max(np.abs(gd - rd)) / np.max(np.abs(rd)))
+        secs_f, rc, rd = cpu_baseline(m, pv, q, build="speed")
+        rel_f = max(np.max(np.abs(gc - rc)) / np.max(np.abs(rc)), np.max(np.abs(gd - rd)) / np.max(np.abs(rd)))
         cpu = dict(value=head["bytes"] / secs / 1e9, unit="GB/s", cores=1, kind="port",
                    sample="one full G+L+R2 Roe FP64 evaluation of the same 160^3 mesh and state (%.1f s)" % secs,
-                   parity_max_rel_diff=float(rel))
-        print("bench: cpu oracle %.2f s, GPU-vs-oracle max rel diff %.3e" % (secs, rel), file=sys.stderr, flush=True)
+                   build="oracle.c -O2 -ffp-contract=off (parity build)",
+                   parity_max_rel_diff=float(rel),
+                   speed_build=dict(value=head["bytes"] / secs_f / 1e9, seconds=secs_f,
+                                    build="oracle.c -O3 -march=x86-64-v3", max_rel_diff_vs_gpu=float(rel_f)))
+        print("bench: cpu oracle %.2f s (speed build %.2f s), GPU-vs-oracle max rel diff %.3e (speed build %.1e)" % (
+            secs, secs_f, rel, rel_f), file=sys.stderr, flush=True)
     dm.close()
     solves = serving = batched = None
     if not args.no_solves and not args.only:
         del dm
         if "solve" in legs:
-            solves = solve_leg(ctx)
+            solves = solve_leg(ctx, cpu=not args.no_cpu_solve)
         if "serve" in legs:
             nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
             serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight,
@@ -531,6 +623,9 @@ def main():
             batched = batch_leg(ctx, args.batch_cases, args.batch_fp64_sample)
     ctx.close()
     r_achieved = variants["roe-fp64-GL+R2"]["r_gbs"]
+    copy_gbs = measured_copy_gbs()
+    if copy_gbs:
+        print("bench: device-to-device copy of 1 GiB: %.0f GB/s" % copy_gbs, file=sys.stderr, flush=True)
     traffic_bytes = measured_traffic("k_residual") if args.n == 160 else None
     out = {
         "metric": "converged solves/s & p99 latency (mixed vs FP64 GPU); residual-eval GB/s",
@@ -555,6 +650,8 @@ def main():
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/residual_cube160_fp64_hbm_counters.md"
                      if traffic_bytes else None,
                      "algorithmic_bytes_per_launch": bytes64["R2split"],
+                     "measured_copy_peak": round(copy_gbs, 1) if copy_gbs else None,
+                     "frac_of_measured_copy": round(r_achieved / copy_gbs, 4) if copy_gbs else None,
                      "avg_launch_ms": round(variants["roe-fp64-GL+R2"]["r_ms"], 4)},
         "cpu_baseline": cpu,
         "solves": solves,

@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
+_BUILD = "parity"  # "parity": -O2 -ffp-contract=off (the checker); "speed": -O3 -march=x86-64-v3 (CPU timings)
 
 ROE, HLLC, AUSMP = 0, 1, 2
 LIM_NONE, LIM_VENKAT, LIM_BJ = 0, 1, 2
@@ -52,14 +53,23 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE, "oracle"], check=True)
 
 
+def use_build(kind):
+    """Selects the library the functions below call: "parity" (the checker, bit-exact build) or "speed" (the same
+    source built for speed, used only to time the CPU path).  Returns the previous choice."""
+    global _BUILD
+    if kind not in ("parity", "speed"):
+        raise ValueError(kind)
+    old, _BUILD = _BUILD, kind
+    return old
+
+
 def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+    if _BUILD not in _LIBS:
+        path = os.path.join(_HERE, "liboracle.so" if _BUILD == "parity" else "liboracle_speed.so")
         if not os.path.exists(path):
             build()
-        _LIB = ctypes.CDLL(path)
-    return _LIB
+        _LIBS[_BUILD] = ctypes.CDLL(path)
+    return _LIBS[_BUILD]
 
 
 def _p(a):
