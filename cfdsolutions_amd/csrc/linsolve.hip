@@ -421,6 +421,9 @@ __global__ __launch_bounds__(128) void k_jacobian_fd(FdArgs a) {
 // Dense 5x5 helpers (same operation order as the CPU oracle)
 // ---------------------------------------------------------------------------
 // Gauss-Jordan inverse with partial pivoting; returns false on a zero pivot.
+// Every loop is unrolled and the pivot-row exchange is a select per entry, so
+// the working matrix stays in registers (a dynamically indexed row swap put it
+// in scratch memory); the arithmetic is the textbook sequence, unchanged.
 __device__ bool inv5(const double *A, double *Ai) {
     double a[5][10];
 #pragma unroll
@@ -430,22 +433,38 @@ __device__ bool inv5(const double *A, double *Ai) {
             a[r][c] = A[5 * r + c];
             a[r][c + 5] = (r == c) ? 1.0 : 0.0;
         }
+#pragma unroll
     for (int c = 0; c < 5; c++) {
+        // first row of largest magnitude in column c (ties keep the earlier row)
         int piv = c;
-        for (int r = c + 1; r < 5; r++)
-            if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
-        if (piv != c)
-            for (int j = 0; j < 10; j++) {
-                double t = a[c][j];
-                a[c][j] = a[piv][j];
-                a[piv][j] = t;
+        double best = fabs(a[c][c]);
+#pragma unroll
+        for (int r = c + 1; r < 5; r++) {
+            const double v = fabs(a[r][c]);
+            if (v > best) {
+                piv = r;
+                best = v;
             }
+        }
+#pragma unroll
+        for (int r = c + 1; r < 5; r++) {
+            const bool sw = piv == r;
+#pragma unroll
+            for (int j = 0; j < 10; j++) {
+                const double t = a[c][j];
+                a[c][j] = sw ? a[r][j] : t;
+                a[r][j] = sw ? t : a[r][j];
+            }
+        }
         const double d = a[c][c];
         if (d == 0.0) return false;
+#pragma unroll
         for (int j = 0; j < 10; j++) a[c][j] = a[c][j] / d;
+#pragma unroll
         for (int r = 0; r < 5; r++) {
             if (r == c) continue;
             const double f = a[r][c];
+#pragma unroll
             for (int j = 0; j < 10; j++) a[r][j] = a[r][j] - f * a[c][j];
         }
     }
@@ -510,19 +529,37 @@ __global__ __launch_bounds__(kBlock) void k_ilu0_level(int nrows, const int *__r
             for (int c = 0; c < 5; c++) Lk[c] = Lr[c];
             // upper blocks of row k that meet a block of row i, in ascending
             // column order (the precomputed merge of the two rows)
+            // CH upper blocks at a time (two in FP32; FP64 has no registers to
+            // spare): their slots, blocks and targets are loaded before any
+            // target is updated.  The targets are distinct blocks of row i, so
+            // each update is the one-at-a-time arithmetic.
+            constexpr int CH = sizeof(TM) == 4 ? 2 : 1;
             const int p0 = iau[k] + 1, np = ia[k + 1] - p0, m0 = moff[kk];
-#pragma unroll 2
-            for (int t2 = 0; t2 < np; t2++) {
-                const int qq = mqq[m0 + t2];
-                if (qq < 0) continue;
-                const TM *Up = luk + 25 * (size_t)(p0 + t2);
-                TM *T = lu + 25 * (size_t)qq + 5 * r;
+            for (int t2 = 0; t2 < np; t2 += CH) {
+                int qs[CH];
+                TM up[CH][25], tv[CH][5];
 #pragma unroll
-                for (int c = 0; c < 5; c++) {
-                    TM s = TM(0.0);
+                for (int u = 0; u < CH; u++) qs[u] = t2 + u < np ? mqq[m0 + t2 + u] : -1;
 #pragma unroll
-                    for (int q = 0; q < 5; q++) s += Lr[q] * Up[5 * q + c];
-                    T[c] = T[c] - s;
+                for (int u = 0; u < CH; u++)
+                    if (qs[u] >= 0) {
+                        const TM *Up = luk + 25 * (size_t)(p0 + t2 + u);
+#pragma unroll
+                        for (int e = 0; e < 25; e++) up[u][e] = Up[e];
+#pragma unroll
+                        for (int c = 0; c < 5; c++) tv[u][c] = lu[25 * (size_t)qs[u] + 5 * r + c];
+                    }
+#pragma unroll
+                for (int u = 0; u < CH; u++) {
+                    if (qs[u] < 0) continue;
+                    TM *T = lu + 25 * (size_t)qs[u] + 5 * r;
+#pragma unroll
+                    for (int c = 0; c < 5; c++) {
+                        TM s = TM(0.0);
+#pragma unroll
+                        for (int q = 0; q < 5; q++) s += Lr[q] * up[u][5 * q + c];
+                        T[c] = tv[u][c] - s;
+                    }
                 }
             }
         }
@@ -542,8 +579,15 @@ __global__ __launch_bounds__(kBlock) void k_ilu0_level(int nrows, const int *__r
         if (r == 0) atomicAdd(flag + 2, 1);
         for (int q = 0; q < 25; q++) Di[q] = 0.0;
     }
+    // row r of the inverse, picked with selects (an index by r would put Di in
+    // scratch memory)
+    double dr[5] = {Di[0], Di[1], Di[2], Di[3], Di[4]};
 #pragma unroll
-    for (int c = 0; c < 5; c++) dinv[25 * (size_t)i + 5 * r + c] = (TM)Di[5 * r + c];
+    for (int rr = 1; rr < 5; rr++)
+#pragma unroll
+        for (int c = 0; c < 5; c++) dr[c] = r == rr ? Di[5 * rr + c] : dr[c];
+#pragma unroll
+    for (int c = 0; c < 5; c++) dinv[25 * (size_t)i + 5 * r + c] = (TM)dr[c];
 }
 
 // Inverse diagonal blocks (block-Jacobi).
