@@ -250,7 +250,7 @@ int build_union(cfd_ctx *ctx, const std::vector<int> &cases, const cfd_mesh_desc
         nb[c] = std::min(grid_for(nc), 1024);  // launch_rms
         eoffv[c] = 5 * noff[c];
         elen[c] = 5 * nc;
-        nbd[c] = (int)std::max<size_t>(1, std::min<size_t>((5 * (size_t)nc + kBlock - 1) / kBlock, kDotParts));  // mdot
+        nbd[c] = dot_parts(5 * (size_t)nc);  // mdot
         bo[c] = boff[c];
         blen[c] = bc;
         nbf[c] = bc ? std::min(grid_for(bc), 256) : 1;  // launch_forces

@@ -59,7 +59,14 @@ struct LevelSchedule {
 };
 
 constexpr int kMaxKrylov = 64;   // largest FGMRES restart length
-constexpr int kDotParts = 512;   // partial sums of one Krylov dot product
+constexpr int kDotParts = 512;   // partial sums of one Krylov dot product, at most
+// Blocks of 256 threads for a Krylov dot product over n entries: one entry per
+// thread up to kDotParts blocks, grid strides beyond.  Batched solves give each
+// case the blocks its solve alone would use, so both take them from here.
+inline int dot_parts(size_t n) {
+    const size_t g = (n + 256 - 1) / 256;
+    return (int)(g < 1 ? 1 : (g < (size_t)kDotParts ? g : kDotParts));
+}
 constexpr int kBOut = 2 * (kMaxKrylov + 1) + 1;  // per-case results of one batched Arnoldi step
 
 // Block map of a segmented reduction over the cases of a union mesh: case c

@@ -1131,7 +1131,8 @@ template <class TV, bool DEEP = false>
 __device__ __forceinline__ void block_dots(size_t o, size_t i0, size_t n, size_t stride, int nv,
                                            const TV *__restrict__ V, size_t ld, const TV *__restrict__ w,
                                            double *__restrict__ part) {
-    __shared__ double sh[8][kBlock];
+    __shared__ double wsum[8][kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int k0 = 0; k0 < nv; k0 += 8) {
         double acc[8];
 #pragma unroll
@@ -1164,18 +1165,20 @@ __device__ __forceinline__ void block_dots(size_t o, size_t i0, size_t n, size_t
             for (int kk = 0; kk < 8; kk++)
                 if (k0 + kk < nv) acc[kk] += (double)V[ld * (k0 + kk) + o + i] * wi;
         }
+        // block sum: a butterfly within each wavefront (every lane ends with
+        // the same value), then the four wavefront sums in order
 #pragma unroll
-        for (int kk = 0; kk < 8; kk++) sh[kk][threadIdx.x] = acc[kk];
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) {
+        for (int kk = 0; kk < 8; kk++) {
+            if (k0 + kk >= nv) break;
+            double v = acc[kk];
 #pragma unroll
-                for (int kk = 0; kk < 8; kk++) sh[kk][threadIdx.x] += sh[kk][threadIdx.x + s];
-            }
-            __syncthreads();
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) wsum[kk][wv] = v;
         }
+        __syncthreads();
         if (threadIdx.x == 0)
-            for (int kk = 0; kk < 8 && k0 + kk < nv; kk++) part[k0 + kk] = sh[kk][0];
+            for (int kk = 0; kk < 8 && k0 + kk < nv; kk++)
+                part[k0 + kk] = ((wsum[kk][0] + wsum[kk][1]) + wsum[kk][2]) + wsum[kk][3];
         __syncthreads();
     }
 }
@@ -1551,7 +1554,7 @@ void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r
 template <class TV>
 void mdot(cfd_mesh *m, int nv, const TV *V, size_t ld, const TV *w, int off) {
     const size_t L = 5 * (size_t)m->N;
-    const int np = grid_cap(L, kDotParts);
+    const int np = dot_parts(L);
     if (L >= 4 * (size_t)np * kBlock)  // each thread walks four strides or more
         hipLaunchKernelGGL((k_mdot<TV, true>), dim3(np), dim3(kBlock), 0, m->stream, L, nv, V, ld, w, m->red);
     else
