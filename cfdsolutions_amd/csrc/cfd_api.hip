@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -77,6 +78,20 @@ extern "C" const char *cfd_last_error(const cfd_ctx *ctx) {
 // ---------------------------------------------------------------------------
 // mesh creation
 // ---------------------------------------------------------------------------
+// Host array without value initialisation, for large set-up arrays that a
+// parallel pass writes in full.
+template <class T>
+struct HostArray {
+    std::unique_ptr<T[]> p;
+    size_t n;
+    explicit HostArray(size_t count) : p(new T[count ? count : 1]), n(count) {}
+    T *data() { return p.get(); }
+    const T *data() const { return p.get(); }
+    size_t size() const { return n; }
+    T &operator[](size_t i) { return p[i]; }
+    const T &operator[](size_t i) const { return p[i]; }
+};
+
 // Pattern of a block-SELL layout (BlockSell) over rows 0..n-1: row i holds
 // the BSR slots rb(i) .. re(i)-1 in that order, with block columns ja[slot].
 template <class RB, class RE>
@@ -363,24 +378,63 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         // node incidence lists (internal numbering) in edge order: interior
         // edges by index, then boundary edges -- the order in which the
         // reference's sequential loops deliver each node's contributions
+        // (a node's interior edges are its n2n neighbours, counted once each)
         std::vector<int> ip(N + 1, 0);
-        for (int e = 0; e < E; e++) {
-            ip[iperm[d->edge[2 * e]] + 1]++;
-            ip[iperm[d->edge[2 * e + 1]] + 1]++;
-        }
+        cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
+            for (int i = (int)i0; i < (int)i1; i++) ip[iperm[i] + 1] = d->n2n_ia[i + 1] - d->n2n_ia[i];
+        });
         for (int b = 0; b < B; b++) ip[iperm[d->bedge[2 * b]] + 1]++;
         for (int i = 0; i < N; i++) ip[i + 1] += ip[i];
         std::vector<int2> inc(ip[N]);
         {
-            std::vector<int> pos(ip.begin(), ip.end() - 1);
-            for (int e = 0; e < E; e++) {
-                // .y = e for the L end of the edge, ~e for the R end (roles follow
-                // the reference numbering, L < R, not the internal one)
-                const int l = iperm[d->edge[2 * e]], r = iperm[d->edge[2 * e + 1]];
-                inc[pos[l]++] = make_int2(r, e);
-                inc[pos[r]++] = make_int2(l, ~e);
+            // Edges and boundary edges in order, each appended to the lists of
+            // its nodes.  A union's cases hold ascending ranges of nodes, edges
+            // and boundary edges with no edge between cases, so each case's
+            // ranges are filled on their own (in parallel) in the same order.
+            std::vector<int> pos(ip.begin(), ip.end() - 1), ecut{0}, bcut{0};
+            bool ranges = node_case != nullptr;
+            auto ecase = [&](int e) { return node_case[d->edge[2 * e]]; };
+            auto bcase = [&](int b) { return node_case[d->bedge[2 * b]]; };
+            for (int e = 0; ranges && e < E; e++) {
+                if (node_case[d->edge[2 * e + 1]] != ecase(e) || (e && ecase(e) < ecase(e - 1))) ranges = false;
+                else if (e && ecase(e) != ecase(e - 1)) ecut.push_back(e);
             }
-            for (int b = 0; b < B; b++) inc[pos[iperm[d->bedge[2 * b]]]++] = make_int2(N + b, E + b);
+            for (int b = 1; ranges && b < B; b++) {
+                if (bcase(b) < bcase(b - 1)) ranges = false;
+                else if (bcase(b) != bcase(b - 1)) bcut.push_back(b);
+            }
+            // piece r: edges [ecut[r], ecut[r+1]) and boundary edges [bcut[r], bcut[r+1]) of one case
+            const int ncs = node_case ? node_case[N - 1] + 1 : 1;
+            if (ranges && E && B && (int)ecut.size() == ncs && (int)bcut.size() == ncs && ecase(0) == 0 &&
+                bcase(0) == 0) {
+                ecut.push_back(E);
+                bcut.push_back(B);
+            } else {
+                ecut.assign({0, E});
+                bcut.assign({0, B});
+            }
+            cfd_parallel_for(
+                (int64_t)ecut.size() - 1,
+                [&](int64_t r0, int64_t r1) {
+                    for (int64_t r = r0; r < r1; r++) {
+                        for (int e = ecut[r]; e < ecut[r + 1]; e++) {
+                            // .y = e for the L end of the edge, ~e for the R end (roles follow
+                            // the reference numbering, L < R, not the internal one)
+                            const int l = iperm[d->edge[2 * e]], rr = iperm[d->edge[2 * e + 1]];
+                            inc[pos[l]++] = make_int2(rr, e);
+                            inc[pos[rr]++] = make_int2(l, ~e);
+                        }
+                        for (int b = bcut[r]; b < bcut[r + 1]; b++)
+                            inc[pos[iperm[d->bedge[2 * b]]]++] = make_int2(N + b, E + b);
+                    }
+                },
+                1);
+            for (int i = 0; i < N; i++)
+                if (pos[i] != ip[i + 1]) {
+                    cfd_mesh_destroy(m);
+                    return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: n2n lists do not match the edges at node %d",
+                                         order[i]);
+                }
         }
         phase("incidence lists");
         // SELL-64 copy of the incidence lists with the area vector of each
@@ -394,37 +448,52 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
                 for (int i = sl * kSlice; i < std::min(N, (sl + 1) * kSlice); i++) w = std::max(w, ip[i + 1] - ip[i]);
                 tot += (int64_t)kSlice * w;
                 if (tot > INT32_MAX) {
-                    delete m;
+                    cfd_mesh_destroy(m);
                     return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: mesh too large for 32-bit incidence offsets");
                 }
                 sptr[sl + 1] = (int)tot;
             }
         }
         m->nsell = (size_t)sptr[nsl];
-        std::vector<int> nbr(m->nsell, 0);
-        std::vector<double> isx(m->nsell, 0.0), isy(m->nsell, 0.0), isz(m->nsell, 0.0);
-        cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
-            for (int i = (int)i0; i < (int)i1; i++) {
-                size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
-                for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) {
-                    const int2 en = inc[k];
-                    const double *S;
-                    if (en.x >= N) {
-                        nbr[idx] = en.x;
-                        S = d->bedge_area + 3 * (size_t)(en.x - N);
-                    } else if (en.y >= 0) {
-                        nbr[idx] = en.x;
-                        S = d->edge_area + 3 * (size_t)en.y;
-                    } else {
-                        nbr[idx] = ~en.x;
-                        S = d->edge_area + 3 * (size_t)(~en.y);
+        // uninitialised storage: the per-slice pass below writes every entry
+        // once (padding included), in parallel, instead of a serial zero fill
+        HostArray<int> nbr(m->nsell);
+        HostArray<double> isx(m->nsell), isy(m->nsell), isz(m->nsell);
+        cfd_parallel_for(
+            nsl,
+            [&](int64_t s0, int64_t s1) {
+                for (int sl = (int)s0; sl < (int)s1; sl++) {
+                    const int w = (sptr[sl + 1] - sptr[sl]) / kSlice;
+                    for (int l = 0; l < kSlice; l++) {
+                        const int i = sl * kSlice + l;
+                        const int cnt = i < N ? ip[i + 1] - ip[i] : 0;
+                        for (int k = 0; k < w; k++) {
+                            const size_t idx = (size_t)sptr[sl] + (size_t)k * kSlice + l;
+                            if (k >= cnt) {  // padding, never read
+                                nbr[idx] = 0;
+                                isx[idx] = isy[idx] = isz[idx] = 0.0;
+                                continue;
+                            }
+                            const int2 en = inc[ip[i] + k];
+                            const double *S;
+                            if (en.x >= N) {
+                                nbr[idx] = en.x;
+                                S = d->bedge_area + 3 * (size_t)(en.x - N);
+                            } else if (en.y >= 0) {
+                                nbr[idx] = en.x;
+                                S = d->edge_area + 3 * (size_t)en.y;
+                            } else {
+                                nbr[idx] = ~en.x;
+                                S = d->edge_area + 3 * (size_t)(~en.y);
+                            }
+                            isx[idx] = S[0];
+                            isy[idx] = S[1];
+                            isz[idx] = S[2];
+                        }
                     }
-                    isx[idx] = S[0];
-                    isy[idx] = S[1];
-                    isz[idx] = S[2];
                 }
-            }
-        });
+            },
+            64);
         phase("sell incidences");
         // BSR pattern, Create_CRS_SolverBlockMatrix (Euler3D/src/Solver.cpp:612-675):
         // row i holds i and its neighbours in ascending column order.
@@ -469,13 +538,21 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
                 cfd_mesh_destroy(m);
                 return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: edge (%d,%d) missing from n2n lists", i, j);
             }
-        std::vector<int> soff(m->nsell, -1);
-        cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
-            for (int i = (int)i0; i < (int)i1; i++) {
-                size_t idx = (size_t)sptr[i / kSlice] + (i % kSlice);
-                for (int k = ip[i]; k < ip[i + 1]; k++, idx += kSlice) soff[idx] = off[k];
-            }
-        });
+        HostArray<int> soff(m->nsell);
+        cfd_parallel_for(
+            nsl,
+            [&](int64_t s0, int64_t s1) {
+                for (int sl = (int)s0; sl < (int)s1; sl++) {
+                    const int w = (sptr[sl + 1] - sptr[sl]) / kSlice;
+                    for (int l = 0; l < kSlice; l++) {
+                        const int i = sl * kSlice + l;
+                        const int cnt = i < N ? ip[i + 1] - ip[i] : 0;
+                        for (int k = 0; k < w; k++)
+                            soff[(size_t)sptr[sl] + (size_t)k * kSlice + l] = k < cnt ? off[ip[i] + k] : -1;
+                    }
+                }
+            },
+            64);
         phase("off-diagonal slots");
         // ILU(0) merge map (see cfd_mesh::ilu_mqq)
         std::vector<int> moff, mqq;
@@ -494,19 +571,47 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         // reference order; rows sorted by (colour, index)
         std::vector<int> mperm(N), mia(N + 1, 0), mja(m->nnz), miau(N), msrc(m->nnz), mmoff, mmqq, mlrows, murows;
         {
-            std::vector<int> colour(N, -1), mark;
-            int nc = 0;
-            for (int i = 0; i < N; i++) {
-                mark.assign(nc + 1, 0);
-                for (int q = d->n2n_ia[i]; q < d->n2n_ia[i + 1]; q++) {
-                    const int cj = colour[d->n2n_ja[q]];
-                    if (cj >= 0) mark[cj] = 1;
-                }
-                int c = 0;
-                while (mark[c]) c++;
-                colour[i] = c;
-                nc = std::max(nc, c + 1);
+            // Greedy colouring in reference order.  The cases of a union hold
+            // ascending node ranges with no edge between them, so each range
+            // is coloured on its own (in parallel), exactly as in one pass.
+            std::vector<int> colour(N, -1), cut{0};
+            bool ranges = node_case != nullptr;
+            for (int i = 1; ranges && i < N; i++) {
+                if (node_case[i] < node_case[i - 1]) ranges = false;
+                else if (node_case[i] != node_case[i - 1]) cut.push_back(i);
             }
+            if (!ranges) cut.assign(1, 0);
+            cut.push_back(N);
+            for (int r = 0; r + 1 < (int)cut.size() && cut.size() > 2; r++)  // no edge may leave its range
+                for (int q = d->n2n_ia[cut[r]]; q < d->n2n_ia[cut[r + 1]]; q++)
+                    if (d->n2n_ja[q] < cut[r] || d->n2n_ja[q] >= cut[r + 1]) {
+                        cut.assign({0, N});
+                        break;
+                    }
+            std::vector<int> ncs(cut.size() - 1, 0);
+            cfd_parallel_for(
+                (int64_t)cut.size() - 1,
+                [&](int64_t r0, int64_t r1) {
+                    std::vector<int> mark;
+                    for (int64_t r = r0; r < r1; r++) {
+                        int nc = 0;
+                        for (int i = cut[r]; i < cut[r + 1]; i++) {
+                            mark.assign(nc + 1, 0);
+                            for (int q = d->n2n_ia[i]; q < d->n2n_ia[i + 1]; q++) {
+                                const int cj = colour[d->n2n_ja[q]];
+                                if (cj >= 0) mark[cj] = 1;
+                            }
+                            int c = 0;
+                            while (mark[c]) c++;
+                            colour[i] = c;
+                            nc = std::max(nc, c + 1);
+                        }
+                        ncs[r] = nc;
+                    }
+                },
+                1);
+            int nc = 1;
+            for (int v : ncs) nc = std::max(nc, v);
             m->ncolor = nc;
             std::vector<int> cnt(nc + 1, 0), miperm(N);
             for (int i = 0; i < N; i++) cnt[colour[i] + 1]++;
@@ -553,16 +658,16 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         }
 
         // packed node records: q (freestream), coordinates
-        std::vector<double> qa((size_t)kQA * m->NT, 0.0), sa((size_t)4 * B, 0.0), vol(N);
+        HostArray<double> qa((size_t)kQA * m->NT);
+        std::vector<double> sa((size_t)4 * B, 0.0), vol(N);
         double q5[5];
         freestream_q(p, q5);
         cfd_parallel_for(m->NT, [&](int64_t i0, int64_t i1) {
             for (int i = (int)i0; i < (int)i1; i++) {
                 for (int k = 0; k < 5; k++) qa[(size_t)kQA * i + k] = q5[k];
-                if (i < N) {
-                    for (int k = 0; k < 3; k++) qa[(size_t)kQA * i + 5 + k] = d->xyz[3 * (size_t)order[i] + k];
-                    vol[i] = d->vol[order[i]];
-                }
+                for (int k = 0; k < 3; k++)  // ghost records hold no coordinates
+                    qa[(size_t)kQA * i + 5 + k] = i < N ? d->xyz[3 * (size_t)order[i] + k] : 0.0;
+                if (i < N) vol[i] = d->vol[order[i]];
             }
         });
         for (int b = 0; b < B; b++)
