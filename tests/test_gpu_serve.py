@@ -255,6 +255,33 @@ def test_server_batches_match_alone(ctx, alone_runs, compact):
         assert 0 < c["device_bytes"] <= c["footprint"], (c["device_bytes"], c["footprint"])
 
 
+def test_server_batches_only_when_requests_outnumber_workers(ctx, alone_runs):
+    """Adaptive batching: with as many workers as requests every request runs alone; with one worker the queued
+    requests are batched, except those on meshes above batch_case_nodes.  Results stay bit-identical."""
+    meshes, alone = alone_runs
+    big_key = max(meshes, key=lambda k: meshes[k].n_node)
+    for workers, case_nodes in ((len(REQS), 0), (1, meshes[big_key].n_node - 1)):
+        srv = cfd.Server(ctx, hbm_budget=8 << 30, max_inflight=workers, batch_max=4, batch_case_nodes=case_nodes)
+        for rid, (k, d, flux, order, mach, alpha) in enumerate(REQS):
+            p = cfd.make_params(mach=mach, alpha_deg=alpha, flux=flux, order=order)
+            srv.submit(rid, k, meshes[k], p, want_state=True, alpha_deg=alpha, **SOLVE)
+        done = []
+        while len(done) < len(REQS):
+            got = srv.poll(wait=60.0)
+            assert got, "server made no progress in 60 s"
+            done += got
+        srv.close()
+        for c in done:
+            it, kry, force, q = alone[c["id"]]
+            assert c["status"] == 0, c["message"]
+            assert (c["result"].iterations, c["result"].krylov_total) == (it, kry)
+            assert np.array_equal(c["q"], q), "request %d differs from its alone run" % c["id"]
+            if workers == len(REQS) or REQS[c["id"]][0] == big_key:
+                assert c["batch_size"] == 1, (workers, c["id"], c["batch_size"])
+        if workers == 1:
+            assert max(c["batch_size"] for c in done) >= 2
+
+
 def test_server_admission_respects_budget(ctx):
     m = cmesh.bump_channel(16, 6, 6)
     need = cfd.footprint_bytes(m.n_node, m.n_edge, m.n_bedge, krylov_m=20, mixed=True)
