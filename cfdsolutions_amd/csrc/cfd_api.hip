@@ -710,7 +710,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         phase("uploads");
         if (!rc) {  // block-SELL patterns of A and of the multicolour factors
             std::vector<int> off, col, src;
-            const std::vector<int> &hia = m->h_ia, &hiau = m->h_iau;
+            const std::vector<int> &hia = m->h_ia;
             build_block_sell(N, [&](int i) { return hia[i]; }, [&](int i) { return hia[i + 1]; }, m->h_ja, off, col,
                              src);
             rc = upload_block_sell(m, off, col, src, m->a_sell);
@@ -723,7 +723,6 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
                                  col, src);
                 rc = upload_block_sell(m, off, col, src, m->mcu_sell);
             }
-            (void)hiau;
         }
         phase("block-sell");
         if (!rc) m->mc_lower.rows = upload(m, mlrows.data(), mlrows.size(), &rc);
