@@ -896,75 +896,6 @@ __global__ __launch_bounds__(kBlock) void k_trsv_mc_sell(int r0, int r1, const i
     }
 }
 
-// k_trsv_mc_sell with two lanes per row: the lane of half h of a pair
-// computes the row's components 2q + h (q = 0, 1, 2; four lanes' worth of
-// loads fewer per block each), and the backward pass trades the partial
-// vectors within the pair before applying D^-1.  Each component's sum runs
-// over the same blocks and columns in the same order, so the bits are those
-// of k_trsv_mc_sell.
-template <class TM, class TV>
-__global__ __launch_bounds__(kBlock) void k_trsv_mc_sell2(int r0, int r1, const int *__restrict__ ia,
-                                                          const int *__restrict__ iau, const int *__restrict__ perm,
-                                                          const int *__restrict__ off, const int *__restrict__ col,
-                                                          const TM *__restrict__ val, const TM *__restrict__ dv,
-                                                          const TV *__restrict__ rhs, TV *w, TV *z, int backward) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int i = (r0 & ~63) + (t >> 1), h = t & 1;
-    if (i < r0 || i >= r1) return;  // both lanes of a pair hold the same row
-    const int l = i & 63;
-    const int len = backward ? ia[i + 1] - iau[i] - 1 : iau[i] - ia[i];
-    TV y[3];
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const int r = 2 * q + h;
-        y[q] = TV(0.0);
-        if (r < 5) y[q] = backward ? w[5 * (size_t)i + r] : rhs[5 * (size_t)perm[i] + r];
-    }
-    size_t p = (size_t)off[i >> 6] + l;
-    for (int k = 0; k < len; k++, p += 64) {
-        const TV *wj = w + 5 * (size_t)col[p];
-        TV wv[5];
-#pragma unroll
-        for (int c = 0; c < 5; c++) wv[c] = wj[c];
-        const TM *A = val + 25 * (p - l) + l;
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int r = 2 * q + h;
-            if (r >= 5) continue;
-            TV s = TV(0.0);
-#pragma unroll
-            for (int c = 0; c < 5; c++) s += (TV)A[64 * (5 * r + c)] * wv[c];
-            y[q] = y[q] - s;
-        }
-    }
-    if (!backward) {
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-            if (2 * q + h < 5) w[5 * (size_t)i + 2 * q + h] = y[q];
-        return;
-    }
-    // component c is this lane's y[c / 2] when c % 2 == h, else its partner's
-    TV ya[5];
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const TV other = __shfl_xor(y[q], 1, 64);
-#pragma unroll
-        for (int e = 0; e < 2; e++)
-            if (2 * q + e < 5) ya[2 * q + e] = e == h ? y[q] : other;
-    }
-    const TM *D = dv + 25 * (size_t)(i - l) + l;
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const int r = 2 * q + h;
-        if (r >= 5) continue;
-        TV s = TV(0.0);
-#pragma unroll
-        for (int c = 0; c < 5; c++) s += (TV)D[64 * (5 * r + c)] * ya[c];
-        w[5 * (size_t)i + r] = s;
-        z[5 * (size_t)perm[i] + r] = s;
-    }
-}
-
 // z = D^-1 r (block-Jacobi apply)
 template <class TM, class TV>
 __global__ __launch_bounds__(kBlock) void k_bjac_apply(int N, const TM *dinv, const TV *rhs, TV *z) {
@@ -1586,19 +1517,12 @@ void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r
         for (int pass = 0; pass < 2; pass++) {
             const BlockSell &bs = pass ? m->mcu_sell : m->mcl_sell;
             const TM *val = pass ? SellVals<TM>::u(m) : SellVals<TM>::l(m);
-            // CFD_SWEEP_LANES=1|2: lanes per row of the colour sweeps
-            static const char *const lanes_env = getenv("CFD_SWEEP_LANES");
-            const bool two = lanes_env && lanes_env[0] == '2';
             for (int t = 0; t < m->ncolor; t++) {
                 const int c = pass ? m->ncolor - 1 - t : t;
                 const int r0 = m->mc_cstart[c], r1 = m->mc_cstart[c + 1];
-                const int g = ((two ? 2 : 1) * (r1 - (r0 & ~63)) + kBlock - 1) / kBlock;
-                if (two)
-                    hipLaunchKernelGGL((k_trsv_mc_sell2<TM, TV>), dim3(g), dim3(kBlock), 0, m->stream, r0, r1,
-                                       m->mc_ia, m->mc_iau, m->mc_perm, bs.off, bs.col, val, dv, r, wk, z, pass);
-                else
-                    hipLaunchKernelGGL((k_trsv_mc_sell<TM, TV>), dim3(g), dim3(kBlock), 0, m->stream, r0, r1,
-                                       m->mc_ia, m->mc_iau, m->mc_perm, bs.off, bs.col, val, dv, r, wk, z, pass);
+                const int g = (r1 - (r0 & ~63) + kBlock - 1) / kBlock;
+                hipLaunchKernelGGL((k_trsv_mc_sell<TM, TV>), dim3(g), dim3(kBlock), 0, m->stream, r0, r1, m->mc_ia,
+                                   m->mc_iau, m->mc_perm, bs.off, bs.col, val, dv, r, wk, z, pass);
             }
         }
     } else if (kind == CFD_PREC_ILU0 && m->sync_free) {
