@@ -328,13 +328,13 @@ def solve3_leg(ctx):
 
 
 def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, batch_max=8, seed=20261202, mach=(0.3, 0.85),
-              aoa=(-2.0, 6.0), fluxes=("roe", "hllc"), stream=True):
+              aoa=(-2.0, 6.0), fluxes=("roe", "hllc"), stream=True, batch_wait_us=50000):
     """Steady-solve requests on bump channels of log-uniform size in [nmin, nmax] nodes, bump height U(0.05,
     0.15), Mach U(0.3, 0.85), AoA U(-2, 6) deg, Roe or HLLC (first order), to RMS 1e-10, FGMRES(30) with the
     multicolour ILU(0).  The plain FP64 GPU path solves them one at a time (meshes resident).  The server solves
     them in mixed precision, first with one request per solve (meshes resident, `inflight` at once) as a burst,
     then with continuous batching (up to batch_max compatible requests per union-mesh solve, a request waiting
-    at most 50 ms for companions while the GPU is busy) as a burst (capacity) and as a Poisson stream at 80% of
+    at most batch_wait_us for companions while the GPU is busy) as a burst (capacity) and as a Poisson stream at 80% of
     that capacity (latency)."""
     from cfdsolutions_amd import cfd, mesh
 
@@ -400,7 +400,8 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, batch_max=8, seed=2026
     print("bench: serve without batching %.2f solves/s" % cap1, file=sys.stderr, flush=True)
 
     # continuous batching
-    srv = cfd.Server(ctx, hbm_budget=200 << 30, max_inflight=inflight, batch_max=batch_max, batch_wait_us=50000)
+    srv = cfd.Server(ctx, hbm_budget=200 << 30, max_inflight=inflight, batch_max=batch_max,
+                     batch_wait_us=batch_wait_us)
     burst, span_b, lat_b = run(srv, np.zeros(n_req))
     cap = n_req / span_b
     if stream:
@@ -504,6 +505,8 @@ def main():
     ap.add_argument("--serve-inflight", type=int, default=4,
                     help="server worker threads; 4 matches the HIP runtime's 4 hardware queues per process")
     ap.add_argument("--serve-batch", type=int, default=8, help="requests per union-mesh batch in the server")
+    ap.add_argument("--serve-wait-us", type=int, default=50000,
+                    help="longest wait of a batchable request for companions while the GPU is busy")
     ap.add_argument("--batch-cases", type=int, default=32,
                     help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")
     ap.add_argument("--batch-fp64-sample", type=int, default=32,
@@ -543,7 +546,7 @@ def main():
         if "serve" in legs:
             nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
             out["serving"] = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight,
-                                       args.serve_batch)
+                                       args.serve_batch, batch_wait_us=args.serve_wait_us)
         if "batch" in legs and args.batch_cases:
             out["batched_small_cases"] = batch_leg(ctx, args.batch_cases, args.batch_fp64_sample)
         ctx.close()
@@ -618,7 +621,7 @@ def main():
         if "serve" in legs:
             nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
             serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight,
-                                args.serve_batch)
+                                args.serve_batch, batch_wait_us=args.serve_wait_us)
         if "batch" in legs and args.batch_cases:
             batched = batch_leg(ctx, args.batch_cases, args.batch_fp64_sample)
     ctx.close()
