@@ -401,9 +401,9 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
     // union's nodes, their states move to a new union of just those cases
     // and the finished ones are harvested.  A case's arithmetic does not
     // depend on the other cases of its union, so its iterates are unchanged.
-    // CFD_BATCH_COMPACT=0 never compacts, =1 compacts at any union size.
+    // CFD_BATCH_COMPACT=0 never compacts.
     const char *ce = getenv("CFD_BATCH_COMPACT");
-    const int64_t compact_min = ce ? (ce[0] == '0' ? INT64_MAX : 0) : 2 * (int64_t)kSellMinRows;
+    const int64_t compact_min = ce && ce[0] == '0' ? INT64_MAX : 0;
     for (int c = 0; c < K; c++) results[c] = cfd_solve_result{};
     std::vector<std::vector<double>> q_keep(K);
     std::vector<char> active(K, 1), keep(K, 0);

@@ -227,9 +227,9 @@ class _env:
 @pytest.mark.parametrize("compact", ["0", "1"])
 def test_server_batches_match_alone(ctx, alone_runs, compact):
     """Continuous batching: with one worker busy on the first request, the queued compatible requests are solved
-    together as union meshes, and every result equals the same request solved alone, bit for bit -- also when
-    each batch is compacted to its unfinished cases as soon as they hold half of its nodes or less
-    (CFD_BATCH_COMPACT=1), the finished ones being posted at that point."""
+    together as union meshes, and every result equals the same request solved alone, bit for bit -- with each
+    batch compacted to its unfinished cases as soon as they hold half of its nodes or less (the default, the
+    finished ones being posted at that point) and without compaction (CFD_BATCH_COMPACT=0)."""
     meshes, alone = alone_runs
     with _env(CFD_BATCH_COMPACT=compact):
         srv = cfd.Server(ctx, hbm_budget=8 << 30, max_inflight=1, batch_max=4)
@@ -340,8 +340,8 @@ def test_batch_solve_matches_alone(ctx, precond):
 
 
 def test_batch_solve_compaction_matches_alone(ctx):
-    """CFD_BATCH_COMPACT=1: whenever the cases still iterating hold half of the union's nodes or less, they move to
-    a new union of their own.  Every case still reproduces its alone solve bit for bit; the finished cases are
+    """Compaction: whenever the cases still iterating hold half of the union's nodes or less, they move to a new
+    union of their own.  Every case still reproduces its alone solve bit for bit; the finished cases are
     harvested at the compaction, so their reported times are shorter than the batch's."""
     cases = [((20, 6, 6), 0.08, 0.35, 0.0), ((16, 6, 4), 0.12, 0.45, 2.0), ((14, 5, 5), 0.1, 0.5, -1.0),
              ((12, 4, 4), 0.1, 0.75, 3.0), ((12, 4, 4), 0.06, 0.7, 1.0)]
