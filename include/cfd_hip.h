@@ -264,7 +264,10 @@ int cfd_mesh_reset(cfd_mesh *mesh, const cfd_params *p);
  * iterates are bit-identical to cfd_solve_steady on that case alone.  All
  * cases must share the scheme fields of cfd_params (everything except the
  * freestream) and the solver fields of cfd_solve_opts (everything except
- * alpha_deg); results[c].seconds is the time of the whole batch. */
+ * alpha_deg).  Once the cases still iterating hold half of the union's
+ * nodes or less, the union is rebuilt from just those cases (their states
+ * carried over; CFD_BATCH_COMPACT=0 disables this), so results[c].seconds is
+ * the time from the start of the batch until case c's result was final. */
 int cfd_batch_solve(cfd_ctx *ctx, int32_t ncase, const cfd_mesh_desc *meshes, const cfd_params *params,
                     const cfd_solve_opts *opts, cfd_solve_result *results);
 
