@@ -1132,7 +1132,7 @@ __device__ __forceinline__ void block_dots(size_t o, size_t i0, size_t n, size_t
                                            const TV *__restrict__ V, size_t ld, const TV *__restrict__ w,
                                            double *__restrict__ part) {
     __shared__ double wsum[8][kBlock / 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int k0 = 0; k0 < nv; k0 += 8) {
         double acc[8];
 #pragma unroll
@@ -1172,8 +1172,8 @@ __device__ __forceinline__ void block_dots(size_t o, size_t i0, size_t n, size_t
             if (k0 + kk >= nv) break;
             double v = acc[kk];
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (lane == 0) wsum[kk][wv] = v;
+            for (int sh = 32; sh > 0; sh >>= 1) v += __shfl_xor(v, sh, 64);
+            if (lane == 0) wsum[kk][wave] = v;
         }
         __syncthreads();
         if (threadIdx.x == 0)
