@@ -328,6 +328,36 @@ __device__ __forceinline__ bool pressure_final(const T *qc, T g, bool zeroed, bo
     return risky;
 }
 
+// Whether no face state of node c can have a negative or near-vacuum pressure,
+// from bounds alone.  Component k of a face state differs from qc[k] by
+// ph[k] |frac (qj - qc) + (1 - frac) gr.R|, which is at most
+// ph[k] (|frac| A_k + |1 - frac| |gr_k| rmax), where A_k bounds |qj - qc| by
+// the neighbour extrema and rmax is the longest half edge.  From these, a lower
+// bound of the density and of the pressure (concave in the state) and an upper
+// bound of |E|; the slack covers the rounding of both this bound and the face
+// states.  When it holds, the pressure pass would change nothing: every face
+// has a positive density and a pressure above the vacuum margin.
+template <class T>
+__device__ __forceinline__ bool faces_safe(const T *qc, const T *gr, const T *qmin, const T *qmax, const T *ph,
+                                           T frac, T rmax, T g) {
+    const T slack = sizeof(T) == 8 ? T(1e-12) : T(1e-5);
+    T e[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const T A = tmax(qmax[k] - qc[k], qc[k] - qmin[k]);
+        const T G = sqrt(gr[3 * k] * gr[3 * k] + gr[3 * k + 1] * gr[3 * k + 1] + gr[3 * k + 2] * gr[3 * k + 2]);
+        e[k] = ph[k] * (fabs(frac) * A + fabs(T(1.0) - frac) * G * rmax);
+        e[k] = e[k] * (T(1.0) + slack) + slack * fabs(qc[k]);
+    }
+    const T rho = qc[0] - e[0];
+    const T m1 = fabs(qc[1]) + e[1], m2 = fabs(qc[2]) + e[2], m3 = fabs(qc[3]) + e[3];
+    const T kin = T(0.5) * (m1 * m1 + m2 * m2 + m3 * m3) / rho;
+    const T P = (g - T(1.0)) * ((qc[4] - e[4]) - kin);
+    const T Eabs = fabs(qc[4]) + e[4];
+    return rho > slack * fabs(qc[0]) && kin >= T(0.0) &&
+           P > vacuum_margin<T>() * Eabs * (T(1.0) + slack) + slack * (g - T(1.0)) * (Eabs + kin);
+}
+
 template <class T, int KIND>  // 1 Venkatakrishnan, 2 Barth-Jespersen
 __device__ __forceinline__ bool limiter_passes(int N, int deg, size_t base, const int *__restrict__ nbr,
                                                const T *__restrict__ qa, const T *qc, T xc, T yc, T zc, const T *gr,
@@ -336,12 +366,16 @@ __device__ __forceinline__ bool limiter_passes(int N, int deg, size_t base, cons
     for (int k = 0; k < 5; k++) ph[k] = T(1.0);
     T eps2 = k3;
     eps2 = T(6.0) * eps2 * v / T(M_PI);
+    T r2 = T(0.0);  // longest half edge, squared
     for (int n = 0; n < deg; n++) {
         const int jr = nbr[base + (size_t)n * kSlice];
         if (jr >= N) continue;
         const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
         limiter_nbr<T, KIND>(rj, rj[5], rj[6], rj[7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph);
+        const T Rx = T(0.5) * (rj[5] - xc), Ry = T(0.5) * (rj[6] - yc), Rz = T(0.5) * (rj[7] - zc);
+        r2 = tmax(r2, Rx * Rx + Ry * Ry + Rz * Rz);
     }
+    if (faces_safe<T>(qc, gr, qmin, qmax, ph, frac, sqrt(r2), g)) return false;
     bool zeroed = false, risky = false;
     for (int n = 0; n < deg; n++) {
         const int jr = nbr[base + (size_t)n * kSlice];
@@ -529,13 +563,19 @@ __global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restr
         for (int k = 0; k < 5; k++) ph[k] = T(1.0);
         T eps2 = k3;
         eps2 = T(6.0) * eps2 * v / T(M_PI);
+        T r2 = T(0.0);  // longest half edge, squared
         staged_walk<T, false>(qa, isx, isy, isz, nbr, base, deg, slot, aux, lane,
                               [&](int jr, const T *r, T, T, T) {
             if (jr >= N) return;
             limiter_nbr<T, KIND>(r, r[5], r[6], r[7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph);
+            const T Rx = T(0.5) * (r[5] - xc), Ry = T(0.5) * (r[6] - yc), Rz = T(0.5) * (r[7] - zc);
+            r2 = tmax(r2, Rx * Rx + Ry * Ry + Rz * Rz);
         });
+        // the pressure pass, unless the bounds show it would change nothing
+        // (a lane with nothing to check walks no incidences)
         bool zeroed = false, risky = false;
-        staged_walk<T, false>(qa, isx, isy, isz, nbr, base, deg, slot, aux, lane,
+        const bool skip = faces_safe<T>(qc, gr, qmin, qmax, ph, frac, sqrt(r2), g);
+        staged_walk<T, false>(qa, isx, isy, isz, nbr, base, skip ? 0 : deg, slot, aux, lane,
                               [&](int jr, const T *r, T, T, T) {
             if (jr >= N) return;
             pressure_nbr<T>(r, r[5], r[6], r[7], qc, xc, yc, zc, gr, frac, g, ph, zeroed, risky);

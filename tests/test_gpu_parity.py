@@ -267,6 +267,43 @@ def test_fused_gg_limiter_equals_separate_calls(case, limiter, precision):
         assert_bitexact(r_fused, dm.get(cfd.FIELD_RES_CONV), "FP32 residual after fused gradient+limiter")
 
 
+@pytest.mark.parametrize("limiter", [cfd.LIM_VENKAT, cfd.LIM_BJ])
+def test_limiter_pressure_correction_near_vacuum(case, limiter):
+    """Compute_Limiter_PressureCorrection acting: every seventh node is put near vacuum (pressure 1e-6 of the
+    freestream, internal energy alone changed), so limited face states towards and from those nodes reach negative
+    pressures and their limiters are zeroed.  The GPU limiter (separate after LSQ, and fused with Green-Gauss, both
+    of which skip the pressure pass only where bounds prove it changes nothing) equals the oracle bit for bit; HLLC,
+    so that the Roe-only pass stays out."""
+    d, dm, _ = case
+    if "grad" not in d:
+        pytest.skip("first-order case")
+    p = gpu_params(d, flux="hllc")
+    p.order, p.limiter = 2, limiter
+    q = np.array(d["q_bc"], dtype=np.float64)
+    n = d["xyz"].shape[0]
+    g = float(d["inf"][8])
+    low = np.arange(0, n, 7)
+    kin = 0.5 * (q[1, low] ** 2 + q[2, low] ** 2 + q[3, low] ** 2) / q[0, low]
+    q[4, low] = kin + 1e-6 * float(d["inf"][4]) / (g - 1.0)
+    om = oracle_mesh(d, orc)
+    op = oracle_params(d, orc)
+    op.flux, op.order = orc.HLLC, 2
+    op.limiter = orc.LIM_VENKAT if limiter == cfd.LIM_VENKAT else orc.LIM_BJ
+    for gradient in (cfd.GRAD_LSQ, cfd.GRAD_GG):
+        p.gradient = gradient
+        dm.set(cfd.FIELD_Q, q)
+        og = orc.lsq_gradient(om, q) if gradient == cfd.GRAD_LSQ else orc.gg_gradient(om, q)
+        ophi = np.asarray(orc.limiter(om, op, q, og)).reshape(5, -1)
+        zeroed = np.count_nonzero(np.all(ophi == 0.0, axis=0))
+        assert zeroed > 0, "the pressure correction did not act"
+        if gradient == cfd.GRAD_LSQ:
+            dm.gradient(p)
+            dm.limiter(p)
+        else:
+            dm.gradient_limiter(p)
+        assert_bitexact(dm.get(cfd.FIELD_PHI), ophi, "limiter near vacuum (%d nodes zeroed)" % zeroed)
+
+
 def test_fp32_residual_close_to_fp64(case):
     d, dm, p = case
     for flux in ("roe", "hllc", "ausm+"):
