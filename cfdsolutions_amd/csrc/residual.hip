@@ -260,13 +260,6 @@ __device__ __forceinline__ bool near_vacuum(const T *Q, T P) {
     return !(Q[0] > T(0.0)) || !(P > vacuum_margin<T>() * fabs(Q[4]));
 }
 
-// Limiter value of every variable of node c from its state qc, coordinates,
-// gradient gr and the neighbour extrema, followed by the pressure correction
-// (Compute_Limiter_Venkatakrishnan / _BarthJespersen and
-// Compute_Limiter_PressureCorrection, Limiters.cpp:130-304).  Returns whether
-// a final face state of the node is near vacuum (the only case in which the
-// Roe pressure correction can act).  Neither pass depends on the order of the
-// neighbours.
 // One neighbour's bound on the limiter of node c (Compute_Limiter_Venkatakrishnan,
 // Limiters.cpp:185-267; Barth-Jespersen, Limiters.cpp:116-180): qj and
 // (xj, yj, zj) are the neighbour's state and coordinates.
@@ -358,6 +351,13 @@ __device__ __forceinline__ bool faces_safe(const T *qc, const T *gr, const T *qm
            P > vacuum_margin<T>() * Eabs * (T(1.0) + slack) + slack * (g - T(1.0)) * (Eabs + kin);
 }
 
+// Limiter value of every variable of node c from its state qc, coordinates,
+// gradient gr and the neighbour extrema, followed by the pressure correction
+// (Compute_Limiter_Venkatakrishnan / _BarthJespersen and
+// Compute_Limiter_PressureCorrection, Limiters.cpp:130-304), which is skipped
+// when faces_safe shows it cannot act.  Returns whether a final face state of
+// the node is near vacuum (the only case in which the Roe pressure correction
+// can act).  Neither pass depends on the order of the neighbours.
 template <class T, int KIND>  // 1 Venkatakrishnan, 2 Barth-Jespersen
 __device__ __forceinline__ bool limiter_passes(int N, int deg, size_t base, const int *__restrict__ nbr,
                                                const T *__restrict__ qa, const T *qc, T xc, T yc, T zc, const T *gr,
