@@ -367,13 +367,27 @@ __device__ __forceinline__ bool limiter_passes(int N, int deg, size_t base, cons
     T eps2 = k3;
     eps2 = T(6.0) * eps2 * v / T(M_PI);
     T r2 = T(0.0);  // longest half edge, squared
-    for (int n = 0; n < deg; n++) {
-        const int jr = nbr[base + (size_t)n * kSlice];
-        if (jr >= N) continue;
-        const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
-        limiter_nbr<T, KIND>(rj, rj[5], rj[6], rj[7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph);
-        const T Rx = T(0.5) * (rj[5] - xc), Ry = T(0.5) * (rj[6] - yc), Rz = T(0.5) * (rj[7] - zc);
-        r2 = tmax(r2, Rx * Rx + Ry * Ry + Rz * Rz);
+    // two neighbours at a time: both records are loaded before either bound
+    // is applied (the minimum does not depend on the order)
+    for (int n = 0; n < deg; n += 2) {
+        const int j0 = nbr[base + (size_t)n * kSlice];
+        const int j1 = n + 1 < deg ? nbr[base + (size_t)(n + 1) * kSlice] : N;
+        T r[2][8];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int jr = u ? j1 : j0;
+            if (jr >= N) continue;
+            const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
+#pragma unroll
+            for (int e = 0; e < 8; e++) r[u][e] = rj[e];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if ((u ? j1 : j0) >= N) continue;
+            limiter_nbr<T, KIND>(r[u], r[u][5], r[u][6], r[u][7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph);
+            const T Rx = T(0.5) * (r[u][5] - xc), Ry = T(0.5) * (r[u][6] - yc), Rz = T(0.5) * (r[u][7] - zc);
+            r2 = tmax(r2, Rx * Rx + Ry * Ry + Rz * Rz);
+        }
     }
     if (faces_safe<T>(qc, gr, qmin, qmax, ph, frac, sqrt(r2), g)) return false;
     bool zeroed = false, risky = false;
