@@ -328,7 +328,7 @@ def solve3_leg(ctx):
 
 
 def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, batch_max=8, seed=20261202, mach=(0.3, 0.85),
-              aoa=(-2.0, 6.0), fluxes=("roe", "hllc"), stream=True, batch_wait_us=50000):
+              aoa=(-2.0, 6.0), fluxes=("roe", "hllc"), stream=True, batch_wait_us=50000, batch_case_nodes=0):
     """Steady-solve requests on bump channels of log-uniform size in [nmin, nmax] nodes, bump height U(0.05,
     0.15), Mach U(0.3, 0.85), AoA U(-2, 6) deg, Roe or HLLC (first order), to RMS 1e-10, FGMRES(30) with the
     multicolour ILU(0).  The plain FP64 GPU path solves them one at a time (meshes resident).  The server solves
@@ -401,7 +401,7 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, batch_max=8, seed=2026
 
     # continuous batching
     srv = cfd.Server(ctx, hbm_budget=200 << 30, max_inflight=inflight, batch_max=batch_max,
-                     batch_wait_us=batch_wait_us)
+                     batch_wait_us=batch_wait_us, batch_case_nodes=batch_case_nodes)
     burst, span_b, lat_b = run(srv, np.zeros(n_req))
     cap = n_req / span_b
     if stream:
@@ -509,6 +509,8 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=8, help="requests per union-mesh batch in the server")
     ap.add_argument("--serve-wait-us", type=int, default=50000,
                     help="longest wait of a batchable request for companions while the GPU is busy")
+    ap.add_argument("--serve-case-nodes", type=int, default=0,
+                    help="largest mesh (nodes) the server batches (0: the library default, 131072)")
     ap.add_argument("--batch-cases", type=int, default=32,
                     help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")
     ap.add_argument("--batch-fp64-sample", type=int, default=32,
@@ -548,7 +550,8 @@ def main():
         if "serve" in legs:
             nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
             out["serving"] = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight,
-                                       args.serve_batch, batch_wait_us=args.serve_wait_us)
+                                       args.serve_batch, batch_wait_us=args.serve_wait_us,
+                                batch_case_nodes=args.serve_case_nodes)
         if "batch" in legs and args.batch_cases:
             out["batched_small_cases"] = batch_leg(ctx, args.batch_cases, args.batch_fp64_sample)
         ctx.close()
@@ -623,7 +626,8 @@ def main():
         if "serve" in legs:
             nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
             serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight,
-                                args.serve_batch, batch_wait_us=args.serve_wait_us)
+                                args.serve_batch, batch_wait_us=args.serve_wait_us,
+                                batch_case_nodes=args.serve_case_nodes)
         if "batch" in legs and args.batch_cases:
             batched = batch_leg(ctx, args.batch_cases, args.batch_fp64_sample)
     ctx.close()
