@@ -10,9 +10,21 @@
  *     orc_lsq_gradient, orc_limiter (Venkatakrishnan, Barth-Jespersen,
  *     pressure corrections), orc_delta_t, orc_jacobian, orc_block_gs,
  *     orc_lusgs
- *   pinned by closed forms only (no reference arithmetic exists):
- *     AUSM+ flux (Liou 1996), orc_gg_gradient, orc_ilu0 / orc_fgmres
- *     (Saad 2003), orc_forces, orc_solve_steady (iteration counts)
+ *   pinned by closed forms only (no reference arithmetic exists), each
+ *   chosen so that a plausible mistake fails it:
+ *     AUSM+ flux (Liou 1996): exact stationary normal shock (pins the
+ *       interface sound speed), the expanded subsonic split polynomials
+ *       (pin alpha = 3/16, beta = 1/8), consistency, supersonic upwinding,
+ *       stationary contact;
+ *     orc_gg_gradient: exact for linear fields at interior nodes, zero for a
+ *       nodal constant at every node whatever the ghost states (pins the
+ *       boundary term);
+ *     orc_ilu0 / orc_fgmres (Saad 2003): (L U)_ij = A_ij on A's pattern for
+ *       a Jacobian with fill, exact LU without fill, dense solves;
+ *     orc_forces: closed surface under uniform pressure gives F = 0; lift
+ *       and drag are F along the reference's freestream axes at 30, 90 and
+ *       -2 degrees;
+ *     orc_solve_steady (iteration counts): the GPU runs the same algorithm.
  *
  * Layouts: Q, residuals, limiter: structure of arrays [5][n];
  * gradients [15][N] ordered Q1x,Q1y,Q1z,Q2x,...,Q5z; coordinates and area

@@ -41,6 +41,8 @@ CASES = [
     ("bump12_roe1", (12, 4, 4), "roe", 1, 0.5, 0.0),
     ("bump12_hllc2", (12, 4, 4), "hllc", 2, 0.6, 2.0),
     ("bump16_roe2", (16, 6, 6), "roe", 2, 0.5, 1.0),
+    # the plain CPU-runnable case: 48x16x16 bump channel, 1st-order Roe, M 0.5, GMRES(30) + ILU(0)
+    ("bump48_roe1", (48, 16, 16), "roe", 1, 0.5, 0.0),
 ]
 
 
@@ -60,6 +62,9 @@ def test_steady_solve_matches_oracle(ctx, name, dims, flux, order, mach, alpha):
     assert abs(r.iterations - ro.iterations) <= 1, (r.iterations, ro.iterations)
     assert abs(r.krylov_total - ro.krylov_total) <= max(3, 0.1 * ro.krylov_total), (r.krylov_total, ro.krylov_total)
     np.testing.assert_allclose(list(r.force), list(ro.force), rtol=1e-8, atol=1e-10)
+    fscale = np.max(np.abs(ro.force))
+    assert abs(r.lift - ro.lift) <= 1e-8 * fscale and abs(r.drag - ro.drag) <= 1e-8 * fscale, (
+        r.lift, ro.lift, r.drag, ro.drag)
     assert abs(r.pressure_integral - ro.pressure_integral) <= 1e-8 * abs(ro.pressure_integral)
     q = dm.get(cfd.FIELD_Q)[:, : m.n_node]
     assert np.max(np.abs(q - qo[:, : m.n_node])) <= 1e-7 * np.max(np.abs(qo))
@@ -71,6 +76,7 @@ def test_steady_solve_matches_oracle(ctx, name, dims, flux, order, mach, alpha):
                              precond=cfd.PREC_ILU0, mixed=True, alpha_deg=alpha)
     assert r2.converged and r2.final_rms < 1e-10
     np.testing.assert_allclose(list(r2.force), list(r.force), rtol=1e-8, atol=1e-10)
+    assert abs(r2.lift - r.lift) <= 1e-8 * fscale and abs(r2.drag - r.drag) <= 1e-8 * fscale
     q2 = dm2.get(cfd.FIELD_Q)[:, : m.n_node]
     assert np.max(np.abs(q2 - q)) <= 1e-7 * np.max(np.abs(q))
     print("%s: fp64 %d iters/%d krylov %.3fs, mixed %d/%d %.3fs, oracle %d/%d" % (
