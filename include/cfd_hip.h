@@ -46,7 +46,8 @@ extern "C" {
 #define CFD_FLUX_ROE 0
 #define CFD_FLUX_HLLC 1
 #define CFD_FLUX_AUSMP 2
-/* limiters (LIMITER_METHOD + 1) */
+/* limiters (LIMITER_METHOD_NONE, _VENKATAKRISHNAN and _BERTH_JESPERSEN are -1, 1 and 0 in
+ * Euler3D/include/SolverParameters.h:343-345; the shim maps them by name) */
 #define CFD_LIM_NONE 0
 #define CFD_LIM_VENKAT 1
 #define CFD_LIM_BJ 2
