@@ -574,22 +574,27 @@ def main():
     bytes64 = algorithmic_bytes(N, E, B, 8)
     bytes32 = algorithmic_bytes(N, E, B, 4)
     variants = {}
-    plan = [("roe", cfd.FP64, True, True)]
+    EX, FA = cfd.ARITH_EXACT, cfd.ARITH_FAST
+    plan = [("roe", cfd.FP64, True, True, EX), ("roe", cfd.FP64, True, True, FA)]
     if not args.no_variants and not args.only:
-        plan += [("roe", cfd.FP64, True, False), ("hllc", cfd.FP64, True, True), ("ausm+", cfd.FP64, True, True),
-                 ("roe", cfd.FP32, True, True), ("hllc", cfd.FP32, True, True), ("ausm+", cfd.FP32, True, True),
-                 ("roe", cfd.FP64, False, True)]
-    for flux, prec, second, fused in plan:
+        plan += [("roe", cfd.FP64, True, False, EX), ("hllc", cfd.FP64, True, True, EX),
+                 ("ausm+", cfd.FP64, True, True, EX), ("roe", cfd.FP32, True, True, EX),
+                 ("hllc", cfd.FP32, True, True, EX), ("ausm+", cfd.FP32, True, True, EX),
+                 ("roe", cfd.FP64, False, True, EX), ("hllc", cfd.FP64, True, True, FA),
+                 ("ausm+", cfd.FP64, True, True, FA), ("roe", cfd.FP32, True, True, FA),
+                 ("hllc", cfd.FP32, True, True, FA), ("ausm+", cfd.FP32, True, True, FA),
+                 ("roe", cfd.FP64, False, True, FA)]
+    for flux, prec, second, fused, arith in plan:
         pv = cfd.make_params(mach=0.8, flux=flux, order=2 if second else 1, limiter=cfd.LIM_VENKAT if second else 0,
-                             venkat_k=5.0, gradient=cfd.GRAD_GG, precision=prec)
-        head_case = (flux, prec, second, fused) == plan[0]
+                             venkat_k=5.0, gradient=cfd.GRAD_GG, precision=prec, arith=arith)
+        head_case = (flux, prec, second, fused, arith) == plan[0]
         steps = args.steps if head_case else max(10, args.steps // 4)
         wall, tg, tl, tr = time_variant(dm, ev, pv, steps, args.warmup if head_case else 5, second, fused)
         by = bytes64 if prec == cfd.FP64 else bytes32
         rkey = ("R2" if second else "R1") + ("split" if flux == "roe" else "")
         total = (by["G"] + by["L"] + by[rkey]) if second else by[rkey]
-        name = "%s-%s-%s" % (flux, "fp64" if prec == cfd.FP64 else "fp32",
-                             ("GL+R2" if fused else "G+L+R2") if second else "R1")
+        name = "%s-%s%s-%s" % (flux, "fp64" if prec == cfd.FP64 else "fp32", "fast" if arith == FA else "",
+                               ("GL+R2" if fused else "G+L+R2") if second else "R1")
         variants[name] = dict(ms_per_step=wall, gbs=total / wall / 1e6, g_ms=tg, l_ms=tl, r_ms=tr,
                               r_gbs=by[rkey] / tr / 1e6 if tr > 0 else None,
                               g_gbs=(by["G"] + (by["L"] if fused else 0)) / tg / 1e6 if tg > 0 else None,

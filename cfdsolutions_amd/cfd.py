@@ -16,6 +16,8 @@ FLUX_ROE, FLUX_HLLC, FLUX_AUSMP = 0, 1, 2
 LIM_NONE, LIM_VENKAT, LIM_BJ = 0, 1, 2
 GRAD_LSQ, GRAD_GG = 0, 1
 FP64, FP32 = 0, 1
+ARITH_EXACT, ARITH_FAST = 0, 1
+SOLVE_JAC_APPROX, SOLVE_JAC_FD = 0, 1
 PREC_NONE, PREC_BJACOBI, PREC_ILU0, PREC_ILU0_MC = 0, 1, 2, 3
 SOLVER_FGMRES, SOLVER_BLOCK_GS, SOLVER_LUSGS = 0, 1, 2
 JAC_FD_CENTRAL, JAC_FD_FORWARD, JAC_FD_BACKWARD, JAC_FD_ALTERNATE = 0, 1, 2, 3
@@ -45,7 +47,7 @@ class Params(ctypes.Structure):
                 ("limiter", ctypes.c_int32), ("limiter_order", ctypes.c_int32), ("entropy_fix", ctypes.c_int32),
                 ("gradient", ctypes.c_int32), ("zpg_iter", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("venkat_k", ctypes.c_double), ("cfl_min", ctypes.c_double), ("cfl_max", ctypes.c_double),
-                ("cfl_ramp", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("cfl_ramp", ctypes.c_int32), ("arith", ctypes.c_int32)]
 
 
 class LinsolveOpts(ctypes.Structure):
@@ -62,7 +64,7 @@ class LinsolveStats(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int32), ("tol", ctypes.c_double), ("first_order_iters", ctypes.c_int32),
                 ("relaxation", ctypes.c_double), ("wall_tag", ctypes.c_int32), ("alpha_deg", ctypes.c_double),
-                ("lin", LinsolveOpts)]
+                ("lin", LinsolveOpts), ("jacobian", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class SolveResult(ctypes.Structure):
@@ -100,9 +102,9 @@ class ServerStats(ctypes.Structure):
 
 def solve_opts(max_iter=200, tol=1e-10, first_order_iters=0, relaxation=1.0, wall_tag=3, alpha_deg=0.0,
                precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, mixed=False, solver=SOLVER_FGMRES,
-               sweeps=0):
+               sweeps=0, jacobian=0):
     lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
-    return SolveOpts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, lin)
+    return SolveOpts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, lin, int(jacobian), 0)
 
 
 def mesh_desc(mesh, bedge_type=None):
@@ -148,7 +150,7 @@ def _p(a):
 
 def make_params(mach=0.5, alpha_deg=0.0, gamma=1.4, flux="roe", order=1, limiter=LIM_NONE, limiter_order=1,
                 entropy_fix=0, gradient=GRAD_LSQ, precision=FP64, venkat_k=5.0, cfl_min=1.0, cfl_max=1000.0,
-                cfl_ramp=50, zpg_iter=0, inf=None):
+                cfl_ramp=50, zpg_iter=0, inf=None, arith=0):
     """Freestream of the GENERIC non-dimensionalisation (Euler3D/src/Material.cpp:112-210):
     rho = 1, T = gamma, R = 1/gamma, p = R*rho*T, c = sqrt(gamma*R*T)."""
     if inf is None:
@@ -165,7 +167,7 @@ def make_params(mach=0.5, alpha_deg=0.0, gamma=1.4, flux="roe", order=1, limiter
         rho, u, v, w, p, outp = inf[0], inf[1], inf[2], inf[3], inf[4], inf[10]
     f = _FLUX_NAMES[flux] if isinstance(flux, str) else flux
     return Params(gamma, rho, u, v, w, p, outp, f, order, limiter, limiter_order, entropy_fix, gradient, zpg_iter,
-                  precision, venkat_k, cfl_min, cfl_max, cfl_ramp, 0)
+                  precision, venkat_k, cfl_min, cfl_max, cfl_ramp, arith)
 
 
 class Context:
@@ -318,9 +320,9 @@ class DeviceMesh:
 
     def solve_steady(self, p, max_iter=200, tol=1e-10, first_order_iters=0, relaxation=1.0, wall_tag=3,
                      alpha_deg=0.0, precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, mixed=False,
-                     solver=SOLVER_FGMRES, sweeps=0, check=True):
+                     solver=SOLVER_FGMRES, sweeps=0, jacobian=SOLVE_JAC_APPROX, check=True):
         o = solve_opts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, precond, krylov_m,
-                       max_restarts, rtol, mixed, solver, sweeps)
+                       max_restarts, rtol, mixed, solver, sweeps, jacobian)
         r = SolveResult()
         hist = np.zeros(max_iter)
         rc = lib().cfd_solve_steady(self.h, ctypes.byref(p), ctypes.byref(o), ctypes.byref(r), _p(hist))

@@ -114,6 +114,22 @@ __global__ __launch_bounds__(kBlock) void k_update_seg(int N, double relax, cons
     for (int k = 0; k < 5; k++) qa[kQA * (size_t)i + k] += relax * x[5 * r + k];
 }
 
+// Sets the nodes [n0, n0 + nn) and the ghosts [g0, g0 + ng) to the state q
+// (a failed case of a union is parked at its freestream, so its rows stay
+// finite and raise no more flags while it rides along).
+__global__ __launch_bounds__(kBlock) void k_park_case(int n0, int nn, int g0, int ng, double q0, double q1, double q2,
+                                                      double q3, double q4, double *qa) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nn + ng) return;
+    const size_t i = t < nn ? (size_t)(n0 + t) : (size_t)(g0 + t - nn);
+    double *r = qa + kQA * i;
+    r[0] = q0;
+    r[1] = q1;
+    r[2] = q2;
+    r[3] = q3;
+    r[4] = q4;
+}
+
 // Segment map: case c gets nb[c] blocks over [off[c], off[c] + len[c]).
 int build_segmap(cfd_mesh *m, SegMap &sg, const std::vector<int> &off, const std::vector<int> &len,
                  const std::vector<int> &nb) {
@@ -129,10 +145,10 @@ int build_segmap(cfd_mesh *m, SegMap &sg, const std::vector<int> &off, const std
     sg.off = (int *)cfd_dalloc(m, sizeof(int) * K);
     sg.len = (int *)cfd_dalloc(m, sizeof(int) * K);
     if (!sg.blk || !sg.first || !sg.off || !sg.len) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory");
-    CFD_HIP(m, hipMemcpy(sg.blk, blk.data(), sizeof(int2) * blk.size(), hipMemcpyHostToDevice));
-    CFD_HIP(m, hipMemcpy(sg.first, sg.h_first.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice));
-    CFD_HIP(m, hipMemcpy(sg.off, off.data(), sizeof(int) * K, hipMemcpyHostToDevice));
-    CFD_HIP(m, hipMemcpy(sg.len, len.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    CFD_HIP(m, cfd_memcpy(sg.blk, blk.data(), sizeof(int2) * blk.size(), hipMemcpyHostToDevice));
+    CFD_HIP(m, cfd_memcpy(sg.first, sg.h_first.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice));
+    CFD_HIP(m, cfd_memcpy(sg.off, off.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    CFD_HIP(m, cfd_memcpy(sg.len, len.data(), sizeof(int) * K, hipMemcpyHostToDevice));
     return CFD_OK;
 }
 
@@ -141,15 +157,16 @@ int build_segmap(cfd_mesh *m, SegMap &sg, const std::vector<int> &off, const std
 bool cfd_same_scheme(const cfd_params &a, const cfd_params &b) {
     return a.gamma == b.gamma && a.flux == b.flux && a.order == b.order && a.limiter == b.limiter &&
            a.limiter_order == b.limiter_order && a.entropy_fix == b.entropy_fix && a.gradient == b.gradient &&
-           a.zpg_iter == b.zpg_iter && a.precision == b.precision && a.venkat_k == b.venkat_k &&
-           a.cfl_min == b.cfl_min && a.cfl_max == b.cfl_max && a.cfl_ramp == b.cfl_ramp;
+           a.zpg_iter == b.zpg_iter && a.precision == b.precision && a.arith == b.arith &&
+           a.venkat_k == b.venkat_k && a.cfl_min == b.cfl_min && a.cfl_max == b.cfl_max && a.cfl_ramp == b.cfl_ramp;
 }
 
 bool cfd_same_solve(const cfd_solve_opts &a, const cfd_solve_opts &b) {
     return a.max_iter == b.max_iter && a.tol == b.tol && a.first_order_iters == b.first_order_iters &&
            a.relaxation == b.relaxation && a.wall_tag == b.wall_tag && a.lin.solver == b.lin.solver &&
            a.lin.precond == b.lin.precond && a.lin.krylov_m == b.lin.krylov_m &&
-           a.lin.max_restarts == b.lin.max_restarts && a.lin.rtol == b.lin.rtol && a.lin.mixed == b.lin.mixed;
+           a.lin.max_restarts == b.lin.max_restarts && a.lin.rtol == b.lin.rtol && a.lin.mixed == b.lin.mixed &&
+           a.jacobian == b.jacobian;
 }
 
 namespace {
@@ -262,17 +279,20 @@ int build_union(cfd_ctx *ctx, const std::vector<int> &cases, const cfd_mesh_desc
     m->bcoef = (double *)cfd_dalloc(m, sizeof(double) * (kMaxKrylov + 1) * K);
     m->bkdim = (int *)cfd_dalloc(m, sizeof(int) * K);
     u.d_active = (int *)cfd_dalloc(m, sizeof(int) * K);
-    if (!m->case_inf || !m->bcase || !m->ecase || !m->bred || !m->bcoef || !m->bkdim || !u.d_active)
+    m->case_flag = (int *)cfd_dalloc(m, sizeof(int) * K);
+    if (!m->case_inf || !m->bcase || !m->ecase || !m->bred || !m->bcoef || !m->bkdim || !u.d_active ||
+        !m->case_flag)
         return cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of device memory");
-    if (hipHostMalloc((void **)&m->hbred, sizeof(double) * kBOut * K, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&m->hbcoef, sizeof(double) * (kMaxKrylov + 1) * K, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&m->hbkdim, sizeof(int) * K, hipHostMallocDefault) != hipSuccess)
+    if (cfd_host_alloc((void **)&m->hbred, sizeof(double) * kBOut * K) != hipSuccess ||
+        cfd_host_alloc((void **)&m->hbcoef, sizeof(double) * (kMaxKrylov + 1) * K) != hipSuccess ||
+        cfd_host_alloc((void **)&m->hbkdim, sizeof(int) * K) != hipSuccess)
         return cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of pinned host memory");
-    const std::vector<int> ones(K, 1);
-    if (hipMemcpy(m->case_inf, inf.data(), sizeof(double) * inf.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        (B && hipMemcpy(m->bcase, bcase.data(), sizeof(int) * B, hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemcpy(m->ecase, node_case.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(u.d_active, ones.data(), sizeof(int) * K, hipMemcpyHostToDevice) != hipSuccess)
+    const std::vector<int> ones(K, 1), zeros(K, 0);
+    if (cfd_memcpy(m->case_inf, inf.data(), sizeof(double) * inf.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        (B && cfd_memcpy(m->bcase, bcase.data(), sizeof(int) * B, hipMemcpyHostToDevice) != hipSuccess) ||
+        cfd_memcpy(m->ecase, node_case.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess ||
+        cfd_memcpy(u.d_active, ones.data(), sizeof(int) * K, hipMemcpyHostToDevice) != hipSuccess ||
+        cfd_memcpy(m->case_flag, zeros.data(), sizeof(int) * K, hipMemcpyHostToDevice) != hipSuccess)
         return cfd_set_error(ctx, CFD_E_HIP, "hipMemcpy failed");
     m->case_active = u.d_active;
     rc = build_segmap(m, m->seg_rms, zoff, zlen, nb);
@@ -325,11 +345,13 @@ int harvest(Union &u, const std::vector<int> &take, const cfd_solve_opts *opts, 
         hipStreamSynchronize(m->stream) != hipSuccess)
         return cfd_set_error(m->ctx, CFD_E_HIP, "cfd_batch_solve: reading forces failed");
     bool need_q = false;
-    for (int lc : take) need_q = need_q || (q_out && q_out[u.cases[lc]]);
+    for (int lc : take) need_q = need_q || (q_out && q_out[u.cases[lc]] && !results[u.cases[lc]].status);
     for (int lc = 0; lc < K; lc++) need_q = need_q || keep[u.cases[lc]];
     for (int lc : take) {
         const int c = u.cases[lc];
         cfd_solve_result &r = results[c];
+        r.seconds = secs;
+        if (r.status) continue;  // a failed solve reports no forces (as cfd_solve_steady)
         const double *v = m->hbred + (size_t)lc * kBOut;
         const double a = opts[c].alpha_deg * M_PI / 180.0;
         r.force[0] = v[0];
@@ -338,7 +360,6 @@ int harvest(Union &u, const std::vector<int> &take, const cfd_solve_opts *opts, 
         r.drag = v[0] * cos(a) + v[1] * sin(a);
         r.lift = -v[0] * sin(a) + v[1] * cos(a);
         r.pressure_integral = v[3];
-        r.seconds = secs;
     }
     if (!need_q) return CFD_OK;
     const int N = m->N, B = m->B;
@@ -356,7 +377,7 @@ int harvest(Union &u, const std::vector<int> &take, const cfd_solve_opts *opts, 
         }
     };
     for (int lc : take)
-        if (q_out && q_out[u.cases[lc]]) extract(lc, q_out[u.cases[lc]]);
+        if (q_out && q_out[u.cases[lc]] && !results[u.cases[lc]].status) extract(lc, q_out[u.cases[lc]]);
     for (int lc = 0; lc < K; lc++) {
         const int c = u.cases[lc];
         if (!keep[c]) continue;
@@ -413,10 +434,55 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
     int rc = build_union(ctx, all, meshes, params, q_keep, allow_graphs, u);
     if (device_bytes) *device_bytes = u.m ? u.m->bytes : 0;
     std::vector<char> solve;
-    std::vector<int> iters, act_i;
+    std::vector<int> iters, act_i, lstatus;
     std::vector<double> fres;
     cfd_params p = params[0];
     p.precision = CFD_FP64;
+    // Failure isolation.  A case that fails ends with the status and
+    // iteration count its solve alone would report (cfd_solve_steady), is
+    // parked at its freestream so its rows stay finite and raise no more
+    // flags, and is dropped at the next compaction; the other cases go on
+    // unaffected, since no case's arithmetic depends on another's.
+    auto park = [&](int lc, int status, int it) -> int {
+        const int c = u.cases[lc];
+        results[c].status = status;
+        results[c].iterations = it;
+        active[c] = 0;
+        const cfd_params &pc = params[c];
+        double et = pc.inf_u * pc.inf_u + pc.inf_v * pc.inf_v + pc.inf_w * pc.inf_w;
+        et = pc.inf_p / ((pc.gamma - 1.0) * pc.inf_rho) + 0.5 * et;
+        const int nn = u.noff[lc + 1] - u.noff[lc], ng = u.boff[lc + 1] - u.boff[lc];
+        hipLaunchKernelGGL(k_park_case, dim3((nn + ng + kBlock - 1) / kBlock), dim3(kBlock), 0, u.m->stream,
+                           u.noff[lc], nn, u.m->N + u.boff[lc], ng, pc.inf_rho, pc.inf_rho * pc.inf_u,
+                           pc.inf_rho * pc.inf_v, pc.inf_rho * pc.inf_w, pc.inf_rho * et, u.m->d.qa);
+        u.m->q_version++;
+        return hipGetLastError() == hipSuccess ? CFD_OK
+                                               : cfd_set_error(ctx, CFD_E_HIP, "cfd_batch_solve: parking a case failed");
+    };
+    // cfd_bc and the residual checks report a failure for the whole union;
+    // the per-case failure bits name the cases it belongs to, which then fail
+    // on their own with the code a solve of that case alone returns.  If no
+    // case is named, the failure stands for the whole batch.
+    auto isolate = [&](int rc0, int it) -> int {
+        cfd_mesh *m = u.m;
+        const int Ku = (int)u.cases.size();
+        std::vector<int> bits(Ku, 0);
+        if (hipMemcpyAsync(bits.data(), m->case_flag, sizeof(int) * Ku, hipMemcpyDeviceToHost, m->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(m->stream) != hipSuccess ||
+            hipMemsetAsync(m->case_flag, 0, sizeof(int) * Ku, m->stream) != hipSuccess)
+            return cfd_set_error(ctx, CFD_E_HIP, "cfd_batch_solve: reading case flags failed");
+        bool named = false;
+        for (int lc = 0; lc < Ku; lc++) {
+            if (!bits[lc]) continue;
+            named = true;
+            if (!active[u.cases[lc]]) continue;
+            const int st = (bits[lc] & 2) ? CFD_E_ARG : (bits[lc] & 1) ? CFD_E_NEGATIVE_PRESSURE : CFD_E_HLLC;
+            const int r2 = park(lc, st, it);
+            if (r2) return r2;
+        }
+        return named ? CFD_OK : rc0;
+    };
     // backward-Euler Newton iterations in lockstep (cfd_solve_steady per case)
     for (int it = 0; !rc && it < o.max_iter; it++) {
         if (it > 0 && u.m->N >= compact_min) {
@@ -458,6 +524,7 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
         cfd_params pi = p;
         if (pi.order == 2 && it < o.first_order_iters) pi.order = 1;
         rc = cfd_bc(m, &pi, it);
+        if (rc == CFD_E_NEGATIVE_PRESSURE || rc == CFD_E_ARG) rc = isolate(rc, it);
         if (rc) break;
         if (pi.order == 2) {
             rc = pi.limiter != CFD_LIM_NONE ? cfd_gradient_limiter(m, &pi) : cfd_gradient(m, &pi);
@@ -467,6 +534,7 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
         if (rc) break;
         double dummy;
         rc = cfd_residual_rms(m, &dummy);  // flag checks of the residual
+        if (rc == CFD_E_HLLC) rc = isolate(rc, it);
         if (rc) break;
         hipLaunchKernelGGL(k_rms_seg, dim3(m->seg_rms.nblocks), dim3(kBlock), 0, m->stream, m->seg_rms.blk,
                            m->seg_rms.first, m->seg_rms.off, m->seg_rms.len, m->d.res, m->bpart);
@@ -489,22 +557,25 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
             r.final_rms = rms;
             r.iterations = it;
             if (!(rms == rms) || std::isinf(rms)) {
-                r.status = CFD_E_NONFINITE;
-                active[c] = 0;
+                rc = park(lc, CFD_E_NONFINITE, it);
+                if (rc) break;
             } else if (rms < o.tol) {
                 r.converged = 1;
                 active[c] = 0;
             }
             if (active[c]) live_nodes += u.noff[lc + 1] - u.noff[lc];
         }
-        if (!live_nodes) break;
+        if (rc || !live_nodes) break;
         const int Kn = Ku;
         rc = cfd_delta_t(m, &pi, it, nullptr);
-        if (!rc) rc = cfd_jacobian(m, &pi, 1, o.lin.mixed);
+        if (!rc) rc = solve_jacobian(m, pi, &o, it);
         if (rc) break;
         solve.assign(Kn, 0);
         for (int lc = 0; lc < Kn; lc++) solve[lc] = active[u.cases[lc]];
-        rc = run_linsolve_batch(m, &o.lin, solve, iters, fres);
+        rc = run_linsolve_batch(m, &o.lin, solve, iters, fres, lstatus);
+        if (rc) break;
+        for (int lc = 0; !rc && lc < Kn; lc++)
+            if (lstatus[lc] && active[u.cases[lc]]) rc = park(lc, lstatus[lc], it);
         if (rc) break;
         act_i.assign(Kn, 0);
         int n_active = 0, kmax = 0;

@@ -42,16 +42,45 @@ int cfd_hip_check(cfd_ctx *ctx, hipError_t e, const char *what) {
     return cfd_set_error(ctx, e == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+std::shared_mutex &cfd_capture_mutex() {
+    static std::shared_mutex mu;
+    return mu;
+}
+
+hipError_t cfd_memcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    CfdRuntimeGuard g;
+    return hipMemcpy(dst, src, bytes, kind);
+}
+
+hipError_t cfd_host_alloc(void **p, size_t bytes) {
+    CfdRuntimeGuard g;
+    return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
+
 void *cfd_dalloc(cfd_mesh *m, size_t bytes) {
     void *p = nullptr;
     if (bytes == 0) bytes = 8;
+    CfdRuntimeGuard g;
     if (hipMalloc(&p, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
     }
-    m->allocs.push_back(p);
+    m->allocs.emplace_back(p, bytes);
     m->bytes += (int64_t)bytes;
     return p;
+}
+
+void cfd_dfree(cfd_mesh *m, void *p) {
+    if (!p) return;
+    for (size_t k = 0; k < m->allocs.size(); k++) {
+        if (m->allocs[k].first != p) continue;
+        m->bytes -= (int64_t)m->allocs[k].second;
+        m->allocs[k] = m->allocs.back();
+        m->allocs.pop_back();
+        CfdRuntimeGuard g;
+        hipFree(p);
+        return;
+    }
 }
 
 extern "C" int cfd_ctx_create(int device, cfd_ctx **out) {
@@ -199,7 +228,7 @@ static T *upload(cfd_mesh *m, const T *h, size_t n, int *rc) {
         *rc = cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory");
         return nullptr;
     }
-    if (n && hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    if (n && cfd_memcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess) {
         *rc = cfd_set_error(m->ctx, CFD_E_HIP, "hipMemcpy failed");
         return nullptr;
     }
@@ -221,7 +250,8 @@ extern "C" void cfd_mesh_destroy(cfd_mesh *m) {
     hipSetDevice(m->ctx->device);
     if (m->stream) hipStreamSynchronize(m->stream);
     drop_krylov_graphs(m);
-    for (void *p : m->allocs) hipFree(p);
+    CfdRuntimeGuard g;
+    for (auto &a : m->allocs) hipFree(a.first);
     if (m->hred) hipHostFree(m->hred);
     if (m->hflag) hipHostFree(m->hflag);
     if (m->hbred) hipHostFree(m->hbred);
@@ -299,7 +329,12 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         tph = t;
     };
     try {
-        if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+        hipError_t se;
+        {
+            CfdRuntimeGuard g;
+            se = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+        }
+        if (se != hipSuccess) {
             delete m;
             return cfd_set_error(ctx, CFD_E_HIP, "hipStreamCreate failed");
         }
@@ -771,8 +806,8 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             cfd_mesh_destroy(m);
             return cfd_set_error(ctx, CFD_E_OOM, "cfd_mesh_create: out of device memory");
         }
-        if (hipHostMalloc((void **)&m->hred, sizeof(double) * 4 * 65, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&m->hflag, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess) {
+        if (cfd_host_alloc((void **)&m->hred, sizeof(double) * 4 * 65) != hipSuccess ||
+            cfd_host_alloc((void **)&m->hflag, sizeof(int) * 4) != hipSuccess) {
             cfd_mesh_destroy(m);
             return cfd_set_error(ctx, CFD_E_OOM, "cfd_mesh_create: pinned allocation failed");
         }
@@ -867,6 +902,139 @@ int cfd_ensure_fp32(cfd_mesh *m) {
 }
 
 // ---------------------------------------------------------------------------
+// Work-edge patches of the flux-once residual (CFD_ARITH_FAST).  Built from
+// the device incidence lists: a node's incidence with neighbour j is an
+// interior edge (node is L when nbr >= 0, R when nbr = ~j) or boundary edge b
+// (nbr = N + b), with the edge's area vector.  An edge inside a patch becomes
+// one work edge (taken from its L end); an edge between patches becomes a
+// work edge of each; a boundary edge one of its node's patch.  Work edges of
+// a patch are ordered by (first end in the patch, L, R), and each node lists
+// the slots of its incidences in ascending order (sign bit: the node is R).
+// Patches hold kPatch consecutive nodes, restarting at every case of a union.
+// ---------------------------------------------------------------------------
+constexpr int kPatch = 256;  // nodes per patch: one per thread of the patch kernel's workgroup
+
+int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
+    if (!m->pw_ptr) {
+        const int N = m->N, nsl = (N + kSlice - 1) / kSlice;
+        // patches of kPatch nodes from the start of the mesh, or of each case
+        // of a union (so a case's patches, and its sums, are those it has alone)
+        std::vector<int> pstart;
+        const std::vector<int> whole{0, N};
+        const std::vector<int> &cut = m->ncase ? m->case_node : whole;
+        for (size_t c = 0; c + 1 < cut.size(); c++)
+            for (int n = cut[c]; n < cut[c + 1]; n += kPatch) pstart.push_back(n);
+        pstart.push_back(N);
+        const int P = (int)pstart.size() - 1;
+        std::vector<int> ip(N + 1), sptr(nsl + 1), nbr(m->nsell);
+        std::vector<double> sx(m->nsell), sy(m->nsell), sz(m->nsell);
+        CFD_HIP(m, hipStreamSynchronize(m->stream));
+        CFD_HIP(m, cfd_memcpy(ip.data(), m->inc_ptr, sizeof(int) * ip.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(sptr.data(), m->sptr, sizeof(int) * sptr.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(nbr.data(), m->nbr, sizeof(int) * nbr.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(sx.data(), m->d.isx, sizeof(double) * sx.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(sy.data(), m->d.isy, sizeof(double) * sy.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(sz.data(), m->d.isz, sizeof(double) * sz.size(), hipMemcpyDeviceToHost));
+        auto sell = [&](int i, int k) { return (size_t)sptr[i / kSlice] + (size_t)kSlice * k + i % kSlice; };
+        // work edges of each patch: (key, l, r, incidence of the area vector)
+        struct Work {
+            int key, l, r;
+            size_t inc;
+        };
+        std::vector<std::vector<Work>> pw(P);
+        cfd_parallel_for(
+            P,
+            [&](int64_t p0, int64_t p1) {
+                for (int p = (int)p0; p < (int)p1; p++) {
+                    const int n0 = pstart[p], n1 = pstart[p + 1];
+                    std::vector<Work> &w = pw[p];
+                    for (int i = n0; i < n1; i++) {
+                        for (int k = 0; k < ip[i + 1] - ip[i]; k++) {
+                            const size_t x = sell(i, k);
+                            const int jr = nbr[x];
+                            if (jr >= N) {
+                                w.push_back({i - n0, i, jr, x});
+                                continue;
+                            }
+                            const bool isL = jr >= 0;
+                            const int j = isL ? jr : ~jr;
+                            const bool inside = j >= n0 && j < n1;
+                            if (inside && !isL) continue;  // taken from its L end
+                            const int key = inside ? std::min(i, j) - n0 : i - n0;
+                            w.push_back({key, isL ? i : j, isL ? j : i, x});
+                        }
+                    }
+                    std::sort(w.begin(), w.end(), [](const Work &a, const Work &b) {
+                        return a.key != b.key ? a.key < b.key : (a.l != b.l ? a.l < b.l : a.r < b.r);
+                    });
+                }
+            },
+            1);
+        std::vector<int> wptr(P + 1, 0);
+        for (int p = 0; p < P; p++) wptr[p + 1] = wptr[p] + (int)pw[p].size();
+        const int64_t W = wptr[P];
+        std::vector<int> wl(W), wr(W), slot(m->nsell, 0);
+        std::vector<double> ws(4 * (size_t)W);
+        bool ok = true;
+        std::vector<char> bad(P, 0);
+        cfd_parallel_for(
+            P,
+            [&](int64_t p0, int64_t p1) {
+                for (int p = (int)p0; p < (int)p1; p++) {
+                    const int n0 = pstart[p], n1 = pstart[p + 1];
+                    std::vector<std::vector<int>> lists(n1 - n0);
+                    for (int q = 0; q < (int)pw[p].size(); q++) {
+                        const Work &u = pw[p][q];
+                        const size_t o = (size_t)wptr[p] + q;
+                        wl[o] = u.l;
+                        wr[o] = u.r;
+                        ws[4 * o] = sx[u.inc];
+                        ws[4 * o + 1] = sy[u.inc];
+                        ws[4 * o + 2] = sz[u.inc];
+                        ws[4 * o + 3] = 0.0;
+                        if (u.l >= n0 && u.l < n1) lists[u.l - n0].push_back(q);
+                        if (u.r >= n0 && u.r < n1) lists[u.r - n0].push_back(q | (int)0x80000000u);
+                    }
+                    for (int i = n0; i < n1; i++) {
+                        const std::vector<int> &li = lists[i - n0];  // ascending: work edges are visited in order
+                        if ((int)li.size() != ip[i + 1] - ip[i]) {
+                            bad[p] = 1;
+                            continue;
+                        }
+                        for (int k = 0; k < (int)li.size(); k++) slot[sell(i, k)] = li[k];
+                    }
+                }
+            },
+            1);
+        for (char b : bad) ok = ok && !b;
+        if (!ok) return cfd_set_error(m->ctx, CFD_E_STATE, "work-edge patches do not cover the incidence lists");
+        int rc = 0;
+        m->pw_ptr = upload(m, wptr.data(), wptr.size(), &rc);
+        if (!rc) m->pn_start = upload(m, pstart.data(), pstart.size(), &rc);
+        if (!rc) m->pw_l = upload(m, wl.data(), wl.size(), &rc);
+        if (!rc) m->pw_r = upload(m, wr.data(), wr.size(), &rc);
+        if (!rc) m->pn_slot = upload(m, slot.data(), slot.size(), &rc);
+        if (!rc) m->d.ws = upload(m, ws.data(), ws.size(), &rc);
+        if (rc) {
+            m->pw_ptr = nullptr;
+            return rc;
+        }
+        m->npatch = P;
+        m->npw = W;
+    }
+    if (fp32 && !m->f.ws) {
+        std::vector<double> ws(4 * (size_t)m->npw);
+        CFD_HIP(m, hipStreamSynchronize(m->stream));
+        CFD_HIP(m, cfd_memcpy(ws.data(), m->d.ws, sizeof(double) * ws.size(), hipMemcpyDeviceToHost));
+        std::vector<float> wf(ws.begin(), ws.end());
+        int rc = 0;
+        m->f.ws = upload(m, wf.data(), wf.size(), &rc);
+        if (rc) return rc;
+    }
+    return CFD_OK;
+}
+
+// ---------------------------------------------------------------------------
 // state reset (for reuse of a resident mesh by another solve)
 // ---------------------------------------------------------------------------
 static int check_params(cfd_mesh *m, const cfd_params *p);
@@ -915,29 +1083,29 @@ extern "C" int cfd_field_set(cfd_mesh *m, int field, const double *h) {
         const size_t rec = isq ? kQA : kQB, n = isq ? NT : N;
         double *dev = isq ? m->d.qa : m->d.qb;
         std::vector<double> buf(rec * n);
-        CFD_HIP(m, hipMemcpy(buf.data(), dev, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(buf.data(), dev, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
         const int nf = field == CFD_FIELD_GRAD ? 15 : 5, o = field == CFD_FIELD_PHI ? kPHI : 0;
         for (int f = 0; f < nf; f++)
             for (size_t i = 0; i < n; i++) buf[rec * i + o + f] = h[f * n + api_index(m, i)];
-        CFD_HIP(m, hipMemcpy(dev, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+        CFD_HIP(m, cfd_memcpy(dev, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
         m->q_version++;
         return CFD_OK;
     }
     if (field == CFD_FIELD_RES_CONV || field == CFD_FIELD_RES_DISS || field == CFD_FIELD_DT_ACC) {
         std::vector<double> buf((size_t)kRES * N);
-        CFD_HIP(m, hipMemcpy(buf.data(), m->d.res, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(buf.data(), m->d.res, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
         const int nf = field == CFD_FIELD_DT_ACC ? 1 : 5;
         const int o = field == CFD_FIELD_RES_CONV ? 0 : (field == CFD_FIELD_RES_DISS ? 5 : 10);
         for (int f = 0; f < nf; f++)
             for (size_t i = 0; i < N; i++) buf[kRES * i + o + f] = h[f * N + api_index(m, i)];
-        CFD_HIP(m, hipMemcpy(m->d.res, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+        CFD_HIP(m, cfd_memcpy(m->d.res, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
         m->res_prec = CFD_FP64;
         return CFD_OK;
     }
     if (field == CFD_FIELD_DT) {
         std::vector<double> buf(N);
         for (size_t i = 0; i < N; i++) buf[i] = h[api_index(m, i)];
-        CFD_HIP(m, hipMemcpy(m->dt, buf.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+        CFD_HIP(m, cfd_memcpy(m->dt, buf.data(), sizeof(double) * N, hipMemcpyHostToDevice));
         return CFD_OK;
     }
     double *dev = nullptr;
@@ -947,7 +1115,7 @@ extern "C" int cfd_field_set(cfd_mesh *m, int field, const double *h) {
     case CFD_FIELD_X: dev = m->x; n = 5 * N; break;
     default: return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_field_set: unknown field %d", field);
     }
-    CFD_HIP(m, hipMemcpy(dev, h, sizeof(double) * n, hipMemcpyHostToDevice));
+    CFD_HIP(m, cfd_memcpy(dev, h, sizeof(double) * n, hipMemcpyHostToDevice));
     return CFD_OK;
 }
 
@@ -975,7 +1143,7 @@ extern "C" int cfd_field_get(cfd_mesh *m, int field, double *h) {
         if (!isq && m->f_ready && m->res_prec == CFD_FP32) {
             // not reached: FP32 fields are exposed through the FP64 mirror only
         }
-        CFD_HIP(m, hipMemcpy(buf.data(), isq ? m->d.qa : m->d.qb, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(buf.data(), isq ? m->d.qa : m->d.qb, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
         const int nf = field == CFD_FIELD_GRAD ? 15 : 5, o = field == CFD_FIELD_PHI ? kPHI : 0;
         for (int f = 0; f < nf; f++)
             for (size_t i = 0; i < n; i++) h[f * n + api_index(m, i)] = buf[rec * i + o + f];
@@ -983,7 +1151,7 @@ extern "C" int cfd_field_get(cfd_mesh *m, int field, double *h) {
     }
     if (field == CFD_FIELD_RES_CONV || field == CFD_FIELD_RES_DISS || field == CFD_FIELD_DT_ACC) {
         std::vector<double> buf((size_t)kRES * N);
-        CFD_HIP(m, hipMemcpy(buf.data(), m->d.res, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(buf.data(), m->d.res, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
         const int nf = field == CFD_FIELD_DT_ACC ? 1 : 5;
         const int o = field == CFD_FIELD_RES_CONV ? 0 : (field == CFD_FIELD_RES_DISS ? 5 : 10);
         for (int f = 0; f < nf; f++)
@@ -992,7 +1160,7 @@ extern "C" int cfd_field_get(cfd_mesh *m, int field, double *h) {
     }
     if (field == CFD_FIELD_DT) {
         std::vector<double> buf(N);
-        CFD_HIP(m, hipMemcpy(buf.data(), m->dt, sizeof(double) * N, hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(buf.data(), m->dt, sizeof(double) * N, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < N; i++) h[api_index(m, i)] = buf[i];
         return CFD_OK;
     }
@@ -1003,7 +1171,7 @@ extern "C" int cfd_field_get(cfd_mesh *m, int field, double *h) {
     case CFD_FIELD_X: dev = m->x; n = 5 * N; break;
     default: return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_field_get: unknown field %d", field);
     }
-    CFD_HIP(m, hipMemcpy(h, dev, sizeof(double) * n, hipMemcpyDeviceToHost));
+    CFD_HIP(m, cfd_memcpy(h, dev, sizeof(double) * n, hipMemcpyDeviceToHost));
     return CFD_OK;
 }
 
@@ -1016,7 +1184,7 @@ static int check_params(cfd_mesh *m, const cfd_params *p) {
     if (!m || !p) return CFD_E_ARG;
     if (p->flux < 0 || p->flux > 2 || (p->order != 1 && p->order != 2) || p->limiter < 0 || p->limiter > 2 ||
         p->gradient < 0 || p->gradient > 1 || (p->precision != CFD_FP64 && p->precision != CFD_FP32) ||
-        !(p->gamma > 1.0))
+        (p->arith != CFD_ARITH_EXACT && p->arith != CFD_ARITH_FAST) || !(p->gamma > 1.0))
         return cfd_set_error(m->ctx, CFD_E_ARG, "invalid cfd_params");
     hipSetDevice(m->ctx->device);
     return CFD_OK;
@@ -1052,34 +1220,31 @@ extern "C" int cfd_bc(cfd_mesh *m, const cfd_params *p, int iter) {
     return CFD_OK;
 }
 
+// Runs the launcher of the build p->arith selects (res_exact or res_fast)
+// on the buffers of p->precision.
+#define CFD_RES_DISPATCH(m, p, launcher, ...)                                                             \
+    ((p)->precision == CFD_FP32                                                                           \
+         ? ((p)->arith == CFD_ARITH_FAST ? res_fast::launcher<float>((m), (m)->f, (p), ##__VA_ARGS__)      \
+                                         : res_exact::launcher<float>((m), (m)->f, (p), ##__VA_ARGS__))    \
+         : ((p)->arith == CFD_ARITH_FAST ? res_fast::launcher<double>((m), (m)->d, (p), ##__VA_ARGS__)     \
+                                         : res_exact::launcher<double>((m), (m)->d, (p), ##__VA_ARGS__)))
+
 extern "C" int cfd_gradient(cfd_mesh *m, const cfd_params *p) {
     int rc = check_params(m, p);
-    if (rc) return rc;
-    if (p->precision == CFD_FP32) {
-        rc = cfd_ensure_fp32(m);
-        return rc ? rc : launch_gradient<float>(m, m->f, p);
-    }
-    return launch_gradient<double>(m, m->d, p);
+    if (!rc && p->precision == CFD_FP32) rc = cfd_ensure_fp32(m);
+    return rc ? rc : CFD_RES_DISPATCH(m, p, launch_gradient);
 }
 
 extern "C" int cfd_limiter(cfd_mesh *m, const cfd_params *p) {
     int rc = check_params(m, p);
-    if (rc) return rc;
-    if (p->precision == CFD_FP32) {
-        rc = cfd_ensure_fp32(m);
-        return rc ? rc : launch_limiter<float>(m, m->f, p);
-    }
-    return launch_limiter<double>(m, m->d, p);
+    if (!rc && p->precision == CFD_FP32) rc = cfd_ensure_fp32(m);
+    return rc ? rc : CFD_RES_DISPATCH(m, p, launch_limiter);
 }
 
 extern "C" int cfd_gradient_limiter(cfd_mesh *m, const cfd_params *p) {
     int rc = check_params(m, p);
-    if (rc) return rc;
-    if (p->precision == CFD_FP32) {
-        rc = cfd_ensure_fp32(m);
-        return rc ? rc : launch_gradient_limiter<float>(m, m->f, p);
-    }
-    return launch_gradient_limiter<double>(m, m->d, p);
+    if (!rc && p->precision == CFD_FP32) rc = cfd_ensure_fp32(m);
+    return rc ? rc : CFD_RES_DISPATCH(m, p, launch_gradient_limiter);
 }
 
 extern "C" int cfd_residual(cfd_mesh *m, const cfd_params *p, int add_time, double *eig6) {
@@ -1097,14 +1262,9 @@ extern "C" int cfd_residual(cfd_mesh *m, const cfd_params *p, int add_time, doub
         CFD_HIP(m, hipMemcpyAsync(m->eigbits, init, sizeof(init), hipMemcpyHostToDevice, m->stream));
     }
     CFD_HIP(m, hipMemsetAsync(m->flag, 0, 4 * sizeof(int), m->stream));
-    if (p->precision == CFD_FP32) {
-        rc = cfd_ensure_fp32(m);
-        if (!rc) rc = launch_residual<float>(m, m->f, p, stats);
-        m->res_prec = CFD_FP32;
-    } else {
-        rc = launch_residual<double>(m, m->d, p, stats);
-        m->res_prec = CFD_FP64;
-    }
+    if (p->precision == CFD_FP32) rc = cfd_ensure_fp32(m);
+    if (!rc) rc = CFD_RES_DISPATCH(m, p, launch_residual, stats);
+    m->res_prec = p->precision == CFD_FP32 ? CFD_FP32 : CFD_FP64;
     if (rc) return rc;
     m->residual_valid = true;
     m->hllc_check = p->flux == CFD_FLUX_HLLC;
@@ -1227,6 +1387,9 @@ extern "C" int cfd_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int
     if (m->hflag[0])
         return cfd_set_error(m->ctx, CFD_E_NEGATIVE_PRESSURE, "cfd_jacobian_fd: %d perturbed ghost states with "
                              "negative pressure", m->hflag[0]);
+    if (m->hflag[1])
+        return cfd_set_error(m->ctx, CFD_E_HLLC, "cfd_jacobian_fd: HLLC without a wave configuration in %d perturbed "
+                             "fluxes", m->hflag[1]);
     return CFD_OK;
 }
 
@@ -1246,13 +1409,13 @@ extern "C" int cfd_jacobian_get(cfd_mesh *m, double *a, double *b) {
     if (a) {
         if (m->jac_mixed) {
             std::vector<float> tmp(25 * (size_t)m->nnz);
-            CFD_HIP(m, hipMemcpy(tmp.data(), m->af, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost));
+            CFD_HIP(m, cfd_memcpy(tmp.data(), m->af, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost));
             for (size_t i = 0; i < tmp.size(); i++) a[i] = tmp[i];
         } else {
-            CFD_HIP(m, hipMemcpy(a, m->a, sizeof(double) * 25 * (size_t)m->nnz, hipMemcpyDeviceToHost));
+            CFD_HIP(m, cfd_memcpy(a, m->a, sizeof(double) * 25 * (size_t)m->nnz, hipMemcpyDeviceToHost));
         }
     }
-    if (b) CFD_HIP(m, hipMemcpy(b, m->b, sizeof(double) * 5 * (size_t)m->N, hipMemcpyDeviceToHost));
+    if (b) CFD_HIP(m, cfd_memcpy(b, m->b, sizeof(double) * 5 * (size_t)m->N, hipMemcpyDeviceToHost));
     return CFD_OK;
 }
 
@@ -1262,8 +1425,8 @@ extern "C" int cfd_jacobian_set(cfd_mesh *m, const double *a, const double *b) {
     int rc = ensure_jacobian_storage(m, 0);
     if (rc) return rc;
     CFD_HIP(m, hipStreamSynchronize(m->stream));
-    CFD_HIP(m, hipMemcpy(m->a, a, sizeof(double) * 25 * (size_t)m->nnz, hipMemcpyHostToDevice));
-    CFD_HIP(m, hipMemcpy(m->b, b, sizeof(double) * 5 * (size_t)m->N, hipMemcpyHostToDevice));
+    CFD_HIP(m, cfd_memcpy(m->a, a, sizeof(double) * 25 * (size_t)m->nnz, hipMemcpyHostToDevice));
+    CFD_HIP(m, cfd_memcpy(m->b, b, sizeof(double) * 5 * (size_t)m->N, hipMemcpyHostToDevice));
     m->jac_valid = true;
     m->jac_mixed = false;
     m->prec_valid = false;
@@ -1304,11 +1467,25 @@ extern "C" int cfd_forces(cfd_mesh *m, const cfd_params *p, int wall_tag, double
     return CFD_OK;
 }
 
+// The Jacobian of one Newton step: first-order flux, V/dt on the diagonal,
+// B = -(Conv + Diss); approximate Roe or finite differences of the scheme's
+// own flux (o->jacobian).
+int solve_jacobian(cfd_mesh *m, const cfd_params &p, const cfd_solve_opts *o, int it) {
+    if (o->jacobian == CFD_SOLVE_JAC_FD) {
+        cfd_params p1 = p;
+        p1.order = 1;
+        return cfd_jacobian_fd(m, &p1, CFD_JAC_FD_CENTRAL, it, 1, 1, o->lin.mixed);
+    }
+    return cfd_jacobian(m, &p, 1, o->lin.mixed);
+}
+
 extern "C" int cfd_solve_steady(cfd_mesh *m, const cfd_params *p0, const cfd_solve_opts *o, cfd_solve_result *res,
                                 double *rms_hist) {
     int rc = check_params(m, p0);
     if (rc) return rc;
     if (!o || !res) return CFD_E_ARG;
+    if (o->jacobian != CFD_SOLVE_JAC_APPROX && o->jacobian != CFD_SOLVE_JAC_FD)
+        return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_solve_steady: unknown Jacobian %d", o->jacobian);
     const auto t0 = std::chrono::steady_clock::now();
     *res = cfd_solve_result{};
     cfd_params p = *p0;
@@ -1340,7 +1517,7 @@ extern "C" int cfd_solve_steady(cfd_mesh *m, const cfd_params *p0, const cfd_sol
             break;
         }
         rc = cfd_delta_t(m, &pi, it, nullptr);
-        if (!rc) rc = cfd_jacobian(m, &pi, 1, o->lin.mixed);
+        if (!rc) rc = solve_jacobian(m, pi, o, it);
         if (rc) break;
         cfd_linsolve_stats st{};
         rc = cfd_linsolve(m, &o->lin, &st);

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -41,6 +42,7 @@ struct ResidualBufs {
     T *vol = nullptr;
     T *lsq = nullptr;
     T *res = nullptr;
+    T *ws = nullptr;  // [W][4] area vector of each work edge of the flux-once residual
 };
 constexpr int kQA = 8;
 constexpr int kQB = 24;
@@ -140,7 +142,7 @@ struct cfd_mesh {
     hipStream_t stream = nullptr;
     int N = 0, E = 0, B = 0, NT = 0;
     int64_t bytes = 0;
-    std::vector<void *> allocs;
+    std::vector<std::pair<void *, size_t>> allocs;  // device allocations and their sizes
 
     // topology
     int *order = nullptr;     // [N] reference index of each internal node (internal = Morton order)
@@ -163,6 +165,20 @@ struct cfd_mesh {
     int *mark = nullptr;      // [N] limiter correction marks
     int *risk = nullptr;      // [1] nodes with a near-vacuum face state
     int mark_epoch = 0;
+
+    // Work-edge patches of the flux-once residual (CFD_ARITH_FAST), built on
+    // first use: patch p holds the internal nodes [pn_start[p], pn_start[p+1])
+    // (at most 256, never across cases of a union) and its work edges
+    // pw_ptr[p] .. pw_ptr[p+1]: each interior edge with both ends in the
+    // patch once, each edge to another patch, each boundary edge of a patch
+    // node, ordered by their first end in the patch.
+    int npatch = 0;
+    int64_t npw = 0;
+    int *pn_start = nullptr;  // [npatch + 1]
+    int *pw_ptr = nullptr;    // [npatch + 1]
+    int *pw_l = nullptr;      // [npw] internal index of the L end
+    int *pw_r = nullptr;      // [npw] internal index of the R end, or N + b for boundary edge b
+    int *pn_slot = nullptr;   // SELL as nbr: patch-local slot of each incidence, ascending per node; bit 31: R end
 
     ResidualBufs<double> d;   // FP64
     ResidualBufs<float> f;    // FP32 (allocated on first use)
@@ -216,8 +232,7 @@ struct cfd_mesh {
     // Krylov workspace
     bool use_graphs = true;   // replay captured Arnoldi steps
     KrylovGraphs kgraph;
-    int kry_m = 0;
-    bool kry_mixed = false;
+    int kry_m64 = 0, kry_m32 = 0;  // restart length the FP64 / FP32 Krylov basis holds
     double *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr;
     float *Vf = nullptr, *Zf = nullptr, *wf = nullptr;
     // reductions
@@ -241,6 +256,10 @@ struct cfd_mesh {
     int *bcase = nullptr;                    // device [B] case of each boundary edge
     int *ecase = nullptr;                    // device [N] case of each node
     int *case_active = nullptr;              // device [ncase] 1 while a case still iterates
+    // device [ncase] failure bits of each case, raised by the kernels that
+    // flag a failure for the whole mesh: 1 negative ghost pressure, 2 no
+    // boundary-condition instance applies, 4 no HLLC wave configuration
+    int *case_flag = nullptr;
     SegMap seg_rms, seg_dot, seg_forces;
     double *bpart = nullptr;                 // device partial sums of segmented reductions
     double *bred = nullptr, *hbred = nullptr;  // per-case results [ncase][kBOut] (device, pinned host)
@@ -256,8 +275,27 @@ int cfd_set_error(cfd_ctx *ctx, int code, const char *fmt, ...);
 int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params *p, const int *node_case,
                           cfd_mesh **out);
 int cfd_hip_check(cfd_ctx *ctx, hipError_t e, const char *what);
+// Stream capture beside other threads.  HIP (as CUDA) counts device and
+// pinned-host allocation and free, stream creation and destruction and
+// synchronous copies and fills as unsafe while any stream of the process is
+// being captured; the relaxed capture mode only stops it from rejecting
+// them.  Worker threads of the solve server drive meshes concurrently, so the
+// library holds this lock shared around every such call and exclusively
+// around each capture (begin, launches, end, instantiate): no unsafe call of
+// the library overlaps a capture of another of its threads.
+std::shared_mutex &cfd_capture_mutex();
+struct CfdRuntimeGuard {
+    std::shared_lock<std::shared_mutex> lk{cfd_capture_mutex()};
+};
+hipError_t cfd_memcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind);  // hipMemcpy, guarded
+hipError_t cfd_host_alloc(void **p, size_t bytes);                                     // hipHostMalloc, guarded
 void *cfd_dalloc(cfd_mesh *m, size_t bytes);  // device allocation tracked on the mesh
+void cfd_dfree(cfd_mesh *m, void *p);          // frees and untracks an allocation of cfd_dalloc (NULL: nothing)
 int cfd_ensure_fp32(cfd_mesh *m);
+// Builds the work-edge patches of the flux-once residual from the device
+// incidence lists (once per mesh) and the FP32 copy of their area vectors
+// when fp32 is set.
+int cfd_ensure_patches(cfd_mesh *m, bool fp32);
 
 #define CFD_HIP(m, call)                                                   \
     do {                                                                   \
@@ -285,17 +323,24 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 // ---------------------------------------------------------------------------
 // kernel launchers implemented in residual.hip / linsolve.hip
 // ---------------------------------------------------------------------------
+// residual.hip is compiled twice: res_exact holds every launcher (the
+// reference's operation order, CFD_ARITH_EXACT); res_fast
+// (residual_fast.hip: FMA contraction, shared reciprocals, CFD_ARITH_FAST)
+// holds the gradient, limiter and residual launchers.
+#define CFD_RES_LAUNCHERS                                                             \
+    template <class T>                                                                \
+    int launch_gradient(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);        \
+    template <class T>                                                                \
+    int launch_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);         \
+    template <class T>                                                                \
+    int launch_gradient_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p); \
+    template <class T>                                                                \
+    int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats);
+namespace res_exact {
+CFD_RES_LAUNCHERS
 int launch_lsq_coef(cfd_mesh *m);
 template <class T>
 int launch_bc(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, int iter);
-template <class T>
-int launch_gradient(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);
-template <class T>
-int launch_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);
-template <class T>
-int launch_gradient_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p);
-template <class T>
-int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats);
 int launch_delta_t(cfd_mesh *m, double cfl);
 int launch_rms(cfd_mesh *m, double *sums5);
 int launch_update(cfd_mesh *m, double relax);
@@ -303,8 +348,16 @@ int launch_forces(cfd_mesh *m, const cfd_params *p, int wall_tag, double *out4);
 int launch_convert_state_to_fp32(cfd_mesh *m);
 int launch_convert_geometry_fp32(cfd_mesh *m);
 int launch_residual_to_fp64(cfd_mesh *m);
+}  // namespace res_exact
+namespace res_fast {
+CFD_RES_LAUNCHERS
+}  // namespace res_fast
+#undef CFD_RES_LAUNCHERS
+using namespace res_exact;
 
 int launch_jacobian(cfd_mesh *m, const cfd_params *p, int rhs_mode, int mixed);
+// The Jacobian of one step of the steady loop (cfd_solve_steady, batches).
+int solve_jacobian(cfd_mesh *m, const cfd_params &p, const cfd_solve_opts *o, int it);
 // batched solves (batch.hip): cases that may share one union mesh, and the
 // solve itself with each case's final state copied to q_out[c] (may be NULL);
 // allow_graphs false launches the Arnoldi steps directly (callers whose other
@@ -323,7 +376,9 @@ int launch_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, i
                        int mixed);
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);
 // Batched FGMRES over a union mesh: solves the cases with solve[c] != 0,
-// writes each case's iterations and final residual estimate.
+// writes each case's iterations, final residual estimate and status (a case
+// whose Krylov residual turns non-finite gets CFD_E_NONFINITE and no
+// correction; the others are unaffected).
 int run_linsolve_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const std::vector<char> &solve,
-                       std::vector<int> &iters, std::vector<double> &final_res);
+                       std::vector<int> &iters, std::vector<double> &final_res, std::vector<int> &status);
 void drop_krylov_graphs(cfd_mesh *m);

@@ -11,7 +11,20 @@
 
 #include <hip/hip_runtime.h>
 
+// CFD_FAST_MATH 1 (residual_fast.hip, compiled with FMA contraction): each
+// shared denominator is inverted once and multiplied, and the 5x5 products
+// skip the transformation matrix's structural zeros.  The functions of each
+// build live in their own inline namespace, so both builds link together.
+#ifndef CFD_FAST_MATH
+#define CFD_FAST_MATH 0
+#endif
+
 namespace cfd {
+#if CFD_FAST_MATH
+inline namespace fast_math {
+#else
+inline namespace exact_math {
+#endif
 
 template <class T>
 struct Eos {
@@ -27,6 +40,17 @@ __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }  // MAX, U
 template <class T>
 __device__ __forceinline__ void eos_face(T g, const T *Q, T nx, T ny, T nz, Eos<T> &e) {
     e.rho = Q[0];
+#if CFD_FAST_MATH
+    const T ir = T(1.0) / e.rho;
+    e.u = Q[1] * ir;
+    e.v = Q[2] * ir;
+    e.w = Q[3] * ir;
+    e.et = Q[4] * ir;
+    e.q2 = e.u * e.u + e.v * e.v + e.w * e.w;
+    e.p = (g - T(1.0)) * (Q[4] - T(0.5) * e.rho * e.q2);
+    e.ht = (Q[4] + e.p) * ir;
+    e.c = sqrt(g * e.p * ir);
+#else
     e.u = Q[1] / e.rho;
     e.v = Q[2] / e.rho;
     e.w = Q[3] / e.rho;
@@ -35,6 +59,7 @@ __device__ __forceinline__ void eos_face(T g, const T *Q, T nx, T ny, T nz, Eos<
     e.p = (g - T(1.0)) * (e.rho * e.et - T(0.5) * e.rho * e.q2);
     e.ht = (e.rho * e.et + e.p) / e.rho;
     e.c = sqrt((g * e.p) / e.rho);
+#endif
     e.ubar = e.u * nx + e.v * ny + e.w * nz;
 }
 
@@ -46,9 +71,16 @@ __device__ __forceinline__ T area_normal(T sx, T sy, T sz, T &nx, T &ny, T &nz) 
     ny = sy;
     nz = sz;
     if (mag > T(1.0e-20)) {
+#if CFD_FAST_MATH
+        const T im = T(1.0) / mag;
+        nx = sx * im;
+        ny = sy * im;
+        nz = sz * im;
+#else
         nx = sx / mag;
         ny = sy / mag;
         nz = sz / mag;
+#endif
     }
     return mag;
 }
@@ -103,15 +135,37 @@ __device__ __forceinline__ T roe_flux(T g, int efix, T sx, T sy, T sz, const T *
     const T ubar = u * nx + v * ny + w * nz;
     roe_eigen(efix, L.ubar, L.c, R.ubar, R.c, ubar, c, E0, E1, E2);
     const T maxlambda = tmax(E0, tmax(E1, E2));
-    // M = Compute_Transformation_Conservative_To_RUP, Euler3D/src/Material.cpp:1172-1235
     const T phi = T(0.5) * (g - T(1.0)) * (u * u + v * v + w * w);
+    const T dQ0 = R.rho - L.rho, dQ1 = R.u - L.u, dQ2 = R.v - L.v, dQ3 = R.w - L.w, dQ4 = R.p - L.p;
+    const T dU = dQ1 * nx + dQ2 * ny + dQ3 * nz;
+#if CFD_FAST_MATH
+    {
+        // the same dissipation with 1/c, 1/(rho c) and 1/(g - 1) formed once,
+        // and M dw over M's nonzero entries only
+        const T ic = T(1.0) / c, irc = T(1.0) / (rho * c), igm1 = T(1.0) / (g - T(1.0));
+        const T e12 = (E1 + E2) - T(2.0) * E0, d12 = E1 - E2;
+        const T dw0 = E0 * dQ0 + T(0.5) * e12 * dQ4 * ic * ic + T(0.5) * rho * d12 * dU * ic;
+        const T a = T(0.5) * dU * e12, b = T(0.5) * d12 * dQ4 * irc;
+        const T dw1 = E0 * dQ1 + nx * a + nx * b;
+        const T dw2 = E0 * dQ2 + ny * a + ny * b;
+        const T dw3 = E0 * dQ3 + nz * a + nz * b;
+        const T dw4 = T(0.5) * dQ4 * (E1 + E2) + T(0.5) * rho * c * dU * d12;
+        const T fa[5] = {dw0, u * dw0 + rho * dw1, v * dw0 + rho * dw2, w * dw0 + rho * dw3,
+                         phi * igm1 * dw0 + rho * (u * dw1 + v * dw2 + w * dw3) + igm1 * dw4};
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            conv[i] = T(0.5) * (fL[i] + fR[i]) * area;
+            diss[i] = T(-0.5) * fa[i] * area;
+        }
+        return area * maxlambda;
+    }
+#endif
+    // M = Compute_Transformation_Conservative_To_RUP, Euler3D/src/Material.cpp:1172-1235
     const T M[5][5] = {{T(1.0), T(0.0), T(0.0), T(0.0), T(0.0)},
                        {u, rho, T(0.0), T(0.0), T(0.0)},
                        {v, T(0.0), rho, T(0.0), T(0.0)},
                        {w, T(0.0), T(0.0), rho, T(0.0)},
                        {phi / (g - T(1.0)), rho * u, rho * v, rho * w, T(1.0) / (g - T(1.0))}};
-    const T dQ0 = R.rho - L.rho, dQ1 = R.u - L.u, dQ2 = R.v - L.v, dQ3 = R.w - L.w, dQ4 = R.p - L.p;
-    const T dU = dQ1 * nx + dQ2 * ny + dQ3 * nz;
     T dw[5];
     dw[0] = E0 * dQ0 + (E1 + E2 - T(2.0) * E0) * dQ4 / (T(2.0) * c * c) + rho * (E1 - E2) * dU / (T(2.0) * c);
     dw[1] = E0 * dQ1 + T(0.5) * nx * dU * ((E1 + E2) - T(2.0) * E0) + nx * (E1 - E2) * dQ4 / (T(2.0) * rho * c);
@@ -344,7 +398,12 @@ __device__ __forceinline__ T ausmp_flux(T g, T sx, T sy, T sz, const T *QL, cons
     const T chL = cs2L / tmax(csL, L.ubar);
     const T chR = cs2R / tmax(csR, -R.ubar);
     const T c12 = tmin(chL, chR);
+#if CFD_FAST_MATH
+    const T ic12 = T(1.0) / c12;
+    const T ML = L.ubar * ic12, MR = R.ubar * ic12;
+#else
     const T ML = L.ubar / c12, MR = R.ubar / c12;
+#endif
     const T m12 = ausmp_mach_p(ML) + ausmp_mach_m(MR);
     const T p12 = ausmp_pres_p(ML) * L.p + ausmp_pres_m(MR) * R.p;
     const T mp = T(0.5) * (m12 + fabs(m12)), mm = T(0.5) * (m12 - fabs(m12));
@@ -367,4 +426,5 @@ __device__ __forceinline__ T ausmp_flux(T g, T sx, T sy, T sz, const T *QL, cons
     return area * tmax(fabs(ubar), tmax(fabs(ubar + c), fabs(ubar - c)));
 }
 
+}  // inline namespace
 }  // namespace cfd

@@ -312,6 +312,27 @@ __device__ __forceinline__ void fd_load(const FdArgs &a, int j, FdNode &r) {
     }
 }
 
+// The flux one finite-difference column differentiates: the matrix-form Roe
+// flux (Compute_RoeFlux), or the scheme's own flux for HLLC and AUSM+ (the
+// scheme-agnostic Compute_Jacobian_FiniteDifference,
+// MPFLOW3D/src/FiniteDifference_Jacobian.cpp:112-667), whose dissipative
+// part is zero.  F = conv + diss, as the oracle sums them.
+template <int FLUX>
+__device__ __forceinline__ void fd_edge_flux(const FdArgs &a, const double *QL, const double *QR, double sx, double sy,
+                                             double sz, double *F) {
+    if (FLUX == CFD_FLUX_ROE) {
+        roe_flux_matrix(a.g, a.efix, QL, QR, sx, sy, sz, F);
+        return;
+    }
+    double cf[5];
+    int bad = 0;
+    if (FLUX == CFD_FLUX_HLLC) hllc_flux(a.g, sx, sy, sz, QL, QR, cf, bad);
+    else ausmp_flux(a.g, sx, sy, sz, QL, QR, cf);
+    if (bad) atomicAdd(a.flag + 1, 1);
+#pragma unroll
+    for (int r = 0; r < 5; r++) F[r] = cf[r] + 0.0;
+}
+
 // Compute_Jacobian_FiniteDifference_Roe, Euler3D/src/Roe_Jacobian.cpp:236-776,
 // assembled by rows with ten lanes per node: lane (role, c) walks the node's
 // incidences in the reference's edge order and forms column c of
@@ -324,6 +345,7 @@ __device__ __forceinline__ void fd_load(const FdArgs &a, int j, FdNode &r) {
 // or perturbed and unperturbed for the one-sided methods (the reference
 // evaluates the unperturbed flux once per edge end and reuses it; the value
 // is the same).
+template <int FLUX>
 __global__ __launch_bounds__(128) void k_jacobian_fd(FdArgs a) {
     const int t = blockIdx.x * 128 + threadIdx.x;
     const int i = t / 10;  // internal node
@@ -365,7 +387,7 @@ __global__ __launch_bounds__(128) void k_jacobian_fd(FdArgs a) {
                 } else if (st == 1) {
                     atomicAdd(a.flag + 0, 1);
                 }
-                roe_flux_matrix(a.g, a.efix, QP, QG, sx, sy, sz, Fp[pm]);
+                fd_edge_flux<FLUX>(a, QP, QG, sx, sy, sz, Fp[pm]);
             }
 #pragma unroll
             for (int r = 0; r < 5; r++) D[r] += 0.5 * (Fp[0][r] - Fp[1][r]) / eps;
@@ -379,9 +401,9 @@ __global__ __launch_bounds__(128) void k_jacobian_fd(FdArgs a) {
         {
             double QL[5], QR[5];
             fd_faces(a, own, nb, iL, a.plus, pend, c, eps, QL, QR);
-            roe_flux_matrix(a.g, a.efix, QL, QR, sx, sy, sz, FP);
+            fd_edge_flux<FLUX>(a, QL, QR, sx, sy, sz, FP);
             fd_faces(a, own, nb, iL, a.minus, pend, c, -eps, QL, QR);
-            roe_flux_matrix(a.g, a.efix, QL, QR, sx, sy, sz, FM);
+            fd_edge_flux<FLUX>(a, QL, QR, sx, sy, sz, FM);
         }
         double col[5];
 #pragma unroll
@@ -1406,7 +1428,12 @@ int launch_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, i
     a.free = BcFree<double>{p->inf_rho, p->inf_u, p->inf_v, p->inf_w, p->inf_p, p->outflow_p};
     a.flag = m->flag;
     const int nb = (int)((10 * (size_t)m->N + 127) / 128);
-    hipLaunchKernelGGL(k_jacobian_fd, dim3(nb), dim3(128), 0, m->stream, a);
+    if (p->flux == CFD_FLUX_HLLC)
+        hipLaunchKernelGGL(k_jacobian_fd<CFD_FLUX_HLLC>, dim3(nb), dim3(128), 0, m->stream, a);
+    else if (p->flux == CFD_FLUX_AUSMP)
+        hipLaunchKernelGGL(k_jacobian_fd<CFD_FLUX_AUSMP>, dim3(nb), dim3(128), 0, m->stream, a);
+    else
+        hipLaunchKernelGGL(k_jacobian_fd<CFD_FLUX_ROE>, dim3(nb), dim3(128), 0, m->stream, a);
     CFD_HIP(m, hipGetLastError());
     m->jac_valid = true;
     m->jac_mixed = mixed != 0;
@@ -1562,27 +1589,46 @@ void mdot(cfd_mesh *m, int nv, const TV *V, size_t ld, const TV *w, int off) {
     hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(64), 0, m->stream, np, nv, m->red, m->redout + off);
 }
 
+// Krylov basis of one storage type for restart length mk.  Each precision
+// keeps its own basis, so alternating FP64 and mixed solves on one mesh
+// allocates each once; a basis too short for a larger mk is replaced (the
+// old one freed, after the stream is idle and the graphs captured on it are
+// dropped), so the mesh's footprint never grows past one basis per
+// precision.
 int ensure_krylov(cfd_mesh *m, int mk, bool mixed) {
-    if (m->kry_m >= mk && m->kry_mixed == mixed && (mixed ? m->Vf != nullptr : m->V != nullptr)) return CFD_OK;
     const size_t L = 5 * (size_t)m->N;
     if (mixed && !m->zpf) m->zpf = (float *)cfd_dalloc(m, sizeof(float) * L);
     if (!mixed && !m->zp) m->zp = (double *)cfd_dalloc(m, sizeof(double) * L);
     if ((mixed && !m->zpf) || (!mixed && !m->zp))
         return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (multicolour work vector)");
+    if (!m->r) m->r = (double *)cfd_dalloc(m, sizeof(double) * L);
+    if (!m->r) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (Krylov)");
+    int &have = mixed ? m->kry_m32 : m->kry_m64;
+    if (have >= mk) return CFD_OK;
+    if (have > 0) {
+        CFD_HIP(m, hipStreamSynchronize(m->stream));
+        drop_krylov_graphs(m);
+    }
     if (mixed) {
+        cfd_dfree(m, m->Vf);
+        cfd_dfree(m, m->Zf);
+        cfd_dfree(m, m->wf);
         m->Vf = (float *)cfd_dalloc(m, sizeof(float) * L * (mk + 1));
         m->Zf = (float *)cfd_dalloc(m, sizeof(float) * L * mk);
         m->wf = (float *)cfd_dalloc(m, sizeof(float) * L);
+        have = 0;
         if (!m->Vf || !m->Zf || !m->wf) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (Krylov FP32)");
     } else {
+        cfd_dfree(m, m->V);
+        cfd_dfree(m, m->Z);
+        cfd_dfree(m, m->w);
         m->V = (double *)cfd_dalloc(m, sizeof(double) * L * (mk + 1));
         m->Z = (double *)cfd_dalloc(m, sizeof(double) * L * mk);
         m->w = (double *)cfd_dalloc(m, sizeof(double) * L);
+        have = 0;
         if (!m->V || !m->Z || !m->w) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (Krylov)");
     }
-    if (!m->r) m->r = (double *)cfd_dalloc(m, sizeof(double) * L);
-    m->kry_m = mk;
-    m->kry_mixed = mixed;
+    have = mk;
     return CFD_OK;
 }
 
@@ -1635,13 +1681,17 @@ int enqueue_step(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, TV *V, 
         // sequence.
         hipGraph_t g = nullptr;
         hipGraphExec_t ex = nullptr;
-        hipError_t e = hipStreamBeginCapture(m->stream, hipStreamCaptureModeRelaxed);
-        if (e == hipSuccess) {
-            arnoldi_step<TM, TV>(m, a, lu, dinv, V, Z, w, j);
-            e = hipStreamEndCapture(m->stream, &g);
+        hipError_t e;
+        {
+            std::unique_lock<std::shared_mutex> lk(cfd_capture_mutex());
+            e = hipStreamBeginCapture(m->stream, hipStreamCaptureModeRelaxed);
+            if (e == hipSuccess) {
+                arnoldi_step<TM, TV>(m, a, lu, dinv, V, Z, w, j);
+                e = hipStreamEndCapture(m->stream, &g);
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (g) hipGraphDestroy(g);
         }
-        if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-        if (g) hipGraphDestroy(g);
         if (e != hipSuccess || !ex) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             if (hipStreamIsCapturing(m->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
@@ -1703,6 +1753,7 @@ int enqueue_step_batch(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, T
     }
     KrylovGraphs &kg = m->kgraph_b;
     if (kg.key != (const void *)V || kg.prec != m->prec_kind || kg.m != mk || kg.lu != (const void *)lu) {
+        CfdRuntimeGuard guard;
         for (auto &g : kg.exec)
             if (g) hipGraphExecDestroy((hipGraphExec_t)g);
         kg.key = V;
@@ -1718,13 +1769,17 @@ int enqueue_step_batch(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, T
         // sequence.
         hipGraph_t g = nullptr;
         hipGraphExec_t ex = nullptr;
-        hipError_t e = hipStreamBeginCapture(m->stream, hipStreamCaptureModeRelaxed);
-        if (e == hipSuccess) {
-            arnoldi_step_batch<TM, TV>(m, a, lu, dinv, V, Z, w, j);
-            e = hipStreamEndCapture(m->stream, &g);
+        hipError_t e;
+        {
+            std::unique_lock<std::shared_mutex> lk(cfd_capture_mutex());
+            e = hipStreamBeginCapture(m->stream, hipStreamCaptureModeRelaxed);
+            if (e == hipSuccess) {
+                arnoldi_step_batch<TM, TV>(m, a, lu, dinv, V, Z, w, j);
+                e = hipStreamEndCapture(m->stream, &g);
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (g) hipGraphDestroy(g);
         }
-        if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-        if (g) hipGraphDestroy(g);
         if (e != hipSuccess || !ex) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             if (hipStreamIsCapturing(m->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
@@ -1750,7 +1805,8 @@ int enqueue_step_batch(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, T
 // not being solved) keeps riding along but its results are not used.
 template <class TM, class TV>
 int fgmres_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM *lu, const TM *dinv, TV *V, TV *Z,
-                 TV *w, const std::vector<char> &solve, std::vector<int> &iters, std::vector<double> &fres) {
+                 TV *w, const std::vector<char> &solve, std::vector<int> &iters, std::vector<double> &fres,
+                 std::vector<int> &status) {
     const int K = m->ncase, mk = o->krylov_m;
     const size_t L = 5 * (size_t)m->N;
     const int gN = grid_cap(L, 4096);
@@ -1768,6 +1824,7 @@ int fgmres_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM 
         gv((size_t)K * (mk + 1)), hcol(mk + 2), y((size_t)mk);
     iters.assign(K, 0);
     fres.assign(K, 0.0);
+    status.assign(K, CFD_OK);
     for (int c = 0; c < K; c++) {
         beta[c] = sqrt(m->hbred[(size_t)c * kBOut]);
         tol[c] = o->rtol * beta[c];
@@ -1818,8 +1875,13 @@ int fgmres_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const TM *a, const TM 
                 iters[c]++;
                 kdim[c] = j + 1;
                 res[c] = fabs(gvc[j + 1]);
-                if (!(res[c] == res[c]))
-                    return cfd_set_error(m->ctx, CFD_E_NONFINITE, "batched FGMRES: non-finite residual in case %d", c);
+                if (!(res[c] == res[c])) {
+                    // this case fails on its own (as its solve alone would);
+                    // its correction is dropped and the other cases go on
+                    status[c] = CFD_E_NONFINITE;
+                    step[c] = cyc[c] = live[c] = false;
+                    continue;
+                }
                 if (res[c] <= tol[c] || hn == 0.0) step[c] = false;
                 stepping = stepping || step[c];
             }
@@ -1955,6 +2017,7 @@ int read_solve_flags(cfd_mesh *m);
 }  // namespace
 
 void drop_krylov_graphs(cfd_mesh *m) {
+    CfdRuntimeGuard guard;
     for (KrylovGraphs *kg : {&m->kgraph, &m->kgraph_b}) {
         for (auto &g : kg->exec)
             if (g) hipGraphExecDestroy((hipGraphExec_t)g);
@@ -1963,9 +2026,42 @@ void drop_krylov_graphs(cfd_mesh *m) {
     }
 }
 
+// Forward block Gauss-Seidel sweeps of MC_Iterative_Block_LU_Jacobi_CRS
+// (MATH/src/MC_Iterative_Jacobi.c:346-535) with the LU factors of the
+// diagonal blocks in gslu ([N][25]) and the per-row sums of the RMS in rowbuf
+// ([N]); both are written in full before they are read.
+static int gs_sweeps(cfd_mesh *m, int sweeps, double *gslu, double *rowbuf, cfd_linsolve_stats *st) {
+    const size_t L = 5 * (size_t)m->N;
+    hipLaunchKernelGGL(k_gs_lu, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, m->iau, m->a,
+                       gslu);
+    hipLaunchKernelGGL(k_fill, dim3(grid_cap(L, 4096)), dim3(kBlock), 0, m->stream, L, 0.0, m->x);
+    double rms = 0.0;
+    int it;
+    for (it = 0; it < sweeps; it++) {
+        for (int l = 0; l < m->lower.nlevels; l++) {
+            const int b0 = m->lower.ptr[l], nr = m->lower.ptr[l + 1] - b0;
+            hipLaunchKernelGGL(k_gs_level, dim3((nr + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, nr,
+                               m->lower.rows + b0, m->ia, m->ja, m->iau, m->a, m->b, gslu, m->x, rowbuf);
+        }
+        // row RMS: ordered reduction of the per-row sums
+        const int np = grid_cap(m->N, kDotParts);
+        hipLaunchKernelGGL(k_vsum_partial, dim3(np), dim3(kBlock), 0, m->stream, (size_t)m->N, rowbuf, m->red);
+        hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(64), 0, m->stream, np, 1, m->red, m->redout);
+        CFD_HIP(m, hipMemcpyAsync(m->hred, m->redout, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        CFD_HIP(m, hipStreamSynchronize(m->stream));
+        rms = sqrt(m->hred[0] / (double)m->N);
+        if (rms < 100.0 * DBL_EPSILON) {
+            it++;
+            break;
+        }
+    }
+    st->iterations = it;
+    st->final_res = rms;
+    return CFD_OK;
+}
+
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st) {
     if (!m->jac_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "cfd_linsolve: no Jacobian assembled");
-    const size_t L = 5 * (size_t)m->N;
     *st = cfd_linsolve_stats{};
     CFD_HIP(m, hipMemsetAsync(m->flag, 0, 4 * sizeof(int), m->stream));
     if (o->solver == CFD_SOLVER_BLOCK_GS || o->solver == CFD_SOLVER_LUSGS) {
@@ -1975,33 +2071,8 @@ int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st
             if (!m->gslu) m->gslu = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->N);
             if (!m->rowbuf) m->rowbuf = (double *)cfd_dalloc(m, sizeof(double) * (size_t)m->N);
             if (!m->gslu || !m->rowbuf) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory");
-            hipLaunchKernelGGL(k_gs_lu, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, m->iau,
-                               m->a, m->gslu);
-            hipLaunchKernelGGL(k_fill, dim3(grid_cap(L, 4096)), dim3(kBlock), 0, m->stream, L, 0.0, m->x);
-            double rms = 0.0;
-            int it;
-            for (it = 0; it < sweeps; it++) {
-                for (int l = 0; l < m->lower.nlevels; l++) {
-                    const int b0 = m->lower.ptr[l], nr = m->lower.ptr[l + 1] - b0;
-                    hipLaunchKernelGGL(k_gs_level, dim3((nr + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, nr,
-                                       m->lower.rows + b0, m->ia, m->ja, m->iau, m->a, m->b, m->gslu, m->x, m->rowbuf);
-                }
-                // row RMS: ordered reduction of the per-row sums
-                const int np = grid_cap(m->N, kDotParts);
-                hipLaunchKernelGGL(k_vsum_partial, dim3(np), dim3(kBlock), 0, m->stream, (size_t)m->N, m->rowbuf,
-                                   m->red);
-                hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(64), 0, m->stream, np, 1, m->red,
-                                   m->redout);
-                CFD_HIP(m, hipMemcpyAsync(m->hred, m->redout, sizeof(double), hipMemcpyDeviceToHost, m->stream));
-                CFD_HIP(m, hipStreamSynchronize(m->stream));
-                rms = sqrt(m->hred[0] / (double)m->N);
-                if (rms < 100.0 * DBL_EPSILON) {
-                    it++;
-                    break;
-                }
-            }
-            st->iterations = it;
-            st->final_res = rms;
+            int rc = gs_sweeps(m, sweeps, m->gslu, m->rowbuf, st);
+            if (rc) return rc;
         } else {
             hipLaunchKernelGGL(k_lusgs_init, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N,
                                m->b, m->x);
@@ -2032,7 +2103,7 @@ int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st
 }
 
 int run_linsolve_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const std::vector<char> &solve,
-                       std::vector<int> &iters, std::vector<double> &final_res) {
+                       std::vector<int> &iters, std::vector<double> &final_res, std::vector<int> &status) {
     if (m->ncase < 1) return cfd_set_error(m->ctx, CFD_E_STATE, "batched solve on a mesh that is not a union of cases");
     if (!m->jac_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "batched linsolve: no Jacobian assembled");
     if (o->solver != CFD_SOLVER_FGMRES) return cfd_set_error(m->ctx, CFD_E_ARG, "batched solves use FGMRES");
@@ -2040,9 +2111,11 @@ int run_linsolve_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const std::vecto
     int rc = prepare_fgmres(m, o);
     if (rc) return rc;
     if (o->mixed)
-        rc = fgmres_batch<float, float>(m, o, m->af, m->luf, m->dinvf, m->Vf, m->Zf, m->wf, solve, iters, final_res);
+        rc = fgmres_batch<float, float>(m, o, m->af, m->luf, m->dinvf, m->Vf, m->Zf, m->wf, solve, iters, final_res,
+                                        status);
     else
-        rc = fgmres_batch<double, double>(m, o, m->a, m->lu, m->dinv, m->V, m->Z, m->w, solve, iters, final_res);
+        rc = fgmres_batch<double, double>(m, o, m->a, m->lu, m->dinv, m->V, m->Z, m->w, solve, iters, final_res,
+                                          status);
     if (rc) return rc;
     return read_solve_flags(m);
 }

@@ -13,6 +13,15 @@
 #include "cfd_internal.h"
 #include "flux.h"
 
+// This file is also the fast-arithmetic build (residual_fast.hip defines
+// CFD_FAST_MATH 1 and compiles it with FMA contraction); its launchers then
+// live in res_fast instead of res_exact.
+#if CFD_FAST_MATH
+#define CFD_RES_NS res_fast
+#else
+#define CFD_RES_NS res_exact
+#endif
+
 using namespace cfd;
 
 namespace {
@@ -91,6 +100,8 @@ struct BcArgs {
     // cases still iterating (union of cases); a finished case keeps the
     // ghost states of its last iteration.  NULL: every edge.
     const int *case_active;
+    // failure bits per case (union of cases), NULL for one case
+    int *case_flag;
 };
 
 template <class T>
@@ -111,9 +122,13 @@ __global__ __launch_bounds__(kBlock) void k_bc(BcArgs<T> a) {
     const int st = bc_ghost(a.g, QL, QR, s[0], s[1], s[2], a.btype[e], a.iter, a.zpg, f, QB);
     if (st == 2) {
         flag_set(a.flag, 3);
+        if (a.case_flag) atomicOr(a.case_flag + a.bcase[e], 2);
         return;
     }
-    if (st == 1) flag_set(a.flag, 0);
+    if (st == 1) {
+        flag_set(a.flag, 0);
+        if (a.case_flag) atomicOr(a.case_flag + a.bcase[e], 1);
+    }
     T *o = a.qa + kQA * nr;
 #pragma unroll
     for (int k = 0; k < 5; k++) o[k] = QB[k];
@@ -710,6 +725,10 @@ struct ResArgs {
     int efix;
     unsigned long long *eig;
     int *flag;
+    // union of cases: failure bits per case and the case of each node
+    // (internal numbering; cases hold contiguous ranges), NULL for one case
+    int *case_flag;
+    const int *ecase;
 };
 
 // Neighbour data one incidence needs: q and coordinates, and for second
@@ -915,7 +934,10 @@ struct NodeResidual {
         }
         o[10] = dt;
         o[11] = T(0.0);
-        if (bad) flag_set(a.flag, 1);
+        if (bad) {
+            flag_set(a.flag, 1);
+            if (a.case_flag) atomicOr(a.case_flag + a.ecase[i], 4);
+        }
         if (STATS && deg > 0) {
             for (int t = 0; t < 3; t++) {
                 atomic_min_nonneg(a.eig + 2 * t, (double)emin[t]);
@@ -978,6 +1000,169 @@ __global__ __launch_bounds__(kBlock) void k_residual(ResArgs<T> a) {
     }
     nr.store(a, i, deg);
 }
+
+#if CFD_FAST_MATH
+// ---------------------------------------------------------------------------
+// Flux-once residual (CFD_ARITH_FAST).  Workgroup p owns a patch of at most
+// kBlock consecutive internal nodes (a compact piece of the Morton order;
+// patches restart at each case of a union) and its work edges: each interior edge with both ends in the patch
+// once, each edge to another patch (both patches evaluate it, each for its
+// own end) and each boundary edge of a patch node.  Chunks of kBlock work
+// edges are evaluated one per thread into LDS; then every node adds the
+// chunk's fluxes of its incidences, whose slots it lists in ascending order,
+// so each node's sum has a fixed order (deterministic results).  Two buffers
+// alternate, so one barrier per chunk suffices: chunk c + 2 overwrites a
+// buffer only after every thread has passed the barrier of chunk c + 1,
+// which follows its use of chunk c.  Interior fluxes are evaluated once per
+// patch instead of once per end, and every flux stays in the patch's LDS.
+// ---------------------------------------------------------------------------
+template <class T>
+struct PatchArgs {
+    int N, npatch;
+    const int *pstart;  // [npatch + 1] first node of each patch (at most kBlock nodes each)
+    const int *ip, *sptr;
+    // SELL (as the incidence lists): patch-local work-edge slot of each
+    // incidence of a node, ascending; bit 31 set where the node is the R end
+    const int *slot;
+    const int *wptr;    // [npatch + 1] first work edge of each patch
+    const int *wl, *wr; // L and R end of each work edge (R = N + b: ghost of boundary edge b)
+    const T *ws;        // [W][4] area vector, oriented L -> R (outward for boundary edges)
+    const T *qa, *qb;
+    T *res;
+    T g, frac;
+    int efix;
+    unsigned long long *eig;
+    int *flag, *case_flag;
+    const int *ecase;
+};
+
+// Flux of work edge w: conv, diss and the time-step term area * max |lambda|.
+// Second-order face states on interior edges (Compute_SecondOrderReconstructQ,
+// Euler3D/src/HigherOrderReconstructQ.cpp:20-71), first order on boundary edges.
+template <class T, int FLUX, int ORDER, bool LIM, bool STATS>
+__device__ __forceinline__ T edge_flux(const PatchArgs<T> &a, int w, T *cf, T *df, T *emin, T *emax, int &bad) {
+    const int l = a.wl[w], r = a.wr[w];
+    const T *sv = a.ws + 4 * (size_t)w;
+    const T s0 = sv[0], s1 = sv[1], s2 = sv[2];
+    const T *ql = a.qa + kQA * (size_t)l, *qr = a.qa + kQA * (size_t)r;
+    T QL[5], QR[5];
+    if (ORDER == 2 && r < a.N) {
+        const T *bl = a.qb + kQB * (size_t)l, *br = a.qb + kQB * (size_t)r;
+        const T Rx = T(0.5) * (qr[5] - ql[5]), Ry = T(0.5) * (qr[6] - ql[6]), Rz = T(0.5) * (qr[7] - ql[7]);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const T pl = LIM ? bl[kPHI + k] : T(1.0), pr = LIM ? br[kPHI + k] : T(1.0);
+            const T dq = qr[k] - ql[k];
+            QL[k] = ql[k] + pl * (a.frac * dq + (T(1.0) - a.frac) * (bl[3 * k] * Rx + bl[3 * k + 1] * Ry + bl[3 * k + 2] * Rz));
+            QR[k] = qr[k] - pr * (a.frac * dq + (T(1.0) - a.frac) * (br[3 * k] * Rx + br[3 * k + 1] * Ry + br[3 * k + 2] * Rz));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            QL[k] = ql[k];
+            QR[k] = qr[k];
+        }
+    }
+    if (FLUX == 0) {
+        T E0, E1, E2;
+        const T lam = roe_flux(a.g, a.efix, s0, s1, s2, QL, QR, cf, df, E0, E1, E2);
+        if (STATS) {
+            emin[0] = tmin(emin[0], E0);
+            emax[0] = tmax(emax[0], E0);
+            emin[1] = tmin(emin[1], E1);
+            emax[1] = tmax(emax[1], E1);
+            emin[2] = tmin(emin[2], E2);
+            emax[2] = tmax(emax[2], E2);
+        }
+        return lam;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) df[k] = T(0.0);
+    if (FLUX == 1) return hllc_flux(a.g, s0, s1, s2, QL, QR, cf, bad);
+    return ausmp_flux(a.g, s0, s1, s2, QL, QR, cf);
+}
+
+template <class T, int FLUX, int ORDER, bool LIM, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_residual_patch(PatchArgs<T> a) {
+    constexpr int NV = FLUX == 0 ? 11 : 6;  // conv, [diss,] area * max |lambda|
+    __shared__ T buf[2][NV][kBlock];
+    const int p = xcd_block();
+    if (p >= a.npatch) return;  // padding blocks of the XCD-rounded grid
+    const int t = threadIdx.x;
+    const int i = a.pstart[p] + t;
+    const bool own = i < a.pstart[p + 1];
+    const int w0 = a.wptr[p], W = a.wptr[p + 1] - w0;
+    int deg = 0, nxt = INT_MAX;
+    size_t idx = 0;
+    if (own) {
+        deg = a.ip[i + 1] - a.ip[i];
+        idx = (size_t)a.sptr[i / kSlice] + (i % kSlice);
+        if (deg > 0) nxt = ld_stream(a.slot + idx);
+    }
+    T ac[5], ad[5], dt = T(0.0), emin[3], emax[3];
+#pragma unroll
+    for (int k = 0; k < 5; k++) ac[k] = ad[k] = T(0.0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        emin[k] = T(DBL_MAX);
+        emax[k] = T(0.0);
+    }
+    int bad = 0, k = 0;
+    for (int c0 = 0, c = 0; c0 < W; c0 += kBlock, c ^= 1) {
+        if (c0 + t < W) {
+            T cf[5], df[5];
+            int wbad = 0;
+            const T lam = edge_flux<T, FLUX, ORDER, LIM, STATS>(a, w0 + c0 + t, cf, df, emin, emax, wbad);
+            if (wbad) {  // no HLLC wave configuration: flag the mesh and the edge's case
+                bad = 1;
+                if (a.case_flag) atomicOr(a.case_flag + a.ecase[a.wl[w0 + c0 + t]], 4);
+            }
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                buf[c][v][t] = cf[v];
+                if (FLUX == 0) buf[c][5 + v][t] = df[v];
+            }
+            buf[c][NV - 1][t] = lam;
+        }
+        __syncthreads();
+        // this node's incidences among the chunk's work edges, in slot order
+        while (k < deg && (nxt & 0x7fffffff) < c0 + kBlock) {
+            const int s = (nxt & 0x7fffffff) - c0;
+            const bool neg = nxt < 0;
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                const T f = buf[c][v][s];
+                ac[v] += neg ? -f : f;
+                if (FLUX == 0) {
+                    const T d = buf[c][5 + v][s];
+                    ad[v] += neg ? -d : d;
+                }
+            }
+            dt += buf[c][NV - 1][s];
+            k++;
+            idx += kSlice;
+            nxt = k < deg ? ld_stream(a.slot + idx) : INT_MAX;
+        }
+    }
+    if (own) {
+        T *o = a.res + kRES * (size_t)i;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            o[v] = ac[v];
+            o[5 + v] = ad[v];
+        }
+        o[10] = dt;
+        o[11] = T(0.0);
+    }
+    if (bad) flag_set(a.flag, 1);
+    if (STATS && W > t) {
+        for (int v = 0; v < 3; v++) {
+            atomic_min_nonneg(a.eig + 2 * v, (double)emin[v]);
+            atomic_max_nonneg(a.eig + 2 * v + 1, (double)emax[v]);
+        }
+    }
+}
+#endif  // CFD_FAST_MATH
 
 // ---------------------------------------------------------------------------
 // Local time step, RMS, update, forces
@@ -1079,6 +1264,9 @@ int grid_for(size_t n) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+namespace CFD_RES_NS {
+
+#if !CFD_FAST_MATH
 int launch_lsq_coef(cfd_mesh *m) {
     const int nb = (m->N + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_lsq_coef, dim3(xcd_grid(nb)), dim3(kBlock), 0, m->stream, m->N, m->n2n_ia, m->n2n_ja, m->d.qa,
@@ -1092,11 +1280,13 @@ int launch_bc(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, int iter) {
     if (m->B == 0) return CFD_OK;
     BcArgs<T> a{m->N, m->E, m->B, iter, p->zpg_iter, m->bnode, m->btype, b.sa, b.qa, (T)p->gamma, (T)p->inf_rho,
                 (T)p->inf_u, (T)p->inf_v, (T)p->inf_w, (T)p->inf_p, (T)p->outflow_p, m->flag, m->case_inf,
-                m->bcase, m->ncase ? m->case_active : nullptr};
+                m->bcase, m->ncase ? m->case_active : nullptr, m->ncase ? m->case_flag : nullptr};
     hipLaunchKernelGGL(k_bc<T>, dim3((m->B + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, a);
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;
 }
+
+#endif
 
 template <class T>
 int launch_gradient(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p) {
@@ -1192,10 +1382,46 @@ static void launch_res_flux(cfd_mesh *m, const ResArgs<T> &a, int order, bool li
     }
 }
 
+#if CFD_FAST_MATH
+template <class T, int FLUX>
+static void launch_patch_flux(cfd_mesh *m, const PatchArgs<T> &a, int order, bool lim, bool stats) {
+    dim3 g(xcd_grid(m->npatch)), bl(kBlock);
+    if (order == 2) {
+        if (lim) {
+            if (stats) hipLaunchKernelGGL((k_residual_patch<T, FLUX, 2, true, true>), g, bl, 0, m->stream, a);
+            else hipLaunchKernelGGL((k_residual_patch<T, FLUX, 2, true, false>), g, bl, 0, m->stream, a);
+        } else {
+            if (stats) hipLaunchKernelGGL((k_residual_patch<T, FLUX, 2, false, true>), g, bl, 0, m->stream, a);
+            else hipLaunchKernelGGL((k_residual_patch<T, FLUX, 2, false, false>), g, bl, 0, m->stream, a);
+        }
+    } else {
+        if (stats) hipLaunchKernelGGL((k_residual_patch<T, FLUX, 1, false, true>), g, bl, 0, m->stream, a);
+        else hipLaunchKernelGGL((k_residual_patch<T, FLUX, 1, false, false>), g, bl, 0, m->stream, a);
+    }
+}
+
+// The flux-once residual over the work-edge patches (built on first use).
+template <class T>
+int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats) {
+    int rc = cfd_ensure_patches(m, sizeof(T) == 4);
+    if (rc) return rc;
+    PatchArgs<T> a{m->N, m->npatch, m->pn_start, m->inc_ptr, m->sptr, m->pn_slot, m->pw_ptr, m->pw_l, m->pw_r, b.ws, b.qa, b.qb,
+                   b.res, (T)p->gamma, (p->limiter_order == 2) ? T(0.5) : T(0.0), p->entropy_fix, m->eigbits,
+                   m->flag, m->ncase ? m->case_flag : nullptr, m->ecase};
+    const bool lim = p->limiter != CFD_LIM_NONE;
+    const bool st = stats && p->flux == CFD_FLUX_ROE;
+    if (p->flux == CFD_FLUX_ROE) launch_patch_flux<T, 0>(m, a, p->order, lim, st);
+    else if (p->flux == CFD_FLUX_HLLC) launch_patch_flux<T, 1>(m, a, p->order, lim, false);
+    else launch_patch_flux<T, 2>(m, a, p->order, lim, false);
+    CFD_HIP(m, hipGetLastError());
+    return CFD_OK;
+}
+#else
 template <class T>
 int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats) {
     ResArgs<T> a{m->N, m->NT, m->inc_ptr, m->sptr, m->nbr, b.isx, b.isy, b.isz, b.qa, b.qb, b.res, (T)p->gamma,
-                 (p->limiter_order == 2) ? T(0.5) : T(0.0), p->entropy_fix, m->eigbits, m->flag};
+                 (p->limiter_order == 2) ? T(0.5) : T(0.0), p->entropy_fix, m->eigbits, m->flag,
+                 m->ncase ? m->case_flag : nullptr, m->ecase};
     const int nb = xcd_grid((m->N + kBlock - 1) / kBlock);
     const bool lim = p->limiter != CFD_LIM_NONE;
     const bool st = stats && p->flux == CFD_FLUX_ROE;
@@ -1205,7 +1431,9 @@ int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool s
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;
 }
+#endif
 
+#if !CFD_FAST_MATH
 int launch_delta_t(cfd_mesh *m, double cfl) {
     hipLaunchKernelGGL(k_delta_t, dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream, m->N, m->d.vol,
                        m->d.res, cfl, m->dt);
@@ -1276,8 +1504,12 @@ int launch_convert_geometry_fp32(cfd_mesh *m) {
     return CFD_OK;
 }
 
+#endif
+
+#if !CFD_FAST_MATH
 template int launch_bc<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *, int);
 template int launch_bc<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *, int);
+#endif
 template int launch_gradient<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *);
 template int launch_gradient<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *);
 template int launch_limiter<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *);
@@ -1286,3 +1518,5 @@ template int launch_gradient_limiter<double>(cfd_mesh *, ResidualBufs<double> &,
 template int launch_gradient_limiter<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *);
 template int launch_residual<double>(cfd_mesh *, ResidualBufs<double> &, const cfd_params *, bool);
 template int launch_residual<float>(cfd_mesh *, ResidualBufs<float> &, const cfd_params *, bool);
+
+}  // namespace CFD_RES_NS

@@ -241,10 +241,22 @@ cfd_completion batch_completion(const BatchRun &b, int c, int status) {
 
 // Called by the batch solve when a request's result is final: posts its
 // completion at once, while the rest of the batch keeps iterating.
+// Text of a failure status of one batched case (the solve alone reports the
+// same status with cfd_last_error's text).
+const char *case_failure_text(int status) {
+    switch (status) {
+    case CFD_E_NONFINITE: return "non-finite residual";
+    case CFD_E_NEGATIVE_PRESSURE: return "negative pressure in the boundary condition";
+    case CFD_E_HLLC: return "no HLLC wave configuration";
+    case CFD_E_ARG: return "no boundary-condition instance applies";
+    default: return "solve failed";
+    }
+}
+
 void post_batch_case(void *arg, int c) {
     BatchRun &b = *(BatchRun *)arg;
     cfd_completion o = batch_completion(b, c, b.res[c].status);
-    if (o.status) snprintf(o.message, sizeof(o.message), "solve failed");
+    if (o.status) snprintf(o.message, sizeof(o.message), "%s", case_failure_text(o.status));
     std::lock_guard<std::mutex> lk(b.s->mu);
     b.s->done.push_back(o);
     b.posted[c] = 1;
@@ -267,16 +279,18 @@ std::vector<cfd_completion> run_batch(cfd_server *s, const Job &job) {
         qo[c] = job.reqs[c].req.q_out;
     }
     BatchRun b{s, &job, std::vector<cfd_solve_result>(K), std::vector<char>(K, 0), s->now()};
-    // direct launches, as for served meshes: other workers use the runtime
-    // while this batch runs, and stream capture is not safe beside them
+    // captured Arnoldi steps, as for served meshes: the library serialises
+    // each capture against the allocations, frees and synchronous copies of
+    // the other workers (cfd_capture_mutex)
     const int rc = cfd_batch_solve_into(s->ctx, K, descs.data(), params.data(), opts.data(), b.res.data(), qo.data(),
-                                        &b.bytes, false, post_batch_case, &b);
+                                        &b.bytes, true, post_batch_case, &b);
     std::vector<cfd_completion> out;
     std::lock_guard<std::mutex> lk(s->mu);  // posted[] is written under the lock
     for (int c = 0; c < K; c++) {
         if (b.posted[c]) continue;
         cfd_completion o = batch_completion(b, c, rc ? rc : b.res[c].status);
-        if (o.status) snprintf(o.message, sizeof(o.message), "%s", rc ? cfd_last_error(s->ctx) : "solve failed");
+        if (o.status)
+            snprintf(o.message, sizeof(o.message), "%s", rc ? cfd_last_error(s->ctx) : case_failure_text(o.status));
         out.push_back(o);
     }
     return out;
@@ -292,10 +306,6 @@ cfd_completion run_one(cfd_server *s, const Pending &p, Resident *slot) {
     int rc;
     if (!slot->mesh) {
         rc = cfd_mesh_create(s->ctx, p.req.mesh, &p.req.params, &slot->mesh);
-        // Several worker threads drive their meshes at once; graph capture on
-        // one thread can be invalidated by another thread's allocations, so
-        // served meshes launch their Krylov steps directly.
-        if (!rc) slot->mesh->use_graphs = false;
     } else {
         c.reused = 1;
         rc = cfd_mesh_reset(slot->mesh, &p.req.params);

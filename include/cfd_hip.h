@@ -57,6 +57,16 @@ extern "C" {
 /* precision of residual-path arithmetic and storage */
 #define CFD_FP64 0
 #define CFD_FP32 1
+/* arithmetic of the gradient, limiter and residual kernels:
+ * EXACT: the reference's operation order, each edge flux evaluated by both
+ *        of its nodes and summed in the reference's edge order; FP64
+ *        results are bit-identical to the reference.
+ * FAST:  fused multiply-adds, one reciprocal per shared denominator, each
+ *        interior edge flux evaluated once per patch of nodes; sums in a
+ *        fixed order of their own (deterministic), FP64 results within 1e-12
+ *        of EXACT relative to the largest residual entry. */
+#define CFD_ARITH_EXACT 0
+#define CFD_ARITH_FAST 1
 /* preconditioners */
 #define CFD_PREC_NONE 0
 #define CFD_PREC_BJACOBI 1
@@ -115,7 +125,7 @@ typedef struct cfd_params {
     double venkat_k;
     double cfl_min, cfl_max;
     int32_t cfl_ramp;
-    int32_t pad_;
+    int32_t arith;          /* CFD_ARITH_* of gradient, limiter and residual */
 } cfd_params;
 
 typedef struct cfd_linsolve_opts {
@@ -143,7 +153,19 @@ typedef struct cfd_solve_opts {
     int32_t wall_tag;       /* surface tag for forces (<= 0: every Euler wall) */
     double alpha_deg;       /* for lift/drag axes */
     cfd_linsolve_opts lin;
+    int32_t jacobian;       /* CFD_SOLVE_JAC_* */
+    int32_t pad_;
 } cfd_solve_opts;
+
+/* Jacobian of the steady loop (always of the first-order flux, as the
+ * reference's Compute_Jacobian switches to first order,
+ * Euler3D/src/Jacobian.cpp:18-24):
+ * APPROX: the approximate Roe Jacobian (Compute_Jacobian_Approximate_Roe),
+ *         the reference's, for every scheme;
+ * FD:     central finite differences of the scheme's own flux
+ *         (cfd_jacobian_fd), the Jacobian of HLLC and AUSM+ themselves. */
+#define CFD_SOLVE_JAC_APPROX 0
+#define CFD_SOLVE_JAC_FD 1
 
 typedef struct cfd_solve_result {
     int32_t iterations;     /* pseudo-time (Newton) iterations */
@@ -215,7 +237,10 @@ int cfd_jacobian(cfd_mesh *mesh, const cfd_params *p, int rhs_mode, int mixed);
  * selected by Compute_Jacobian_Roe (Euler3D/src/Roe_Jacobian.cpp:780-804) for
  * the central, forward, backward and alternate methods: finite differences
  * (eps 1e-8) of the matrix-form Roe flux Compute_RoeFlux
- * (Euler3D/src/Roe_Fluxes.cpp:552-905) with respect to each end's state;
+ * (Euler3D/src/Roe_Fluxes.cpp:552-905) with respect to each end's state; for
+ * HLLC and AUSM+ (p->flux) the same perturbations differentiate the scheme's
+ * own flux, as MPFLOW3D's scheme-agnostic Compute_Jacobian_FiniteDifference
+ * (MPFLOW3D/src/FiniteDifference_Jacobian.cpp:112-667) does;
  * second-order face states when p->order is 2.  Boundary edges are always
  * central, with the characteristic ghost state recomputed for each
  * perturbation.  CFD_JAC_FD_ALTERNATE is forward on even iterations and

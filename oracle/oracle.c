@@ -914,8 +914,13 @@ double orc_edge_flux_matrix(const orc_params *p, const double *s, const double *
     return 0.0;
 }
 
-/* The states Compute_RoeFlux takes for edge (nl, nr): second-order
- * reconstruction between physical nodes, node values otherwise. */
+/* The flux a finite-difference column differentiates for edge (nl, nr), from
+ * the states Compute_RoeFlux takes (second-order reconstruction between
+ * physical nodes, node values otherwise): the matrix-form Roe flux
+ * (Compute_Jacobian_FiniteDifference_Roe), or the scheme's own flux for HLLC
+ * and AUSM+ (the scheme-agnostic Compute_Jacobian_FiniteDifference,
+ * MPFLOW3D/src/FiniteDifference_Jacobian.cpp:112-667, with this build's
+ * residual fluxes), whose dissipative part is zero. */
 static void fd_flux(const orc_mesh *m, const orc_params *P, const double *q, const double *grad, const double *phi,
                     int nl, int nr, const double *s, double *conv, double *diss) {
     const int N = m->n_node, NT = N + m->n_bedge;
@@ -928,7 +933,15 @@ static void fd_flux(const orc_mesh *m, const orc_params *P, const double *q, con
             QR[k] = q[(size_t)k * NT + nr];
         }
     }
-    roe_flux_matrix(P, s, QL, QR, conv, diss);
+    if (P->flux == ORC_ROE) {
+        roe_flux_matrix(P, s, QL, QR, conv, diss);
+        return;
+    }
+    if (P->flux == ORC_HLLC)
+        hllc_flux(P, s, QL, QR, conv);
+    else
+        ausmp_flux(P, s, QL, QR, conv);
+    for (int k = 0; k < 5; k++) diss[k] = 0.0;
 }
 
 /* Compute_Jacobian_FiniteDifference_Roe, Euler3D/src/Roe_Jacobian.cpp:236-776.

@@ -8,8 +8,8 @@
  *   pinned by golden vectors of the compiled reference (bit-exact):
  *     orc_residual (Roe, HLLC; first and second order), orc_bc,
  *     orc_lsq_gradient, orc_limiter (Venkatakrishnan, Barth-Jespersen,
- *     pressure corrections), orc_delta_t, orc_jacobian, orc_block_gs,
- *     orc_lusgs
+ *     pressure corrections), orc_delta_t, orc_jacobian, orc_jacobian_fd
+ *     (Roe), orc_block_gs, orc_lusgs
  *   pinned by closed forms only (no reference arithmetic exists), each
  *   chosen so that a plausible mistake fails it:
  *     AUSM+ flux (Liou 1996): exact stationary normal shock (pins the
@@ -24,6 +24,9 @@
  *     orc_forces: closed surface under uniform pressure gives F = 0; lift
  *       and drag are F along the reference's freestream axes at 30, 90 and
  *       -2 degrees;
+ *     orc_jacobian_fd for HLLC and AUSM+: its product with a small state
+ *       change equals the change of the (pinned) first-order residual to
+ *       first order, which fixes the flux, signs and block placement;
  *     orc_solve_steady (iteration counts): the GPU runs the same algorithm.
  *
  * Layouts: Q, residuals, limiter: structure of arrays [5][n];
@@ -109,9 +112,12 @@ void orc_jacobian(const orc_mesh *m, const orc_params *p, const double *q, const
                   const double *res_conv, const double *res_diss, int rhs_mode, const int32_t *ia,
                   const int32_t *ja, const int32_t *iau, double *a, double *b);
 
-/* Finite-difference Roe Jacobian (Compute_Jacobian_FiniteDifference_Roe,
+/* Finite-difference Jacobian (Compute_Jacobian_FiniteDifference_Roe,
  * Euler3D/src/Roe_Jacobian.cpp:236-776) of the matrix-form Roe flux
- * (Compute_RoeFlux, Euler3D/src/Roe_Fluxes.cpp:552-905), eps 1e-8.  method:
+ * (Compute_RoeFlux, Euler3D/src/Roe_Fluxes.cpp:552-905), eps 1e-8; for HLLC
+ * and AUSM+ the same perturbation sequence differentiates the scheme's own
+ * flux (the scheme-agnostic Compute_Jacobian_FiniteDifference,
+ * MPFLOW3D/src/FiniteDifference_Jacobian.cpp:112-667).  method:
  * ORC_FD_CENTRAL / FORWARD / BACKWARD / ALTERNATE (forward on even
  * iterations, backward on odd); boundary edges are central.  q is [5][N+B]
  * with the ghost states of the last boundary-condition pass; grad/phi are

@@ -375,3 +375,42 @@ def test_green_gauss_constant_field_zero_at_every_node():
     q[:, m.n_node:] = c[:, None] * (1.0 + np.random.default_rng(12).random((5, m.n_bedge)))
     g = orc.gg_gradient(om, q)
     assert np.max(np.abs(g)) < 1e-12 * np.max(np.abs(c)) / np.min(m.vol) * np.max(np.abs(m.edge_area))
+
+
+@pytest.mark.parametrize("flux", [orc.ROE, orc.HLLC, orc.AUSMP])
+def test_fd_jacobian_linearises_residual(small_mesh, flux):
+    """The finite-difference Jacobian (no time term) times a small change of the node states equals the change of
+    the first-order residual Conv + Diss, ghosts recomputed by the boundary condition, to first order.  Differentiating
+    another scheme's flux, a sign, or a block placed in the transposed position gives an O(1) mismatch."""
+    m, om = small_mesh
+    p = orc.params(FREESTREAM, flux=flux)
+    rng = np.random.default_rng(11 + flux)
+    N = om.n_node
+    q = np.tile(prim_to_q(FREESTREAM["rho"], FREESTREAM["u"], FREESTREAM["v"], FREESTREAM["w"],
+                          FREESTREAM["p"])[:, None], (1, om.n_total))
+    q[:, :N] *= 1.0 + 0.05 * rng.standard_normal((5, N))
+    q, bad = orc.bc(om, p, 3, q)
+    assert bad == 0
+    # R(x): the residual with the ghosts the boundary condition gives for the node states x and the current ghosts
+    # of q (each FD column of a boundary edge recomputes its ghost the same way)
+    q0, _ = orc.bc(om, p, 3, q)
+    rc0, rd0, _, _ = orc.residual(om, p, q0)
+    ia, ja, iau = orc.bsr_structure(om)
+    a, _, bad = orc.jacobian_fd(om, p, q, None, None, np.ones(N), rc0, rd0, 1, orc.FD_CENTRAL, 3, 0, ia, ja, iau)
+    assert bad == 0
+    delta = 1e-6 * np.abs(q[:, :N]) * rng.standard_normal((5, N))
+    q1 = q.copy()
+    q1[:, :N] += delta
+    q1, _ = orc.bc(om, p, 3, q1)
+    rc1, rd1, _, _ = orc.residual(om, p, q1)
+    dR = ((rc1 + rd1) - (rc0 + rd0)).T
+    Jd = orc.bsr_mv(ia, ja, a, delta.T)
+    assert np.max(np.abs(Jd - dR)) <= 1e-4 * np.max(np.abs(dR))
+    # the same check fails for the transpose of the Jacobian's block pattern
+    at = np.zeros_like(a)
+    for i in range(N):
+        for k in range(ia[i], ia[i + 1]):
+            j = ja[k]
+            kt = ia[j] + int(np.searchsorted(ja[ia[j]:ia[j + 1]], i))
+            at[kt] = a[k]
+    assert np.max(np.abs(orc.bsr_mv(ia, ja, at, delta.T) - dR)) > 1e-2 * np.max(np.abs(dR))
