@@ -8,12 +8,16 @@ kernel) + second-order Roe residual with the Conv/Diss split, FP64, on a
 160^3 hex grid split into 24.6M tetrahedra (4.17M nodes), interior nodes
 jittered by U(-0.2h, 0.2h), RCM-ordered, with a random per-node state (Mach
 U(0.3, 2.0) along +x, density and pressure with 5% Gaussian noise).  Inputs
-are resident in HBM before timing starts.  value = algorithmic bytes of
-G+L+R2 (each array once per kernel, as if gradient and limiter ran
-separately) / wall time per step.  The roofline block reports the residual
-kernel alone from HIP events on the stream it runs on.  cpu_baseline times
-the CPU oracle (single-threaded restatement of the reference) on the same
-mesh and state and checks the GPU result against it.
+are resident in HBM before timing starts.  The headline runs the fast build
+(FMA contraction, shared reciprocals, the flux-once residual over work-edge
+patches), whose FP64 results stay within 1e-12 relative of the reference;
+the exact build (the reference's operation order, bit-identical results) is
+reported beside it.  value = algorithmic bytes of G+L+R2 (each array once
+per kernel, as if gradient and limiter ran separately) / wall time per step.
+The roofline block reports the residual kernel alone from HIP events on the
+stream it runs on.  cpu_baseline times the CPU oracle (single-threaded
+restatement of the reference) on the same mesh and state and checks both
+GPU builds against it.
 
 Also measured: HLLC and AUSM+, FP32, unfused gradient/limiter and first-order
 variants ("variants"); a converged steady solve of the 48x16x16 bump channel,
@@ -28,9 +32,8 @@ import sys
 import time
 
 # The serving leg overlaps many solves, one HIP stream each; HIP maps a
-# process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set), so
-# ask for enough queues before the runtime initialises.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set).
+# The runtime's default of 4 is kept: 8 or 16 queues served more slowly.
 
 import numpy as np
 
@@ -67,7 +70,7 @@ class HipEvents:
 def algorithmic_bytes(N, E, B, s, i=4):
     """Compulsory traffic per evaluation, each array counted once per kernel."""
     R1 = E * (2 * i + 3 * s) + N * (5 * s + 5 * s + s) + B * (3 * i + 3 * s + 5 * s)
-    G = E * (2 * i + 3 * s) + N * (5 * s + s + 15 * s) + B * (2 * i + 3 * s)
+    G = E * (2 * i + 3 * s) + N * (5 * s + s + 15 * s) + B * (2 * i + 3 * s + 5 * s)
     L = E * (2 * i) + N * (5 * s + 15 * s + 3 * s + s + 5 * s)
     R2 = E * (2 * i + 3 * s) + N * (5 * s + 15 * s + 5 * s + 3 * s + 5 * s + s) + B * (3 * i + 3 * s + 5 * s)
     return dict(R1=R1, R1split=R1 + N * 5 * s, G=G, L=L, R2=R2, R2split=R2 + N * 5 * s)
@@ -97,7 +100,8 @@ def make_state(m, p, seed=20261202):
 
 
 def time_variant(dm, ev, p, steps, warmup, second_order=True, fused=True):
-    """Returns (wall ms per step, mean ms of G, L, R kernels).  With fused=True the gradient and limiter run as
+    """Returns (wall ms per step, mean ms of G, L, R kernels, median ms of one step from its events).  With
+    fused=True the gradient and limiter run as
     one kernel (cfd_gradient_limiter) and its time is reported under G, with L = 0."""
     stream = dm.stream()
 
@@ -133,13 +137,15 @@ def time_variant(dm, ev, p, steps, warmup, second_order=True, fused=True):
     dm.sync()
     wall = (time.perf_counter() - t0) * 1e3 / steps
     g = l = r = 0.0
+    per_step = []
     for e in evsets:
         g += ev.elapsed_ms(e[0], e[1])
         l += ev.elapsed_ms(e[1], e[2])
         r += ev.elapsed_ms(e[2], e[3])
+        per_step.append(ev.elapsed_ms(e[0], e[3]))
         for x in e:
             ev.destroy(x)
-    return wall, g / steps, l / steps, r / steps
+    return wall, g / steps, l / steps, r / steps, float(np.median(per_step))
 
 
 def cpu_baseline(m, p, q, build="parity"):
@@ -499,11 +505,14 @@ def main():
     ap.add_argument("--only", action="store_true", help="run only the headline variant (for profiling)")
     ap.add_argument("--no-solves", action="store_true", help="skip the steady-solve and serving legs")
     ap.add_argument("--no-cpu-solve", action="store_true", help="skip the oracle's CPU timing of the 48x16x16 solve")
-    ap.add_argument("--legs", type=str, default="residual,solve,serve,batch",
-                    help="comma list of legs to run: residual, solve, serve, batch, solve3 (192x64x64 solve)")
-    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--legs", type=str, default="residual,solve,solve3,serve,batch",
+                    help="comma list of legs to run: residual, solve (48x16x16), solve3 (192x64x64), serve, batch")
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="requests of the serving leg (the survey's serving stream has 2000; the bench takes a "
+                         "sample of the same distribution)")
     ap.add_argument("--serve-meshes", type=int, default=6)
-    ap.add_argument("--serve-nodes", type=str, default="10000,50000", help="min,max nodes of the serving meshes")
+    ap.add_argument("--serve-nodes", type=str, default="20000,1500000",
+                    help="min,max nodes of the serving meshes (log-uniform)")
     ap.add_argument("--serve-inflight", type=int, default=4,
                     help="server worker threads; 4 matches the HIP runtime's 4 hardware queues per process")
     ap.add_argument("--serve-batch", type=int, default=8, help="requests per union-mesh batch in the server")
@@ -511,9 +520,9 @@ def main():
                     help="longest wait of a batchable request for companions while the GPU is busy")
     ap.add_argument("--serve-case-nodes", type=int, default=0,
                     help="largest mesh (nodes) the server batches (0: the library default, 131072)")
-    ap.add_argument("--batch-cases", type=int, default=32,
+    ap.add_argument("--batch-cases", type=int, default=512,
                     help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")
-    ap.add_argument("--batch-fp64-sample", type=int, default=32,
+    ap.add_argument("--batch-fp64-sample", type=int, default=16,
                     help="cases of the batch also solved one at a time in FP64 for the baseline rate (0: none)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (scale factor for HBM counters)")
@@ -542,7 +551,8 @@ def main():
     legs = set(x.strip() for x in args.legs.split(",") if x.strip())
     if "residual" not in legs:
         ctx = cfd.Context(0)
-        out = {"metric": "converged solves/s & p99 latency (mixed vs FP64 GPU); residual-eval GB/s", "legs": sorted(legs)}
+        out = {"metric": "converged solves/s & p99 latency (mixed vs FP64 GPU); residual-eval GB/s",
+               "legs": sorted(legs)}
         if "solve" in legs:
             out["solves"] = solve_leg(ctx, cpu=not args.no_cpu_solve)
         if "solve3" in legs:
@@ -575,7 +585,7 @@ def main():
     bytes32 = algorithmic_bytes(N, E, B, 4)
     variants = {}
     EX, FA = cfd.ARITH_EXACT, cfd.ARITH_FAST
-    plan = [("roe", cfd.FP64, True, True, EX), ("roe", cfd.FP64, True, True, FA)]
+    plan = [("roe", cfd.FP64, True, True, FA), ("roe", cfd.FP64, True, True, EX)]
     if not args.no_variants and not args.only:
         plan += [("roe", cfd.FP64, True, False, EX), ("hllc", cfd.FP64, True, True, EX),
                  ("ausm+", cfd.FP64, True, True, EX), ("roe", cfd.FP32, True, True, EX),
@@ -589,45 +599,60 @@ def main():
                              venkat_k=5.0, gradient=cfd.GRAD_GG, precision=prec, arith=arith)
         head_case = (flux, prec, second, fused, arith) == plan[0]
         steps = args.steps if head_case else max(10, args.steps // 4)
-        wall, tg, tl, tr = time_variant(dm, ev, pv, steps, args.warmup if head_case else 5, second, fused)
+        wall, tg, tl, tr, tmed = time_variant(dm, ev, pv, steps, args.warmup if head_case else 5, second, fused)
         by = bytes64 if prec == cfd.FP64 else bytes32
         rkey = ("R2" if second else "R1") + ("split" if flux == "roe" else "")
         total = (by["G"] + by["L"] + by[rkey]) if second else by[rkey]
         name = "%s-%s%s-%s" % (flux, "fp64" if prec == cfd.FP64 else "fp32", "fast" if arith == FA else "",
                                ("GL+R2" if fused else "G+L+R2") if second else "R1")
-        variants[name] = dict(ms_per_step=wall, gbs=total / wall / 1e6, g_ms=tg, l_ms=tl, r_ms=tr,
+        variants[name] = dict(ms_per_step=wall, median_step_ms=tmed, gbs=total / wall / 1e6, g_ms=tg, l_ms=tl, r_ms=tr,
                               r_gbs=by[rkey] / tr / 1e6 if tr > 0 else None,
                               g_gbs=(by["G"] + (by["L"] if fused else 0)) / tg / 1e6 if tg > 0 else None,
                               l_gbs=by["L"] / tl / 1e6 if tl > 0 else None, bytes=total)
         print("bench: %-22s %8.3f ms/step %8.1f GB/s  (G %.3f L %.3f R %.3f ms)" %
               (name, wall, total / wall / 1e6, tg, tl, tr), file=sys.stderr, flush=True)
-    head = variants["roe-fp64-GL+R2"]
-    # parity spot check of the measured configuration against the oracle
+    head_name, exact_name = "roe-fp64fast-GL+R2", "roe-fp64-GL+R2"
+    head = variants[head_name]
+    # parity spot check of both builds of the measured configuration against the oracle: the exact build to the
+    # bit, the fast build within 1e-12 relative (max over nodes and components of |gpu - oracle| / max |oracle|)
     cpu = None
     if not args.no_cpu_baseline and not args.only:
-        pv = cfd.make_params(mach=0.8, flux="roe", order=2, limiter=cfd.LIM_VENKAT, venkat_k=5.0,
-                             gradient=cfd.GRAD_GG)
-        dm.gradient_limiter(pv)
-        dm.residual(pv)
-        gc, gd = dm.get(cfd.FIELD_RES_CONV), dm.get(cfd.FIELD_RES_DISS)
+        gpu = {}
+        for arith in (FA, EX):
+            pv = cfd.make_params(mach=0.8, flux="roe", order=2, limiter=cfd.LIM_VENKAT, venkat_k=5.0,
+                                 gradient=cfd.GRAD_GG, arith=arith)
+            dm.gradient_limiter(pv)
+            dm.residual(pv)
+            gpu[arith] = (dm.get(cfd.FIELD_RES_CONV), dm.get(cfd.FIELD_RES_DISS))
+
+        def rel_to(rc, rd, arith):
+            gc, gd = gpu[arith]
+            return float(max(np.max(np.abs(gc - rc)) / np.max(np.abs(rc)),
+                             np.max(np.abs(gd - rd)) / np.max(np.abs(rd))))
+
         secs, rc, rd = cpu_baseline(m, pv, q)
-        rel = max(np.max(np.abs(gc - rc)) / np.max(np.abs(rc)), np.max(np.abs(gd - rd)) / np.max(np.abs(rd)))
+        rel, rel_ex = rel_to(rc, rd, FA), rel_to(rc, rd, EX)
         secs_f, rc, rd = cpu_baseline(m, pv, q, build="speed")
-        rel_f = max(np.max(np.abs(gc - rc)) / np.max(np.abs(rc)), np.max(np.abs(gd - rd)) / np.max(np.abs(rd)))
+        rel_f = rel_to(rc, rd, FA)
         cpu = dict(value=head["bytes"] / secs / 1e9, unit="GB/s", cores=1, kind="port",
                    sample="one full G+L+R2 Roe FP64 evaluation of the same 160^3 mesh and state (%.1f s)" % secs,
                    build="oracle.c -O2 -ffp-contract=off (parity build)",
-                   parity_max_rel_diff=float(rel),
+                   parity_max_rel_diff=rel, parity_tolerance=1e-12, exact_build_max_rel_diff=rel_ex,
                    speed_build=dict(value=head["bytes"] / secs_f / 1e9, seconds=secs_f,
-                                    build="oracle.c -O3 -march=x86-64-v3", max_rel_diff_vs_gpu=float(rel_f)))
-        print("bench: cpu oracle %.2f s (speed build %.2f s), GPU-vs-oracle max rel diff %.3e (speed build %.1e)" % (
-            secs, secs_f, rel, rel_f), file=sys.stderr, flush=True)
+                                    build="oracle.c -O3 -march=x86-64-v3", max_rel_diff_vs_gpu=rel_f))
+        print("bench: cpu oracle %.2f s (speed build %.2f s), GPU-vs-oracle max rel diff: fast %.3e, exact %.3e "
+              "(speed build vs fast %.1e)" % (secs, secs_f, rel, rel_ex, rel_f), file=sys.stderr, flush=True)
+        if rel > 1e-12 or rel_ex != 0.0:
+            print("bench: PARITY FAILURE of the measured configuration", file=sys.stderr, flush=True)
+            sys.exit(1)
     dm.close()
-    solves = serving = batched = None
+    solves = solve3 = serving = batched = None
     if not args.no_solves and not args.only:
         del dm
         if "solve" in legs:
             solves = solve_leg(ctx, cpu=not args.no_cpu_solve)
+        if "solve3" in legs:
+            solve3 = solve3_leg(ctx)
         if "serve" in legs:
             nmin, nmax = (int(x) for x in args.serve_nodes.split(","))
             serving = serve_leg(ctx, args.serve_requests, args.serve_meshes, nmin, nmax, args.serve_inflight,
@@ -636,11 +661,12 @@ def main():
         if "batch" in legs and args.batch_cases:
             batched = batch_leg(ctx, args.batch_cases, args.batch_fp64_sample)
     ctx.close()
-    r_achieved = variants["roe-fp64-GL+R2"]["r_gbs"]
+    r_achieved = head["r_gbs"]
     copy_gbs = measured_copy_gbs()
     if copy_gbs:
         print("bench: device-to-device copy of 1 GiB: %.0f GB/s" % copy_gbs, file=sys.stderr, flush=True)
-    traffic_bytes = measured_traffic("k_residual") if args.n == 160 else None
+    traffic_bytes = measured_traffic("k_residual_patch") if args.n == 160 else None
+    exact = variants.get(exact_name)
     out = {
         "metric": "converged solves/s & p99 latency (mixed vs FP64 GPU); residual-eval GB/s",
         "value": round(head["gbs"], 2),
@@ -655,10 +681,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic (jittered cube mesh, random per-node state)",
         "config": {"workload": "residual microbenchmark: %d^3 jittered cube, %d tets, %d nodes, RCM; "
-                               "GG gradient + Venkatakrishnan limiter + 2nd-order Roe residual (Conv/Diss), FP64"
-                               % (args.n, 6 * args.n ** 3, N),
+                               "GG gradient + Venkatakrishnan limiter + 2nd-order Roe residual (Conv/Diss), FP64, "
+                               "fast build (within 1e-12 of the reference)" % (args.n, 6 * args.n ** 3, N),
                    "nodes": N, "edges": E, "ghosts": B, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_residual<double,Roe,2nd order,limited>",
+        "exact_build": {"value": round(exact["gbs"], 2), "ms_per_step": round(exact["ms_per_step"], 4),
+                        "residual_kernel": "k_residual<double,Roe,2nd order,limited>",
+                        "residual_gbs": round(exact["r_gbs"], 1), "residual_ms": round(exact["r_ms"], 4)}
+        if exact else None,
+        "roofline": {"bound": "hbm", "kernel": "k_residual_patch<double,Roe,2nd order,limited> (fast build)",
                      "achieved": round(r_achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(r_achieved / PEAK_HBM_GBS, 4), "traffic": traffic_bytes,
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/residual_cube160_fp64_hbm_counters.md"
@@ -666,9 +696,10 @@ def main():
                      "algorithmic_bytes_per_launch": bytes64["R2split"],
                      "measured_copy_peak": round(copy_gbs, 1) if copy_gbs else None,
                      "frac_of_measured_copy": round(r_achieved / copy_gbs, 4) if copy_gbs else None,
-                     "avg_launch_ms": round(variants["roe-fp64-GL+R2"]["r_ms"], 4)},
+                     "avg_launch_ms": round(head["r_ms"], 4)},
         "cpu_baseline": cpu,
         "solves": solves,
+        "solve_192x64x64": solve3,
         "serving": serving,
         "batched_small_cases": batched,
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}

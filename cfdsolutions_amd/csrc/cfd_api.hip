@@ -722,6 +722,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         if (!rc) m->d.isx = upload(m, isx.data(), isx.size(), &rc);
         if (!rc) m->d.isy = upload(m, isy.data(), isy.size(), &rc);
         if (!rc) m->d.isz = upload(m, isz.data(), isz.size(), &rc);
+        if (!rc) m->iperm = upload(m, iperm.data(), iperm.size(), &rc);
         if (!rc) m->n2n_ia = upload(m, nia.data(), nia.size(), &rc);
         if (!rc) m->n2n_ja = upload(m, nja.data(), nja.size(), &rc);
         if (!rc) m->bnode = upload(m, bnode.data(), bnode.size(), &rc);
@@ -795,14 +796,15 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         m->dt = (double *)cfd_dalloc(m, sizeof(double) * (size_t)N);
         m->b = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
         m->x = (double *)cfd_dalloc(m, sizeof(double) * 5 * (size_t)N);
-        m->mark = (int *)cfd_dalloc(m, sizeof(int) * (size_t)N);
+        m->roe_cand = (uint64_t *)cfd_dalloc(m, sizeof(uint64_t) * (size_t)N);
+        m->roe_blk = (int *)cfd_dalloc(m, sizeof(int) * (size_t)((N + 4095) / 4096));
         m->risk = (int *)cfd_dalloc(m, sizeof(int));
         m->red = (double *)cfd_dalloc(m, sizeof(double) * 1024 * 65);
         m->redout = (double *)cfd_dalloc(m, sizeof(double) * 4 * 65);
         m->flag = (int *)cfd_dalloc(m, sizeof(int) * 4);
         m->eigbits = (unsigned long long *)cfd_dalloc(m, sizeof(unsigned long long) * 6);
-        if (!m->d.qb || !m->d.lsq || !m->d.res || !m->dt || !m->b || !m->x || !m->mark || !m->risk ||
-            !m->red || !m->redout || !m->flag || !m->eigbits) {
+        if (!m->d.qb || !m->d.lsq || !m->d.res || !m->dt || !m->b || !m->x || !m->roe_cand || !m->roe_blk ||
+            !m->risk || !m->red || !m->redout || !m->flag || !m->eigbits) {
             cfd_mesh_destroy(m);
             return cfd_set_error(ctx, CFD_E_OOM, "cfd_mesh_create: out of device memory");
         }
@@ -812,7 +814,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             return cfd_set_error(ctx, CFD_E_OOM, "cfd_mesh_create: pinned allocation failed");
         }
         hipMemsetAsync(m->d.qb, 0, sizeof(double) * kQB * (size_t)N, m->stream);
-        hipMemsetAsync(m->mark, 0, sizeof(int) * (size_t)N, m->stream);
+        hipMemsetAsync(m->roe_blk, 0, sizeof(int) * (size_t)((N + 4095) / 4096), m->stream);
         hipMemsetAsync(m->flag, 0, sizeof(int) * 4, m->stream);
         hipMemsetAsync(m->d.res, 0, sizeof(double) * kRES * (size_t)N, m->stream);
         phase("workspaces");
@@ -846,8 +848,9 @@ extern "C" int64_t cfd_footprint_bytes(int32_t N, int32_t E, int32_t B, int32_t 
     int64_t bytes = 0;
     // qa, qb, lsq, res, vol, dt, boundary areas, right-hand side, update
     bytes += 8 * (kQA * (n + b) + kQB * n + 8 * n + kRES * n + n + n + 4 * b + 5 * n + 5 * n);
-    // order, inc_ptr, sptr, n2n lists, boundary node/type/tag, marks, ready flags, epoch, risk
-    bytes += 4 * (n + (n + 1) + (n / kSlice + 2) + (n + 1) + 2 * e + 3 * b + n + n + 2);
+    // order, inverse order, inc_ptr, sptr, n2n lists, boundary node/type/tag, Roe-pass candidates and block
+    // flags, ready flags, epoch, risk
+    bytes += 4 * (n + n + (n + 1) + (n / kSlice + 2) + (n + 1) + 2 * e + 3 * b + 2 * n + (n / 4096 + 1) + n + 2);
     bytes += nsell * (4 + 4 + 3 * 8);                       // nbr, soff, isx, isy, isz
     bytes += 4 * ((n + 1) + nnz + n) + 4 * nnz;             // BSR ia, ja, iau; ILU(0) merge offsets
     bytes += 4 * 8 * nnz;                                   // ILU(0) merge map (<= 8 upper blocks per lower entry)
@@ -913,6 +916,19 @@ int cfd_ensure_fp32(cfd_mesh *m) {
 // Patches hold kPatch consecutive nodes, restarting at every case of a union.
 // ---------------------------------------------------------------------------
 constexpr int kPatch = 256;  // nodes per patch: one per thread of the patch kernel's workgroup
+
+bool cfd_residual_uses_patches(const cfd_params *p) {
+    if (p->arith != CFD_ARITH_FAST) return false;
+    static const char *const once = getenv("CFD_FLUX_ONCE");
+    return once && (once[0] == '0' || once[0] == '1') ? once[0] == '1' : p->order == 2;
+}
+
+int64_t cfd_patch_footprint_bytes(int64_t n, int64_t e, int64_t b, int64_t cases) {
+    const int64_t inc = 2 * e + b, nsell = inc + inc / 4 + (int64_t)kSlice * 32;
+    // per work edge (at most one per incidence): its two nodes and its area
+    // vector in FP64 and FP32; a slot per incidence list entry; patch offsets
+    return inc * (4 + 4 + 32 + 16) + 4 * nsell + 8 * (n / kPatch + cases + 2);
+}
 
 int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
     if (!m->pw_ptr) {

@@ -162,7 +162,13 @@ struct cfd_mesh {
     size_t nsell = 0;
     int *n2n_ia = nullptr, *n2n_ja = nullptr;
     int *bnode = nullptr, *btype = nullptr, *btag = nullptr;  // [B]
-    int *mark = nullptr;      // [N] limiter correction marks
+    // Roe pressure correction: bit n of roe_cand[a] set where node a's test
+    // with its n-th neighbour (reference numbering and order) can zero a
+    // limiter (bit 63: every neighbour from the 63rd on); roe_blk[a >> 12] ==
+    // mark_epoch where any node of that block has one.
+    uint64_t *roe_cand = nullptr;  // [N]
+    int *roe_blk = nullptr;        // [ceil(N / 4096)]
+    int *iperm = nullptr;          // [N] internal index of each reference node
     int *risk = nullptr;      // [1] nodes with a near-vacuum face state
     int mark_epoch = 0;
 
@@ -296,6 +302,12 @@ int cfd_ensure_fp32(cfd_mesh *m);
 // incidence lists (once per mesh) and the FP32 copy of their area vectors
 // when fp32 is set.
 int cfd_ensure_patches(cfd_mesh *m, bool fp32);
+// Whether the residual of these parameters runs the flux-once patch kernel:
+// the fast build at second order, unless CFD_FLUX_ONCE=0 or 1 forces it.
+bool cfd_residual_uses_patches(const cfd_params *p);
+// Upper bound of the device bytes of a mesh's work-edge patches (n nodes,
+// e edges, b boundary edges, split into `cases` cases).
+int64_t cfd_patch_footprint_bytes(int64_t n, int64_t e, int64_t b, int64_t cases);
 
 #define CFD_HIP(m, call)                                                   \
     do {                                                                   \

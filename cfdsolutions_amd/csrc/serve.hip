@@ -48,6 +48,7 @@ struct Resident {
     cfd_mesh *mesh;
     int64_t bytes;  // bytes reserved against the budget
     int32_t krylov_m, mixed;
+    bool patches;   // the reservation covers the flux-once residual's patches
     bool busy;
     uint64_t last_use;
 };
@@ -130,7 +131,10 @@ bool admit_batch(cfd_server *s, size_t q, size_t look, Job &job) {
     }
     if (take.size() < 2 || nn > INT32_MAX / 4 || ne > INT32_MAX / 4 || nb > INT32_MAX / 4) return false;
     const int64_t need = cfd_footprint_bytes((int32_t)nn, (int32_t)ne, (int32_t)nb, krylov_of(p.req.opts),
-                                             p.req.opts.lin.mixed);
+                                             p.req.opts.lin.mixed) +
+                         (cfd_residual_uses_patches(&p.req.params)
+                              ? cfd_patch_footprint_bytes(nn, ne, nb, (int64_t)take.size())
+                              : 0);
     if (need > s->opts.hbm_budget || !make_room(s, need)) return false;
     s->in_use += need;
     s->peak = std::max(s->peak, s->in_use);
@@ -185,16 +189,18 @@ bool admit(cfd_server *s, Job &job, double &retry_at) {
             if (admit_batch(s, q, look, job)) return true;
         }
         const int km = krylov_of(p.req.opts);
+        const bool patches = cfd_residual_uses_patches(&p.req.params);
         Resident *r = nullptr;
         for (auto &cm : s->meshes)
-            if (!cm.busy && cm.key == p.req.mesh_key && cm.mixed == p.req.opts.lin.mixed && cm.krylov_m >= km) {
+            if (!cm.busy && cm.key == p.req.mesh_key && cm.mixed == p.req.opts.lin.mixed && cm.krylov_m >= km &&
+                (cm.patches || !patches)) {
                 r = &cm;
                 break;
             }
         if (r) {
             s->reused++;
         } else if (make_room(s, p.need)) {
-            s->meshes.push_back(Resident{p.req.mesh_key, nullptr, p.need, km, p.req.opts.lin.mixed, false, 0});
+            s->meshes.push_back(Resident{p.req.mesh_key, nullptr, p.need, km, p.req.opts.lin.mixed, patches, false, 0});
             r = &s->meshes.back();
             s->in_use += p.need;
             s->peak = std::max(s->peak, s->in_use);
@@ -394,7 +400,10 @@ extern "C" int cfd_server_create(cfd_ctx *ctx, const cfd_server_opts *o, cfd_ser
 extern "C" int cfd_server_submit(cfd_server *s, const cfd_request *r) {
     if (!s || !r || !r->mesh) return cfd_set_error(s ? s->ctx : nullptr, CFD_E_ARG, "cfd_server_submit: null argument");
     const cfd_mesh_desc *d = r->mesh;
-    const int64_t need = cfd_footprint_bytes(d->n_node, d->n_edge, d->n_bedge, krylov_of(r->opts), r->opts.lin.mixed);
+    const int64_t need = cfd_footprint_bytes(d->n_node, d->n_edge, d->n_bedge, krylov_of(r->opts), r->opts.lin.mixed) +
+                         (cfd_residual_uses_patches(&r->params)
+                              ? cfd_patch_footprint_bytes(d->n_node, d->n_edge, d->n_bedge, 1)
+                              : 0);
     if (need > s->opts.hbm_budget)
         return cfd_set_error(s->ctx, CFD_E_OOM, "request %lld needs %lld bytes, budget is %lld", (long long)r->id,
                              (long long)need, (long long)s->opts.hbm_budget);

@@ -324,3 +324,91 @@ def test_fp32_residual_close_to_fp64(case):
         r32 = dm.get(cfd.FIELD_RES_CONV) + dm.get(cfd.FIELD_RES_DISS)
         err = np.max(np.abs(r32 - r64)) / np.max(np.abs(r64))
         assert err < 1e-4, (flux, err)
+
+
+@pytest.mark.parametrize("flux", ["hllc", "ausm+"])
+def test_finite_difference_jacobian_of_scheme_flux_matches_oracle(case, flux):
+    """cfd_jacobian_fd for HLLC and AUSM+ differentiates the scheme's own flux with the reference's perturbation
+    sequence (MPFLOW3D/src/FiniteDifference_Jacobian.cpp:112-667): bit for bit the oracle's, whose product with a
+    small state change equals the residual's change (tests/test_oracle_closed_form.py)."""
+    d, dm, p = case
+    if d["meta"]["args"]["amp"] > 0.1:
+        pytest.skip("strongly perturbed state: the scheme's faces leave the states this case was built for")
+    om = oracle_mesh(d, orc)
+    pf = gpu_params(d, flux=flux)
+    op = oracle_params(d, orc)
+    op.flux = {"hllc": orc.HLLC, "ausm+": orc.AUSMP}[flux]
+    q = d["q_bc"]
+    dm.set(cfd.FIELD_Q, q)
+    grad = phi = None
+    if "grad" in d:
+        grad = d["grad"]
+        dm.set(cfd.FIELD_GRAD, grad)
+    if "phi" in d:
+        phi = d["phi"]
+        dm.set(cfd.FIELD_PHI, phi)
+    dm.residual(pf)
+    rc, rd = dm.get(cfd.FIELD_RES_CONV), dm.get(cfd.FIELD_RES_DISS)
+    dm.set(cfd.FIELD_DT, d["dt"])
+    it = d["meta"]["args"]["iter"]
+    ia, ja, iau = dm.bsr_structure()
+    for method in (cfd.JAC_FD_CENTRAL, cfd.JAC_FD_FORWARD):
+        print("finite-difference", flux, "method", method, flush=True)
+        dm.jacobian_fd(pf, method=method, it=it, add_time=True, rhs_mode=1)
+        a, b = dm.jacobian_get()
+        ao, bo, bad = orc.jacobian_fd(om, op, q, grad, phi, d["dt"], rc, rd, 1, method, it, 1, ia, ja, iau)
+        assert bad == 0
+        assert_bitexact(a, ao, "finite-difference Jacobian (%s)" % flux)
+        assert_bitexact(b, bo, "right-hand side")
+
+
+@pytest.mark.parametrize("limiter", [cfd.LIM_VENKAT, cfd.LIM_BJ])
+def test_roe_pressure_pass_with_neighbouring_flagged_nodes(case, limiter):
+    """The Roe pressure correction acting (Euler3D/src/Limiters.cpp:71-110, 311-356): clusters of interior nodes
+    with negative pressure (a node and two of its neighbours) make Roe-averaged face pressures negative, so the
+    sequential pass zeroes limiters of neighbouring nodes.  The GPU's pass (candidate tests found in parallel, then
+    replayed in the reference's node and neighbour order) equals the oracle's sequential sweep bit for bit, after the
+    LSQ limiter and after the fused Green-Gauss limiter."""
+    d, dm, _ = case
+    if "grad" not in d:
+        pytest.skip("first-order case")
+    p = gpu_params(d, flux="roe")
+    p.order, p.limiter = 2, limiter
+    q = np.array(d["q_bc"], dtype=np.float64)
+    n = d["xyz"].shape[0]
+    g = float(d["inf"][8])
+    ia, ja = d["n2n_ia"], d["n2n_ja"]
+    boundary = set(int(i) for i in d["bedge"][:, 0])
+    interior = [i for i in range(n) if i not in boundary]
+    pres = {}
+    for s in interior[::9]:
+        pres[s] = -1.5 * float(d["inf"][4])
+        for k in range(ia[s], ia[s] + 2):
+            if int(ja[k]) not in boundary:
+                pres.setdefault(int(ja[k]), -0.6 * float(d["inf"][4]))
+    idx = np.array(sorted(pres))
+    kin = 0.5 * (q[1, idx] ** 2 + q[2, idx] ** 2 + q[3, idx] ** 2) / q[0, idx]
+    q[4, idx] = kin + np.array([pres[i] for i in idx]) / (g - 1.0)
+    om = oracle_mesh(d, orc)
+    op = oracle_params(d, orc)
+    op.flux, op.order = orc.ROE, 2
+    op.limiter = orc.LIM_VENKAT if limiter == cfd.LIM_VENKAT else orc.LIM_BJ
+    oh = oracle_params(d, orc)
+    oh.flux, oh.order, oh.limiter = orc.HLLC, 2, op.limiter
+    for gradient in (cfd.GRAD_LSQ, cfd.GRAD_GG):
+        p.gradient = gradient
+        dm.set(cfd.FIELD_Q, q)
+        og = orc.lsq_gradient(om, q) if gradient == cfd.GRAD_LSQ else orc.gg_gradient(om, q)
+        ophi = np.asarray(orc.limiter(om, op, q, og)).reshape(5, -1)
+        pre = np.asarray(orc.limiter(om, oh, q, og)).reshape(5, -1)  # the same limiter without the Roe pass
+        by_pass = np.flatnonzero(np.all(ophi == 0.0, axis=0) & ~np.all(pre == 0.0, axis=0))
+        assert by_pass.size > 0, "the Roe pass did not act"
+        flagged = set(by_pass.tolist())
+        assert any(int(ja[k]) in flagged for i in by_pass for k in range(ia[i], ia[i + 1])), \
+            "no two neighbouring nodes zeroed by the pass"
+        if gradient == cfd.GRAD_LSQ:
+            dm.gradient(p)
+            dm.limiter(p)
+        else:
+            dm.gradient_limiter(p)
+        assert_bitexact(dm.get(cfd.FIELD_PHI), ophi, "limiter after the Roe pass (%d nodes zeroed by it)" % by_pass.size)

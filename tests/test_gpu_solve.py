@@ -112,3 +112,25 @@ def test_multicolour_ilu_solve_converges_to_the_same_state(ctx):
     ref = out[(cfd.PREC_ILU0, False)]
     for key, r in out.items():
         np.testing.assert_allclose(list(r.force), list(ref.force), rtol=1e-8, atol=1e-10)
+
+
+def test_finite_difference_jacobian_solve(ctx):
+    """Newton with the finite-difference Jacobian of the HLLC flux (CFD_SOLVE_JAC_FD) converges to the state the
+    approximate Roe Jacobian reaches (forces within 1e-8), in no more Newton iterations."""
+    print("finite-difference Jacobian solve", flush=True)
+    m = cmesh.bump_channel(12, 4, 4)
+    p = cfd.make_params(mach=0.6, alpha_deg=2.0, flux="hllc", order=2, limiter=cfd.LIM_VENKAT)
+    res = {}
+    for jac in (cfd.SOLVE_JAC_APPROX, cfd.SOLVE_JAC_FD):
+        dm = cfd.DeviceMesh(ctx, m, p)
+        r, _ = dm.solve_steady(p, max_iter=300, tol=1e-10, krylov_m=30, max_restarts=4, rtol=1e-3,
+                               precond=cfd.PREC_ILU0, alpha_deg=2.0, jacobian=jac)
+        res[jac] = r
+        dm.close()
+    a, f = res[cfd.SOLVE_JAC_APPROX], res[cfd.SOLVE_JAC_FD]
+    print("newton: approximate Roe %d, finite differences %d" % (a.iterations, f.iterations), flush=True)
+    assert a.converged and f.converged
+    assert f.iterations <= a.iterations
+    fa, ff = np.array(a.force), np.array(f.force)
+    assert np.max(np.abs(ff - fa)) <= 1e-8 * np.max(np.abs(fa))
+    assert abs(f.lift - a.lift) <= 1e-8 * np.max(np.abs(fa)) and abs(f.drag - a.drag) <= 1e-8 * np.max(np.abs(fa))
