@@ -305,6 +305,13 @@ class DeviceMesh:
         self.ctx.check(lib().cfd_linsolve(self.h, ctypes.byref(o), ctypes.byref(st)))
         return st
 
+    def linsolve_gs_noalloc(self, sweeps):
+        """cfd_linsolve_gs_noalloc: the block Gauss-Seidel solve without device allocation (meshes of at most
+        32,768 nodes).  Returns (status, LinsolveStats)."""
+        st = LinsolveStats()
+        rc = lib().cfd_linsolve_gs_noalloc(self.h, int(sweeps), ctypes.byref(st))
+        return rc, st
+
     def update(self, relax=1.0):
         self.ctx.check(lib().cfd_update(self.h, ctypes.c_double(relax)))
 

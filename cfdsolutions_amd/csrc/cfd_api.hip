@@ -90,6 +90,14 @@ extern "C" int cfd_ctx_create(int device, cfd_ctx **out) {
     cfd_ctx *c = new (std::nothrow) cfd_ctx();
     if (!c) return CFD_E_OOM;
     c->device = device;
+    // this GPU's copy of the allocation-free solve's scratch array; a failed
+    // lookup only disables cfd_linsolve_gs_noalloc
+    if (hipSetDevice(device) == hipSuccess) {
+        hipError_t e = hipSuccess;
+        c->noalloc_scratch = noalloc_scratch_address(&e);
+        if (!c->noalloc_scratch)
+            cfd_set_error(c, CFD_E_HIP, "scratch array of cfd_linsolve_gs_noalloc: %s", hipGetErrorString(e));
+    }
     *out = c;
     return CFD_OK;
 }
@@ -1455,6 +1463,13 @@ extern "C" int cfd_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolv
     hipSetDevice(m->ctx->device);
     cfd_linsolve_stats tmp;
     return run_linsolve(m, o, st ? st : &tmp);
+}
+
+extern "C" int cfd_linsolve_gs_noalloc(cfd_mesh *m, int sweeps, cfd_linsolve_stats *st) {
+    if (!m) return CFD_E_ARG;
+    hipSetDevice(m->ctx->device);
+    cfd_linsolve_stats tmp;
+    return run_gs_noalloc(m, sweeps, st ? st : &tmp);
 }
 
 extern "C" int cfd_update(cfd_mesh *m, double relax) {

@@ -18,6 +18,10 @@ struct cfd_ctx {
     int device = 0;
     std::mutex lock;  // guards err
     std::string err;
+    // device address of this GPU's copy of the scratch array of
+    // cfd_linsolve_gs_noalloc (looked up when the context is created), or
+    // NULL if the lookup failed
+    double *noalloc_scratch = nullptr;
 };
 
 // Device buffers of one precision for the residual path.  Node data used by
@@ -387,6 +391,9 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
 int launch_jacobian_fd(cfd_mesh *m, const cfd_params *p, int method, int iter, int add_time, int rhs_mode,
                        int mixed);
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);
+// The block Gauss-Seidel solve of run_linsolve with its scratch in the
+// file-scope array (cfd_linsolve_gs_noalloc): allocates nothing.
+int run_gs_noalloc(cfd_mesh *m, int sweeps, cfd_linsolve_stats *st);
 // Batched FGMRES over a union mesh: solves the cases with solve[c] != 0,
 // writes each case's iterations, final residual estimate and status (a case
 // whose Krylov residual turns non-finite gets CFD_E_NONFINITE and no
@@ -394,3 +401,7 @@ int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st
 int run_linsolve_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const std::vector<char> &solve,
                        std::vector<int> &iters, std::vector<double> &final_res, std::vector<int> &status);
 void drop_krylov_graphs(cfd_mesh *m);
+// Scratch of cfd_linsolve_gs_noalloc (linsolve.hip): the device address of
+// the current GPU's copy of the file-scope array, or NULL with *err set.
+constexpr int kNoAllocMaxNodes = 32768;
+double *noalloc_scratch_address(hipError_t *err);

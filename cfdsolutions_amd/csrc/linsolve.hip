@@ -2060,6 +2060,38 @@ static int gs_sweeps(cfd_mesh *m, int sweeps, double *gslu, double *rowbuf, cfd_
     return CFD_OK;
 }
 
+// Scratch of cfd_linsolve_gs_noalloc: the LU factors of the diagonal blocks
+// (25 values per row) and the per-row sums (1 per row) of meshes of up to
+// kNoAllocMaxNodes nodes, 6.8 MB.  Each GPU holds its own copy until the
+// process ends; host code uses the address hipGetSymbolAddress returns for
+// the current GPU (kept per context), never the array's name.  Every call on
+// a GPU shares that GPU's copy, so calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[26 * (size_t)kNoAllocMaxNodes];
+
+double *noalloc_scratch_address(hipError_t *err) {
+    void *p = nullptr;
+    *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    return *err == hipSuccess ? (double *)p : nullptr;
+}
+
+int run_gs_noalloc(cfd_mesh *m, int sweeps, cfd_linsolve_stats *st) {
+    *st = cfd_linsolve_stats{};
+    if (!m->jac_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "cfd_linsolve_gs_noalloc: no Jacobian assembled");
+    if (m->jac_mixed) return cfd_set_error(m->ctx, CFD_E_ARG, "cfd_linsolve_gs_noalloc: needs an FP64 Jacobian");
+    if (m->N > kNoAllocMaxNodes)
+        return cfd_set_error(m->ctx, CFD_E_ARG,
+                             "cfd_linsolve_gs_noalloc: %d nodes exceed the scratch array's %d (use cfd_linsolve)",
+                             m->N, kNoAllocMaxNodes);
+    if (!m->ctx->noalloc_scratch)
+        return cfd_set_error(m->ctx, CFD_E_HIP, "cfd_linsolve_gs_noalloc: scratch array address unavailable");
+    double *gslu = m->ctx->noalloc_scratch, *rowbuf = gslu + 25 * (size_t)kNoAllocMaxNodes;
+    const int rc = gs_sweeps(m, sweeps > 0 ? sweeps : 10000, gslu, rowbuf, st);
+    if (rc) return rc;
+    CFD_HIP(m, hipGetLastError());
+    CFD_HIP(m, hipStreamSynchronize(m->stream));
+    return CFD_OK;
+}
+
 int run_linsolve(cfd_mesh *m, const cfd_linsolve_opts *o, cfd_linsolve_stats *st) {
     if (!m->jac_valid) return cfd_set_error(m->ctx, CFD_E_STATE, "cfd_linsolve: no Jacobian assembled");
     *st = cfd_linsolve_stats{};

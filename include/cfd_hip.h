@@ -264,6 +264,16 @@ int cfd_jacobian_set(cfd_mesh *mesh, const double *a, const double *b);
  * (MATH/src/MC_Iterative_Jacobi.c:346-717). X stays on the device
  * (CFD_FIELD_X). */
 int cfd_linsolve(cfd_mesh *mesh, const cfd_linsolve_opts *o, cfd_linsolve_stats *st);
+/* No counterpart in the reference: cfd_linsolve with CFD_SOLVER_BLOCK_GS
+ * (MC_Iterative_Block_LU_Jacobi_CRS, forward) for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It allocates nothing: the LU factors of
+ * the diagonal blocks and the per-row sums live in a file-scope device array
+ * sized for meshes of at most 32,768 nodes (6.8 MB per GPU, held until the
+ * process ends); larger meshes get CFD_E_ARG.  Same results as cfd_linsolve,
+ * bit for bit.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU race. */
+int cfd_linsolve_gs_noalloc(cfd_mesh *mesh, int sweeps, cfd_linsolve_stats *st);
 /* Q += relaxation * X (Solver_Steady_Implicit.cpp:150-156). */
 int cfd_update(cfd_mesh *mesh, double relaxation);
 /* Pressure force on wall boundary edges: F = sum p_node S_out. */
