@@ -502,7 +502,7 @@ def main():
     ap.add_argument("--n", type=int, default=160, help="hexes per side of the cube")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--only", action="store_true", help="run only the headline variant (for profiling)")
+    ap.add_argument("--only", action="store_true", help="run only the headline (fast build) and its exact-build twin (for profiling)")
     ap.add_argument("--no-solves", action="store_true", help="skip the steady-solve and serving legs")
     ap.add_argument("--no-cpu-solve", action="store_true", help="skip the oracle's CPU timing of the 48x16x16 solve")
     ap.add_argument("--legs", type=str, default="residual,solve,solve3,serve,batch",
