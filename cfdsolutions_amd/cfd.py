@@ -9,6 +9,7 @@ import os
 import numpy as np
 
 from . import _build
+from . import mesh as _mesh
 
 OK = 0
 E_ARG, E_HIP, E_OOM, E_NONFINITE, E_NEGATIVE_PRESSURE, E_SINGULAR, E_HLLC, E_STATE = -1, -2, -3, -4, -5, -6, -7, -8
@@ -121,6 +122,49 @@ def mesh_desc(mesh, bedge_type=None):
                  _p(k["bedge"]), _p(k["bedge_area"]), _p(k["bedge_type"]), _p(k["bedge_tag"]), _p(k["xyz"]),
                  _p(k["vol"]), _p(k["n2n_ia"]), _p(k["n2n_ja"]))
     return d, keep
+
+
+class _PrepCounts(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("n_node", "n_tri", "n_tet", "n_edge", "n_bedge", "n_n2n")]
+
+
+class PreprocessedMesh:
+    """A mesh preprocessed on the GPU by cfd_preprocess: the arrays of mesh.Mesh, bit-identical to the host
+    library's, from the UGRID content (xyz, triangles in UGRID winding, their tags, tetrahedra)."""
+
+    def __init__(self, ctx, content):
+        c = np.ascontiguousarray
+        xyz, tri = c(content.xyz, np.float64), c(content.tri, np.int32)
+        tag, tet = c(content.tri_tag, np.int32), c(content.tet, np.int32)
+        h = ctypes.c_void_p()
+        ctx.check(lib().cfd_preprocess(ctx.h, int(xyz.shape[0]), _p(xyz), int(tri.shape[0]), _p(tri), _p(tag),
+                                       int(tet.shape[0]), _p(tet), ctypes.byref(h)))
+        try:
+            k = _PrepCounts()
+            ctx.check(lib().cfd_prep_info(h, ctypes.byref(k)))
+            n, e, b = k.n_node, k.n_edge, k.n_bedge
+            self.n_node, self.n_edge, self.n_bedge, self.n_tet, self.n_tri = n, e, b, k.n_tet, k.n_tri
+            self.xyz, self.tet, self.tri_tag = xyz.copy(), tet.copy(), tag.copy()
+            self.tri = np.zeros((k.n_tri, 3), np.int32)
+            self.edge = np.zeros((e, 2), np.int32)
+            self.edge_area = np.zeros((e, 3))
+            self.bedge = np.zeros((b, 2), np.int32)
+            self.bedge_area = np.zeros((b, 3))
+            self.bedge_tag = np.zeros(b, np.int32)
+            self.vol = np.zeros(n)
+            self.n2n_ia = np.zeros(n + 1, np.int32)
+            self.n2n_ja = np.zeros(k.n_n2n, np.int32)
+            ctx.check(lib().cfd_prep_get(h, _p(self.tri), _p(self.edge), _p(self.edge_area), _p(self.bedge),
+                                         _p(self.bedge_area), _p(self.bedge_tag), _p(self.vol), _p(self.n2n_ia),
+                                         _p(self.n2n_ja)))
+        finally:
+            lib().cfd_prep_destroy(h)
+        # the bump channel's boundary conditions, as mesh.Mesh
+        self.bedge_type = np.where(self.bedge_tag <= 2, _mesh.BC_FAR_FIELD, _mesh.BC_EULER_WALL).astype(np.int32)
+
+    @property
+    def n_total(self):
+        return self.n_node + self.n_bedge
 
 
 _LIB = None

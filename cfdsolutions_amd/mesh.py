@@ -98,6 +98,36 @@ def box(nx, ny, nz, lx=3.0, ly=1.0, lz=1.0, bump_h=0.1, jitter=0.0, seed=2026120
         lib.cfdm_free(h)
 
 
+class Content:
+    """UGRID content of a mesh before preprocessing: xyz [N,3], tri [NT,3] in UGRID winding, tri_tag [NT],
+    tet [NC,4] (the input of cfd_preprocess)."""
+
+    def __init__(self, handle):
+        lib = _lib()
+        info = _Info()
+        _check(lib.cfdm_get_info(handle, ctypes.byref(info)))
+        self.n_node, self.n_tri, self.n_tet = info.n_node, info.n_tri, info.n_tet
+        self.xyz = np.zeros((info.n_node, 3))
+        self.tet = np.zeros((info.n_tet, 4), np.int32)
+        self.tri = np.zeros((info.n_tri, 3), np.int32)
+        self.tri_tag = np.zeros(info.n_tri, np.int32)
+        _check(lib.cfdm_get_arrays(handle, _p(self.xyz), _p(self.tet), _p(self.tri), _p(self.tri_tag), None, None,
+                                   None, None, None, None, None, None))
+
+
+def box_content(nx, ny, nz, lx=3.0, ly=1.0, lz=1.0, bump_h=0.1, jitter=0.0, seed=20261202, reorder=True):
+    """The UGRID content box() would preprocess, unpreprocessed."""
+    lib = _lib()
+    h = ctypes.c_void_p()
+    _check(lib.cfdm_box_generate(int(nx), int(ny), int(nz), ctypes.c_double(lx), ctypes.c_double(ly),
+                                 ctypes.c_double(lz), ctypes.c_double(bump_h), ctypes.c_double(jitter),
+                                 ctypes.c_uint64(seed), int(bool(reorder)), ctypes.byref(h)))
+    try:
+        return Content(h)
+    finally:
+        lib.cfdm_free(h)
+
+
 def read_ugrid(path):
     lib = _lib()
     h = ctypes.c_void_p()

@@ -183,6 +183,28 @@ int cfd_ctx_create(int device, cfd_ctx **out);
 void cfd_ctx_destroy(cfd_ctx *ctx);
 const char *cfd_last_error(const cfd_ctx *ctx);
 
+/* Median-dual preprocessing on the GPU: from the UGRID content (nodes,
+ * boundary triangles in UGRID winding with their tags, tetrahedra) to the
+ * arrays of cfd_mesh_desc, bit-identical to the host library's
+ * cfdm_preprocess and to the reference's Create_Connectivity_Maps
+ * (Euler3D/src/Connectivity_Maps.cpp:44-340, 733-910) and
+ * Calculate_Area_Volume (Euler3D/src/Area_Volume.cpp:462-821, 903-917).
+ * The result is held on the host until cfd_prep_destroy. */
+typedef struct cfd_prep cfd_prep;
+typedef struct cfd_prep_counts {
+    int32_t n_node, n_tri, n_tet, n_edge, n_bedge, n_n2n;
+} cfd_prep_counts;
+int cfd_preprocess(cfd_ctx *ctx, int32_t n_node, const double *xyz, int32_t n_tri, const int32_t *tri,
+                   const int32_t *tri_tag, int32_t n_tet, const int32_t *tet, cfd_prep **out);
+int cfd_prep_info(const cfd_prep *prep, cfd_prep_counts *counts);
+/* Copies out the arrays (any pointer may be NULL), laid out as
+ * cfdm_get_arrays: tri [3*n_tri] after winding translation and orientation,
+ * edge [2E], edge_area [3E], bedge [2B], bedge_area [3B], bedge_tag [B],
+ * vol [N], n2n_ia [N+1], n2n_ja [n_n2n]. */
+int cfd_prep_get(const cfd_prep *prep, int32_t *tri, int32_t *edge, double *edge_area, int32_t *bedge,
+                 double *bedge_area, int32_t *bedge_tag, double *vol, int32_t *n2n_ia, int32_t *n2n_ja);
+void cfd_prep_destroy(cfd_prep *prep);
+
 /* Uploads a mesh and builds the device structures: node incidence lists in
  * the reference's accumulation order, LSQ coefficients
  * (Euler3D/src/Gradient.cpp:137-222), BSR structure
