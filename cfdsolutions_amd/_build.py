@@ -13,7 +13,8 @@ INC = os.path.join(ROOT, "include")
 CSRC = os.path.join(HERE, "csrc")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["cfd_api.hip", "residual.hip", "residual_fast.hip", "linsolve.hip", "serve.hip", "batch.hip", "prep.hip"]
+HIP_SOURCES = ["cfd_api.hip", "residual.hip", "residual_fast.hip", "linsolve.hip", "serve.hip", "batch.hip", "prep.hip",
+               "tune.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # parity build: no FMA contraction, so FP64 results match the CPU

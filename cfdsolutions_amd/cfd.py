@@ -65,7 +65,7 @@ class LinsolveStats(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int32), ("tol", ctypes.c_double), ("first_order_iters", ctypes.c_int32),
                 ("relaxation", ctypes.c_double), ("wall_tag", ctypes.c_int32), ("alpha_deg", ctypes.c_double),
-                ("lin", LinsolveOpts), ("jacobian", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("lin", LinsolveOpts), ("jacobian", ctypes.c_int32), ("autotune", ctypes.c_int32)]
 
 
 class SolveResult(ctypes.Structure):
@@ -73,6 +73,11 @@ class SolveResult(ctypes.Structure):
                 ("status", ctypes.c_int32), ("first_rms", ctypes.c_double), ("final_rms", ctypes.c_double),
                 ("force", ctypes.c_double * 3), ("lift", ctypes.c_double), ("drag", ctypes.c_double),
                 ("pressure_integral", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("gl_ms", ctypes.c_double * 2), ("krylov_ms", (ctypes.c_double * 2) * 2), ("gl_stage", ctypes.c_int32),
+                ("sell", ctypes.c_int32), ("sync_free", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class ServerOpts(ctypes.Structure):
@@ -103,9 +108,10 @@ class ServerStats(ctypes.Structure):
 
 def solve_opts(max_iter=200, tol=1e-10, first_order_iters=0, relaxation=1.0, wall_tag=3, alpha_deg=0.0,
                precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, mixed=False, solver=SOLVER_FGMRES,
-               sweeps=0, jacobian=0):
+               sweeps=0, jacobian=0, autotune=False):
     lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
-    return SolveOpts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, lin, int(jacobian), 0)
+    return SolveOpts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, lin, int(jacobian),
+                     int(bool(autotune)))
 
 
 def mesh_desc(mesh, bedge_type=None):
@@ -349,6 +355,15 @@ class DeviceMesh:
         self.ctx.check(lib().cfd_linsolve(self.h, ctypes.byref(o), ctypes.byref(st)))
         return st
 
+    def autotune(self, p, precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, mixed=False,
+                 solver=SOLVER_FGMRES, sweeps=0):
+        """cfd_autotune: times the same-bits kernel variants on this mesh (the Krylov ones on the assembled
+        Jacobian, if any) and keeps the fastest.  Returns the TuneReport."""
+        lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
+        rep = TuneReport()
+        self.ctx.check(lib().cfd_autotune(self.h, ctypes.byref(p), ctypes.byref(lin), ctypes.byref(rep)))
+        return rep
+
     def linsolve_gs_noalloc(self, sweeps):
         """cfd_linsolve_gs_noalloc: the block Gauss-Seidel solve without device allocation (meshes of at most
         32,768 nodes).  Returns (status, LinsolveStats)."""
@@ -371,9 +386,9 @@ class DeviceMesh:
 
     def solve_steady(self, p, max_iter=200, tol=1e-10, first_order_iters=0, relaxation=1.0, wall_tag=3,
                      alpha_deg=0.0, precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, mixed=False,
-                     solver=SOLVER_FGMRES, sweeps=0, jacobian=SOLVE_JAC_APPROX, check=True):
+                     solver=SOLVER_FGMRES, sweeps=0, jacobian=SOLVE_JAC_APPROX, autotune=False, check=True):
         o = solve_opts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, precond, krylov_m,
-                       max_restarts, rtol, mixed, solver, sweeps, jacobian)
+                       max_restarts, rtol, mixed, solver, sweeps, jacobian, autotune)
         r = SolveResult()
         hist = np.zeros(max_iter)
         rc = lib().cfd_solve_steady(self.h, ctypes.byref(p), ctypes.byref(o), ctypes.byref(r), _p(hist))

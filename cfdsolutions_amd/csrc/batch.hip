@@ -572,6 +572,13 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
         if (rc) break;
         solve.assign(Kn, 0);
         for (int lc = 0; lc < Kn; lc++) solve[lc] = active[u.cases[lc]];
+        if (o.autotune && !m->tuned) {  // each union mesh (a compaction builds a new one) tunes once
+            cfd_tune_report tr;
+            rc = autotune_mesh(m, &pi, &o.lin, &tr,
+                               [&] { return run_linsolve_batch(m, &o.lin, solve, iters, fres, lstatus); });
+            if (rc) break;
+            m->tuned = true;
+        }
         rc = run_linsolve_batch(m, &o.lin, solve, iters, fres, lstatus);
         if (rc) break;
         for (int lc = 0; !rc && lc < Kn; lc++)

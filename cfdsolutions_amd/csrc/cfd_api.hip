@@ -1551,6 +1551,12 @@ extern "C" int cfd_solve_steady(cfd_mesh *m, const cfd_params *p0, const cfd_sol
         if (!rc) rc = solve_jacobian(m, pi, o, it);
         if (rc) break;
         cfd_linsolve_stats st{};
+        if (o->autotune && !m->tuned) {
+            cfd_tune_report tr;
+            rc = autotune_mesh(m, &pi, &o->lin, &tr, [&] { return run_linsolve(m, &o->lin, &st); });
+            if (rc) break;
+            m->tuned = true;
+        }
         rc = cfd_linsolve(m, &o->lin, &st);
         if (rc) break;
         res->krylov_total += st.iterations;

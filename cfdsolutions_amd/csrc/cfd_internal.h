@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
+#include <functional>
 #include <mutex>
 #include <shared_mutex>
 #include <string>
@@ -239,6 +240,12 @@ struct cfd_mesh {
     int prec_kind = -1;
     bool prec_mixed = false;
 
+    // autotuned variants (cfd_autotune): gather mode of the fused
+    // gradient+limiter per precision (FP64, FP32; -1: the fixed rule), and
+    // whether a steady solve has tuned this mesh
+    int8_t tune_gl_stage[2] = {-1, -1};
+    bool tuned = false;
+
     // Krylov workspace
     bool use_graphs = true;   // replay captured Arnoldi steps
     KrylovGraphs kgraph;
@@ -401,6 +408,10 @@ int run_gs_noalloc(cfd_mesh *m, int sweeps, cfd_linsolve_stats *st);
 int run_linsolve_batch(cfd_mesh *m, const cfd_linsolve_opts *o, const std::vector<char> &solve,
                        std::vector<int> &iters, std::vector<double> &final_res, std::vector<int> &status);
 void drop_krylov_graphs(cfd_mesh *m);
+// Autotuning (tune.hip): times the same-bits kernel variants on this mesh and
+// keeps the fastest; linsolve runs one linear solve of the assembled system.
+int autotune_mesh(cfd_mesh *m, const cfd_params *p, const cfd_linsolve_opts *lin, cfd_tune_report *rep,
+                  const std::function<int()> &linsolve);
 // Scratch of cfd_linsolve_gs_noalloc (linsolve.hip): the device address of
 // the current GPU's copy of the file-scope array, or NULL with *err set.
 constexpr int kNoAllocMaxNodes = 32768;

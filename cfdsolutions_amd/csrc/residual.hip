@@ -1446,10 +1446,14 @@ int launch_gradient_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p
     const T g = (T)p->gamma;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, m->stream, 1, 0, m->risk);
     // neighbour records staged in LDS for FP32, direct gathers for FP64 (the
-    // faster of the two for each precision on the 160^3 cube; CFD_STAGE_GL=0/1
-    // forces either)
-    static const char *const env = getenv("CFD_STAGE_GL");
-    const bool stage = env ? env[0] == '1' : sizeof(T) == 4;
+    // faster of the two for each precision on the 160^3 cube), unless the
+    // autotuner chose for this mesh; CFD_STAGE_GL=0/1 (read at each call)
+    // forces either
+    const char *const env = getenv("CFD_STAGE_GL");
+    const int tuned = m->tune_gl_stage[sizeof(T) == 4 ? 1 : 0];
+    const bool stage = env && (env[0] == '0' || env[0] == '1') ? env[0] == '1'
+                       : tuned >= 0                            ? tuned == 1
+                                                               : sizeof(T) == 4;
 #define CFD_GL_LAUNCH(KIND, STAGE)                                                                                 \
     hipLaunchKernelGGL((k_gg_limiter<T, KIND, STAGE>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr,     \
                        m->sptr, m->nbr, b.isx, b.isy, b.isz, b.vol, b.qa, b.qb, g, frac, k3, m->risk)

@@ -154,7 +154,7 @@ typedef struct cfd_solve_opts {
     double alpha_deg;       /* for lift/drag axes */
     cfd_linsolve_opts lin;
     int32_t jacobian;       /* CFD_SOLVE_JAC_* */
-    int32_t pad_;
+    int32_t autotune;       /* 1: cfd_autotune the mesh at the first Newton iteration (same results) */
 } cfd_solve_opts;
 
 /* Jacobian of the steady loop (always of the first-order flux, as the
@@ -296,6 +296,23 @@ int cfd_linsolve(cfd_mesh *mesh, const cfd_linsolve_opts *o, cfd_linsolve_stats 
  * bit for bit.  Every call on one GPU shares that GPU's array: calls that
  * overlap on one GPU race. */
 int cfd_linsolve_gs_noalloc(cfd_mesh *mesh, int sweeps, cfd_linsolve_stats *st);
+/* No counterpart in the reference: autotuning of the kernel variants that
+ * compute the same bits.  On this mesh and these parameters it times the
+ * fused gradient+limiter with direct and with LDS-staged neighbour gathers,
+ * and, when a Jacobian is assembled and lin is FGMRES, whole linear solves
+ * with the BSR and the block-SELL Krylov kernels (and, for the natural-order
+ * ILU(0), sync-free and level-launched sweeps); the mesh keeps the fastest of
+ * each for its later calls.  Results are unchanged bit for bit; the
+ * gradient, limiter and linear-solve outputs are overwritten.  A variable
+ * CFD_STAGE_GL, CFD_SELL or CFD_SYNC_FREE in the environment fixes that
+ * choice instead.  rep (may be NULL) receives the times in ms (0: not
+ * timed) and the choices (-1: not tuned). */
+typedef struct cfd_tune_report {
+    double gl_ms[2];          /* fused gradient+limiter: direct, staged */
+    double krylov_ms[2][2];   /* linear solve: [BSR, block-SELL][sync-free, level-launched] */
+    int32_t gl_stage, sell, sync_free, pad_;
+} cfd_tune_report;
+int cfd_autotune(cfd_mesh *mesh, const cfd_params *params, const cfd_linsolve_opts *lin, cfd_tune_report *rep);
 /* Q += relaxation * X (Solver_Steady_Implicit.cpp:150-156). */
 int cfd_update(cfd_mesh *mesh, double relaxation);
 /* Pressure force on wall boundary edges: F = sum p_node S_out. */
