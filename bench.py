@@ -550,6 +550,9 @@ def main():
                     help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")
     ap.add_argument("--batch-fp64-sample", type=int, default=16,
                     help="cases of the batch also solved one at a time in FP64 for the baseline rate (0: none)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="solve legs tune each mesh's same-bits kernel variants at its first Newton iteration "
+                         "(FP64 and mixed paths alike)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (scale factor for HBM counters)")
     args = ap.parse_args()
@@ -562,6 +565,8 @@ def main():
 
     from cfdsolutions_amd import cfd, mesh
 
+    if args.autotune:
+        SOLVE_KW["autotune"] = True
     if args.calibrate_copy:
         import torch
 
@@ -578,7 +583,7 @@ def main():
     if "residual" not in legs:
         ctx = cfd.Context(0)
         out = {"metric": "converged solves/s & p99 latency (mixed vs FP64 GPU); residual-eval GB/s",
-               "legs": sorted(legs)}
+               "legs": sorted(legs), "autotune": bool(args.autotune)}
         if "solve" in legs:
             out["solves"] = solve_leg(ctx, cpu=not args.no_cpu_solve)
         if "solve3" in legs:
@@ -709,7 +714,8 @@ def main():
         "config": {"workload": "residual microbenchmark: %d^3 jittered cube, %d tets, %d nodes, RCM; "
                                "GG gradient + Venkatakrishnan limiter + 2nd-order Roe residual (Conv/Diss), FP64, "
                                "fast build (within 1e-12 of the reference)" % (args.n, 6 * args.n ** 3, N),
-                   "nodes": N, "edges": E, "ghosts": B, "parallelism": "single GPU"},
+                   "nodes": N, "edges": E, "ghosts": B, "parallelism": "single GPU",
+                   "solve_autotune": bool(args.autotune)},
         "exact_build": {"value": round(exact["gbs"], 2), "ms_per_step": round(exact["ms_per_step"], 4),
                         "residual_kernel": "k_residual<double,Roe,2nd order,limited>",
                         "residual_gbs": round(exact["r_gbs"], 1), "residual_ms": round(exact["r_ms"], 4)}

@@ -504,6 +504,10 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
                 for (int c : rest) keep[c] = 1;
                 rc = harvest(u, take_l, opts, p, elapsed(), results, q_out, keep, q_keep);
                 if (rc) break;
+                // the compacted union keeps the first union's tuned variants
+                // (tuning again at every compaction cost more than it gained)
+                const bool tuned = u.m->tuned, use_sell = u.m->use_sell, sync_free = u.m->sync_free;
+                const int8_t gl0 = u.m->tune_gl_stage[0], gl1 = u.m->tune_gl_stage[1];
                 cfd_mesh_destroy(u.m);
                 u.m = nullptr;
                 for (int c : take)
@@ -514,6 +518,13 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
                     std::vector<double>().swap(q_keep[c]);
                 }
                 if (rc) break;
+                if (tuned) {
+                    u.m->tuned = true;
+                    u.m->use_sell = use_sell;
+                    u.m->sync_free = sync_free;
+                    u.m->tune_gl_stage[0] = gl0;
+                    u.m->tune_gl_stage[1] = gl1;
+                }
                 if (cfd_progress())
                     fprintf(stderr, "cfd_batch_solve: iteration %d, union compacted to %d cases (%d nodes), %.2f s\n",
                             it, (int)rest.size(), u.m->N, elapsed());
@@ -572,7 +583,7 @@ int cfd_batch_solve_into(cfd_ctx *ctx, int32_t K, const cfd_mesh_desc *meshes, c
         if (rc) break;
         solve.assign(Kn, 0);
         for (int lc = 0; lc < Kn; lc++) solve[lc] = active[u.cases[lc]];
-        if (o.autotune && !m->tuned) {  // each union mesh (a compaction builds a new one) tunes once
+        if (o.autotune && !m->tuned) {  // the first union mesh tunes once
             cfd_tune_report tr;
             rc = autotune_mesh(m, &pi, &o.lin, &tr,
                                [&] { return run_linsolve_batch(m, &o.lin, solve, iters, fres, lstatus); });
