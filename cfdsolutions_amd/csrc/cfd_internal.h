@@ -50,7 +50,7 @@ struct ResidualBufs {
     T *ws = nullptr;  // [W][4] area vector of each work edge of the flux-once residual
 };
 constexpr int kQA = 8;
-constexpr int kQB = 24;
+constexpr int kQB = 20;  // gradient (15) and limiter (5) of a node, unpadded
 constexpr int kPHI = 15;  // offset of phi in a qb record
 constexpr int kRES = 12;  // residual record: conv 0-4, diss 5-9, dt accumulator 10
 
