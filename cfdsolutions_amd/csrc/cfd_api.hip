@@ -935,7 +935,7 @@ int64_t cfd_patch_footprint_bytes(int64_t n, int64_t e, int64_t b, int64_t cases
     const int64_t inc = 2 * e + b, nsell = inc + inc / 4 + (int64_t)kSlice * 32;
     // per work edge (at most one per incidence): its two nodes and its area
     // vector in FP64 and FP32; a slot per incidence list entry; patch offsets
-    return inc * (4 + 4 + 32 + 16) + 4 * nsell + 8 * (n / kPatch + cases + 2);
+    return inc * (4 + 4 + 24 + 12) + 4 * nsell + 8 * (n / kPatch + cases + 2);
 }
 
 int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
@@ -998,7 +998,7 @@ int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
         for (int p = 0; p < P; p++) wptr[p + 1] = wptr[p] + (int)pw[p].size();
         const int64_t W = wptr[P];
         std::vector<int> wl(W), wr(W), slot(m->nsell, 0);
-        std::vector<double> ws(4 * (size_t)W);
+        std::vector<double> ws(3 * (size_t)W);
         bool ok = true;
         std::vector<char> bad(P, 0);
         cfd_parallel_for(
@@ -1012,10 +1012,9 @@ int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
                         const size_t o = (size_t)wptr[p] + q;
                         wl[o] = u.l;
                         wr[o] = u.r;
-                        ws[4 * o] = sx[u.inc];
-                        ws[4 * o + 1] = sy[u.inc];
-                        ws[4 * o + 2] = sz[u.inc];
-                        ws[4 * o + 3] = 0.0;
+                        ws[3 * o] = sx[u.inc];
+                        ws[3 * o + 1] = sy[u.inc];
+                        ws[3 * o + 2] = sz[u.inc];
                         if (u.l >= n0 && u.l < n1) lists[u.l - n0].push_back(q);
                         if (u.r >= n0 && u.r < n1) lists[u.r - n0].push_back(q | (int)0x80000000u);
                     }
@@ -1047,7 +1046,7 @@ int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
         m->npw = W;
     }
     if (fp32 && !m->f.ws) {
-        std::vector<double> ws(4 * (size_t)m->npw);
+        std::vector<double> ws(3 * (size_t)m->npw);
         CFD_HIP(m, hipStreamSynchronize(m->stream));
         CFD_HIP(m, cfd_memcpy(ws.data(), m->d.ws, sizeof(double) * ws.size(), hipMemcpyDeviceToHost));
         std::vector<float> wf(ws.begin(), ws.end());

@@ -47,7 +47,7 @@ struct ResidualBufs {
     T *vol = nullptr;
     T *lsq = nullptr;
     T *res = nullptr;
-    T *ws = nullptr;  // [W][4] area vector of each work edge of the flux-once residual
+    T *ws = nullptr;  // [W][3] area vector of each work edge of the flux-once residual
 };
 constexpr int kQA = 8;
 constexpr int kQB = 20;  // gradient (15) and limiter (5) of a node, unpadded

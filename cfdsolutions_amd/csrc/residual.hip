@@ -1121,7 +1121,7 @@ struct PatchArgs {
     const int *slot;
     const int *wptr;    // [npatch + 1] first work edge of each patch
     const int *wl, *wr; // L and R end of each work edge (R = N + b: ghost of boundary edge b)
-    const T *ws;        // [W][4] area vector, oriented L -> R (outward for boundary edges)
+    const T *ws;        // [W][3] area vector, oriented L -> R (outward for boundary edges)
     const T *qa, *qb;
     T *res;
     T g, frac;
@@ -1137,7 +1137,7 @@ struct PatchArgs {
 template <class T, int FLUX, int ORDER, bool LIM, bool STATS>
 __device__ __forceinline__ T edge_flux(const PatchArgs<T> &a, int w, T *cf, T *df, T *emin, T *emax, int &bad) {
     const int l = a.wl[w], r = a.wr[w];
-    const T *sv = a.ws + 4 * (size_t)w;
+    const T *sv = a.ws + 3 * (size_t)w;
     const T s0 = sv[0], s1 = sv[1], s2 = sv[2];
     const T *ql = a.qa + kQA * (size_t)l, *qr = a.qa + kQA * (size_t)r;
     T QL[5], QR[5];
