@@ -256,6 +256,7 @@ def bump_dims(n_nodes):
 
 
 SOLVE_KW = dict(max_iter=400, tol=1e-10, krylov_m=30, max_restarts=4, rtol=1e-3)
+MIXED_LEVEL = True  # mixed= of the serving and batch legs' mixed solves (2 with --fp16-factors)
 
 
 def solve_leg(ctx, cpu=True):
@@ -307,22 +308,24 @@ def solve3_leg(ctx):
     """Bump channel 192x64x64 (815,425 nodes), second-order HLLC with Green-Gauss gradients and the
     Venkatakrishnan limiter, M 0.7, AoA 2 deg, FGMRES(40) with the multicolour ILU(0), to RMS 1e-10: plain FP64 vs
     mixed precision (FP32 Jacobian, ILU(0) and Krylov basis), each alone on the GPU, with the approximate Roe
-    Jacobian; and the mixed solve again with the finite-difference Jacobian of the HLLC flux."""
+    Jacobian; the mixed solve with the ILU(0) factors of the sweeps stored in FP16 (mixed = 2); and the mixed
+    solve with the finite-difference Jacobian of the HLLC flux."""
     from cfdsolutions_amd import cfd, mesh
 
     m = mesh.bump_channel(192, 64, 64, bump_h=0.1)
     p = cfd.make_params(mach=0.7, alpha_deg=2.0, flux="hllc", order=2, limiter=cfd.LIM_VENKAT, gradient=cfd.GRAD_GG)
     kw = dict(SOLVE_KW, krylov_m=40, alpha_deg=2.0, precond=cfd.PREC_ILU0_MC)
     out = {}
-    for name, mixed, jac in (("fp64", False, cfd.SOLVE_JAC_APPROX), ("mixed", True, cfd.SOLVE_JAC_APPROX),
-                             ("mixed_fd", True, cfd.SOLVE_JAC_FD)):
+    runs = (("fp64", False, cfd.SOLVE_JAC_APPROX), ("mixed", True, cfd.SOLVE_JAC_APPROX),
+            ("mixed_fp16", cfd.MIXED_FP16, cfd.SOLVE_JAC_APPROX), ("mixed_fd", True, cfd.SOLVE_JAC_FD))
+    for name, mixed, jac in runs:
         dm = cfd.DeviceMesh(ctx, m, p)
         r, _ = dm.solve_steady(p, mixed=mixed, check=False, jacobian=jac, **kw)
         out[name] = (r, dm.get(cfd.FIELD_Q))
         print("bench: solve 192x64x64 %-6s %.2f s (%d newton / %d krylov), converged %d, rms %.2e" % (
             name, r.seconds, r.iterations, r.krylov_total, r.converged, r.final_rms), file=sys.stderr, flush=True)
         dm.close()
-    (r0, q0), (r1, q1), (r2, q2) = out["fp64"], out["mixed"], out["mixed_fd"]
+    (r0, q0), (r1, q1), (r2, q2), (r3, q3) = out["fp64"], out["mixed"], out["mixed_fd"], out["mixed_fp16"]
     return {"workload": "bump channel 192x64x64 (%d nodes), 2nd-order HLLC + GG + Venkatakrishnan, M 0.7, AoA 2, "
                         "FGMRES(40) + multicolour ILU(0), RMS < 1e-10" % m.n_node,
             "fp64": {"seconds": r0.seconds, "newton": r0.iterations, "krylov": r0.krylov_total,
@@ -338,6 +341,13 @@ def solve3_leg(ctx):
                                                                               np.array(r0.force))) /
                                                                  np.max(np.abs(r0.force))),
                                   "state_rel_err_vs_fp64": float(np.max(np.abs(q2 - q0)) / np.max(np.abs(q0)))},
+            "mixed_fp16_factors": {"seconds": r3.seconds, "newton": r3.iterations, "krylov": r3.krylov_total,
+                                   "converged": bool(r3.converged), "lift": r3.lift, "drag": r3.drag,
+                                   "force_rel_err_vs_fp64": float(np.max(np.abs(np.array(r3.force) -
+                                                                               np.array(r0.force))) /
+                                                                  np.max(np.abs(r0.force))),
+                                   "state_rel_err_vs_fp64": float(np.max(np.abs(q3 - q0)) / np.max(np.abs(q0))),
+                                   "speedup_vs_fp64": r0.seconds / r3.seconds},
             "speedup_mixed": r0.seconds / r1.seconds}
 
 
@@ -387,8 +397,8 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, batch_max=8, seed=2026
             now = time.perf_counter() - t0
             while k < n_req and arrivals[k] <= now:
                 r = reqs[k]
-                srv.submit(k, r[0], pool[r[0]], params(r), alpha_deg=r[2], precond=cfd.PREC_ILU0_MC, mixed=True,
-                           **SOLVE_KW)
+                srv.submit(k, r[0], pool[r[0]], params(r), alpha_deg=r[2], precond=cfd.PREC_ILU0_MC,
+                           mixed=MIXED_LEVEL, **SOLVE_KW)
                 k += 1
             wait = 0.0 if k >= n_req else max(0.0, arrivals[k] - (time.perf_counter() - t0))
             got = srv.poll(wait=min(wait, 0.05) if k < n_req else 1.0)
@@ -403,7 +413,7 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, batch_max=8, seed=2026
     # one request per solve, meshes resident before timing (one solve per mesh)
     srv = cfd.Server(ctx, hbm_budget=200 << 30, max_inflight=inflight)
     for k, m in enumerate(pool):
-        srv.submit(-1 - k, k, m, params(reqs[0]), alpha_deg=reqs[0][2], precond=cfd.PREC_ILU0_MC, mixed=True,
+        srv.submit(-1 - k, k, m, params(reqs[0]), alpha_deg=reqs[0][2], precond=cfd.PREC_ILU0_MC, mixed=MIXED_LEVEL,
                    **SOLVE_KW)
     got = 0
     while got < n_meshes:
@@ -454,6 +464,7 @@ def serve_leg(ctx, n_req, n_meshes, nmin, nmax, inflight, batch_max=8, seed=2026
            "serving_without_batching_solves_per_s": cap1,
            "mean_batch_size": float(np.mean([c["batch_size"] for c in burst + stream])),
            "serving_vs_fp64": cap / base,
+           "mixed_level": int(MIXED_LEVEL),
            "stream_rate_per_s": 0.8 * cap,
            "stream_solves_per_s": n_req / span_s,
            "latency_p50_s": float(np.percentile(lat_s, 50)), "latency_p99_s": float(np.percentile(lat_s, 99)),
@@ -495,7 +506,7 @@ def batch_leg(ctx, n_case, n_sample, nmin=8000, nmax=40000, seed=4):
               file=sys.stderr, flush=True)
     print("bench: batch of %d cases (%d nodes in all)" % (n_case, sum(m.n_node for m in meshes)), file=sys.stderr,
           flush=True)
-    rc, rb = cfd.batch_solve(ctx, meshes, params, alphas=list(alphas), precond=cfd.PREC_ILU0_MC, mixed=True,
+    rc, rb = cfd.batch_solve(ctx, meshes, params, alphas=list(alphas), precond=cfd.PREC_ILU0_MC, mixed=MIXED_LEVEL,
                              **SOLVE_KW)
     # each case reports the time until its own result was final (finished cases are harvested when the union
     # compacts), so the batch took as long as its slowest case
@@ -508,7 +519,7 @@ def batch_leg(ctx, n_case, n_sample, nmin=8000, nmax=40000, seed=4):
                        "AoA -2..4, RMS < 1e-10" % (n_case, min(nodes), max(nodes), sum(nodes)),
            "fp64_sequential_solves_per_s": rate64,
            "fp64_sample_cases": n_sample,
-           "batch_mixed_solves_per_s": n_case / tb,
+           "batch_mixed_solves_per_s": n_case / tb, "mixed_level": int(MIXED_LEVEL),
            "batch_vs_fp64": (n_case / tb) / rate64,
            "batch_seconds": tb, "status": int(rc),
            "converged": int(sum(1 for b in rb if b.converged)),
@@ -550,6 +561,9 @@ def main():
                     help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")
     ap.add_argument("--batch-fp64-sample", type=int, default=16,
                     help="cases of the batch also solved one at a time in FP64 for the baseline rate (0: none)")
+    ap.add_argument("--fp16-factors", action="store_true",
+                    help="the serving and batch legs' mixed solves store the multicolour ILU(0) factors of the "
+                         "block-SELL sweeps in FP16 (mixed = 2)")
     ap.add_argument("--autotune", action="store_true",
                     help="solve legs tune each mesh's same-bits kernel variants at its first Newton iteration "
                          "(FP64 and mixed paths alike)")
@@ -567,6 +581,8 @@ def main():
 
     if args.autotune:
         SOLVE_KW["autotune"] = True
+    global MIXED_LEVEL
+    MIXED_LEVEL = cfd.MIXED_FP16 if args.fp16_factors else True
     if args.calibrate_copy:
         import torch
 

@@ -20,6 +20,8 @@ FP64, FP32 = 0, 1
 ARITH_EXACT, ARITH_FAST = 0, 1
 SOLVE_JAC_APPROX, SOLVE_JAC_FD = 0, 1
 PREC_NONE, PREC_BJACOBI, PREC_ILU0, PREC_ILU0_MC = 0, 1, 2, 3
+# mixed= of the linear solves: False/0 FP64, True/1 FP32 storage, MIXED_FP16 FP32 with FP16-stored multicolour factors
+MIXED_FP16 = 2
 SOLVER_FGMRES, SOLVER_BLOCK_GS, SOLVER_LUSGS = 0, 1, 2
 JAC_FD_CENTRAL, JAC_FD_FORWARD, JAC_FD_BACKWARD, JAC_FD_ALTERNATE = 0, 1, 2, 3
 FIELD_Q, FIELD_GRAD, FIELD_PHI, FIELD_RES_CONV, FIELD_RES_DISS, FIELD_DT_ACC, FIELD_DT, FIELD_JAC_B, FIELD_X = range(9)
@@ -109,7 +111,7 @@ class ServerStats(ctypes.Structure):
 def solve_opts(max_iter=200, tol=1e-10, first_order_iters=0, relaxation=1.0, wall_tag=3, alpha_deg=0.0,
                precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, mixed=False, solver=SOLVER_FGMRES,
                sweeps=0, jacobian=0, autotune=False):
-    lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
+    lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(mixed))
     return SolveOpts(max_iter, tol, first_order_iters, relaxation, wall_tag, alpha_deg, lin, int(jacobian),
                      int(bool(autotune)))
 
@@ -321,12 +323,12 @@ class DeviceMesh:
         return r.value
 
     def jacobian(self, p, rhs_mode=1, mixed=False):
-        self.ctx.check(lib().cfd_jacobian(self.h, ctypes.byref(p), int(rhs_mode), int(bool(mixed))))
+        self.ctx.check(lib().cfd_jacobian(self.h, ctypes.byref(p), int(rhs_mode), int(mixed)))
 
     def jacobian_fd(self, p, method=0, it=0, add_time=True, rhs_mode=0, mixed=False):
         """Finite-difference Roe Jacobian (cfd_jacobian_fd); method JAC_FD_CENTRAL/FORWARD/BACKWARD/ALTERNATE."""
         self.ctx.check(lib().cfd_jacobian_fd(self.h, ctypes.byref(p), int(method), int(it), int(bool(add_time)),
-                                             int(rhs_mode), int(bool(mixed))))
+                                             int(rhs_mode), int(mixed)))
 
     def bsr_structure(self):
         nnz = lib().cfd_bsr_structure(self.h, None, None, None)
@@ -350,7 +352,7 @@ class DeviceMesh:
 
     def linsolve(self, solver=SOLVER_FGMRES, precond=PREC_ILU0, krylov_m=30, max_restarts=4, rtol=1e-3, sweeps=0,
                  mixed=False):
-        o = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
+        o = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(mixed))
         st = LinsolveStats()
         self.ctx.check(lib().cfd_linsolve(self.h, ctypes.byref(o), ctypes.byref(st)))
         return st
@@ -359,7 +361,7 @@ class DeviceMesh:
                  solver=SOLVER_FGMRES, sweeps=0):
         """cfd_autotune: times the same-bits kernel variants on this mesh (the Krylov ones on the assembled
         Jacobian, if any) and keeps the fastest.  Returns the TuneReport."""
-        lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(bool(mixed)))
+        lin = LinsolveOpts(solver, precond, krylov_m, max_restarts, rtol, sweeps, int(mixed))
         rep = TuneReport()
         self.ctx.check(lib().cfd_autotune(self.h, ctypes.byref(p), ctypes.byref(lin), ctypes.byref(rep)))
         return rep
@@ -477,7 +479,7 @@ def batch_solve(ctx, meshes, params, alphas=None, **solve_kw):
 
 
 def footprint_bytes(n_node, n_edge, n_bedge, krylov_m=30, mixed=True):
-    return lib().cfd_footprint_bytes(int(n_node), int(n_edge), int(n_bedge), int(krylov_m), int(bool(mixed)))
+    return lib().cfd_footprint_bytes(int(n_node), int(n_edge), int(n_bedge), int(krylov_m), int(mixed))
 
 
 def gpu_available():

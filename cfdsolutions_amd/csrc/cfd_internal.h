@@ -232,6 +232,11 @@ struct cfd_mesh {
     BlockSell a_sell, mcl_sell, mcu_sell;
     double *asv = nullptr, *lsv = nullptr, *usv = nullptr, *dsv = nullptr;
     float *asvf = nullptr, *lsvf = nullptr, *usvf = nullptr, *dsvf = nullptr;
+    // FP16 copies of the multicolour factors for the sweeps of linear solves
+    // with mixed = 2: each block's entries over its largest magnitude, with
+    // that scale kept per block in FP32
+    _Float16 *lsvh = nullptr, *usvh = nullptr, *dsvh = nullptr;
+    float *lsvs = nullptr, *usvs = nullptr, *dsvs = nullptr;
     bool asell_valid = false;    // A's SELL copy matches the current Jacobian
     bool use_sell = false;       // Krylov loop on the block-SELL copies (else the BSR arrays)
     std::vector<int> mc_cstart;  // [ncolor + 1] first permuted row of each colour
@@ -239,6 +244,7 @@ struct cfd_mesh {
     bool prec_valid = false;
     int prec_kind = -1;
     bool prec_mixed = false;
+    bool prec_half = false;  // the sweeps read the FP16 factor copies
 
     // autotuned variants (cfd_autotune): gather mode of the fused
     // gradient+limiter per precision (FP64, FP32; -1: the fixed rule), and

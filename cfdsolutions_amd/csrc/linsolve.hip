@@ -13,6 +13,8 @@
 #include <vector>
 
 #include "cfd_internal.h"
+
+#include <type_traits>
 #include "flux.h"
 
 using namespace cfd;
@@ -918,6 +920,79 @@ __global__ __launch_bounds__(kBlock) void k_trsv_mc_sell(int r0, int r1, const i
     }
 }
 
+// FP16 block-SELL copy of FP32 blocks: each block's 25 entries over its
+// largest magnitude (a zero block keeps scale 1), the scale kept per slot.
+__global__ __launch_bounds__(kBlock) void k_to_sell_half(size_t nblk, const int *__restrict__ src,
+                                                         const float *__restrict__ v, _Float16 *__restrict__ out,
+                                                         float *__restrict__ scale) {
+    for (size_t p = blockIdx.x * (size_t)kBlock + threadIdx.x; p < nblk; p += (size_t)gridDim.x * kBlock) {
+        const int q = src ? src[p] : (int)p;
+        if (q < 0) continue;
+        const size_t l = p & 63;
+        _Float16 *o = out + 25 * (p - l) + l;
+        const float *in = v + 25 * (size_t)q;
+        float mx = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 25; e++) mx = fmaxf(mx, fabsf(in[e]));
+        const float sc = mx > 0.0f ? mx : 1.0f, inv = 1.0f / sc;
+#pragma unroll
+        for (int e = 0; e < 25; e++) o[64 * e] = (_Float16)(in[e] * inv);
+        scale[p] = sc;
+    }
+}
+
+// k_trsv_mc_sell with the factors stored in FP16 (per-block FP32 scales):
+// each block's product is formed in FP32 and scaled once.
+__global__ __launch_bounds__(kBlock) void k_trsv_mc_sell_half(int r0, int r1, const int *__restrict__ ia,
+                                                              const int *__restrict__ iau, const int *__restrict__ perm,
+                                                              const int *__restrict__ off, const int *__restrict__ col,
+                                                              const _Float16 *__restrict__ val,
+                                                              const float *__restrict__ vsc,
+                                                              const _Float16 *__restrict__ dv,
+                                                              const float *__restrict__ dsc,
+                                                              const float *__restrict__ rhs, float *w, float *z,
+                                                              int backward) {
+    const int i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+    if (i < r0 || i >= r1) return;
+    const int l = i & 63;
+    const int len = backward ? ia[i + 1] - iau[i] - 1 : iau[i] - ia[i];
+    float y[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) y[r] = backward ? w[5 * (size_t)i + r] : rhs[5 * (size_t)perm[i] + r];
+    size_t p = (size_t)off[i >> 6] + l;
+    for (int k = 0; k < len; k++, p += 64) {
+        const float *wj = w + 5 * (size_t)col[p];
+        float wv[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) wv[c] = wj[c];
+        const _Float16 *A = val + 25 * (p - l) + l;
+        const float sc = vsc[p];
+#pragma unroll
+        for (int r = 0; r < 5; r++) {
+            float s = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 5; c++) s += (float)A[64 * (5 * r + c)] * wv[c];
+            y[r] = y[r] - sc * s;
+        }
+    }
+    if (!backward) {
+#pragma unroll
+        for (int r = 0; r < 5; r++) w[5 * (size_t)i + r] = y[r];
+        return;
+    }
+    const _Float16 *D = dv + 25 * (size_t)(i - l) + l;
+    const float sd = dsc[i];
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 5; c++) s += (float)D[64 * (5 * r + c)] * y[c];
+        s *= sd;
+        w[5 * (size_t)i + r] = s;
+        z[5 * (size_t)perm[i] + r] = s;
+    }
+}
+
 // z = D^-1 r (block-Jacobi apply)
 template <class TM, class TV>
 __global__ __launch_bounds__(kBlock) void k_bjac_apply(int N, const TM *dinv, const TV *rhs, TV *z) {
@@ -1470,7 +1545,7 @@ template <>
 const float *bsr_a<float>(cfd_mesh *m) { return m->af; }
 
 template <class TM>
-int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
+int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv, bool half) {
     if (kind == CFD_PREC_ILU0_MC) {
         hipLaunchKernelGGL((k_gather_blocks<TM, TM>), dim3(grid_cap(25 * (size_t)m->nnz, 8192)), dim3(kBlock), 0,
                            m->stream, (size_t)m->nnz, m->mc_src, a, lu);
@@ -1480,8 +1555,18 @@ int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv) {
                                m->mc_lower.rows + b0, m->mc_ia, m->mc_ja, m->mc_iau, m->mc_moff, m->mc_mqq, lu, dinv,
                                lu, dinv, m->flag);
         }
-        // block-SELL copies for the sweeps
-        if (m->use_sell) {
+        // block-SELL copies for the sweeps, FP16 with mixed = 2
+        if constexpr (std::is_same<TM, float>::value) {
+            if (half) {
+                hipLaunchKernelGGL(k_to_sell_half, dim3(grid_cap((size_t)m->mcl_sell.nblk, 8192)), dim3(kBlock), 0,
+                                   m->stream, (size_t)m->mcl_sell.nblk, m->mcl_sell.src, lu, m->lsvh, m->lsvs);
+                hipLaunchKernelGGL(k_to_sell_half, dim3(grid_cap((size_t)m->mcu_sell.nblk, 8192)), dim3(kBlock), 0,
+                                   m->stream, (size_t)m->mcu_sell.nblk, m->mcu_sell.src, lu, m->usvh, m->usvs);
+                hipLaunchKernelGGL(k_to_sell_half, dim3(grid_cap((size_t)m->N, 8192)), dim3(kBlock), 0, m->stream,
+                                   (size_t)m->N, nullptr, dinv, m->dsvh, m->dsvs);
+            }
+        }
+        if (m->use_sell && !half) {
             hipLaunchKernelGGL(k_to_sell<TM>, dim3(grid_cap((size_t)m->mcl_sell.nblk, 8192)), dim3(kBlock), 0, m->stream,
                                (size_t)m->mcl_sell.nblk, m->mcl_sell.src, lu, SellVals<TM>::l(m));
             hipLaunchKernelGGL(k_to_sell<TM>, dim3(grid_cap((size_t)m->mcu_sell.nblk, 8192)), dim3(kBlock), 0, m->stream,
@@ -1535,6 +1620,23 @@ void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r
                 const int nwaves = (nr + 11) / 12;
                 hipLaunchKernelGGL((k_trsv_mc<TM, TV>), dim3((nwaves + 3) / 4), dim3(kBlock), 0, m->stream, nr,
                                    ls.rows + b0, m->mc_ia, m->mc_ja, m->mc_iau, m->mc_perm, lu, dinv, r, wk, z, pass);
+            }
+        }
+    } else if (kind == CFD_PREC_ILU0_MC && m->prec_half) {
+        if constexpr (std::is_same<TM, float>::value && std::is_same<TV, float>::value) {
+            float *wk = mc_work<float>(m);
+            for (int pass = 0; pass < 2; pass++) {
+                const BlockSell &bs = pass ? m->mcu_sell : m->mcl_sell;
+                const _Float16 *val = pass ? m->usvh : m->lsvh;
+                const float *vsc = pass ? m->usvs : m->lsvs;
+                for (int t = 0; t < m->ncolor; t++) {
+                    const int c = pass ? m->ncolor - 1 - t : t;
+                    const int r0 = m->mc_cstart[c], r1 = m->mc_cstart[c + 1];
+                    const int g = (r1 - (r0 & ~63) + kBlock - 1) / kBlock;
+                    hipLaunchKernelGGL(k_trsv_mc_sell_half, dim3(g), dim3(kBlock), 0, m->stream, r0, r1, m->mc_ia,
+                                       m->mc_iau, m->mc_perm, bs.off, bs.col, val, vsc, m->dsvh, m->dsvs, r, wk, z,
+                                       pass);
+                }
             }
         }
     } else if (kind == CFD_PREC_ILU0_MC) {
@@ -2158,12 +2260,25 @@ namespace {
 // Allocates the block-SELL value arrays of storage type TM: A always, the
 // multicolour factors for that preconditioner.
 template <class TM>
-int ensure_sell(cfd_mesh *m, int precond) {
+int ensure_sell(cfd_mesh *m, int precond, bool half) {
     if (!m->use_sell) return CFD_OK;
     TM *&av = SellVals<TM>::a(m);
     if (!av) av = (TM *)cfd_dalloc(m, sizeof(TM) * 25 * (size_t)m->a_sell.nblk);
     if (!av) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (block-SELL Jacobian)");
     if (precond != CFD_PREC_ILU0_MC) return CFD_OK;
+    if (half) {
+        const size_t nl = std::max<int64_t>(m->mcl_sell.nblk, 64), nu = std::max<int64_t>(m->mcu_sell.nblk, 64),
+                     nd = ((size_t)m->N + 63) / 64 * 64;
+        if (!m->lsvh) m->lsvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * nl);
+        if (!m->usvh) m->usvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * nu);
+        if (!m->dsvh) m->dsvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * nd);
+        if (!m->lsvs) m->lsvs = (float *)cfd_dalloc(m, sizeof(float) * nl);
+        if (!m->usvs) m->usvs = (float *)cfd_dalloc(m, sizeof(float) * nu);
+        if (!m->dsvs) m->dsvs = (float *)cfd_dalloc(m, sizeof(float) * nd);
+        if (!m->lsvh || !m->usvh || !m->dsvh || !m->lsvs || !m->usvs || !m->dsvs)
+            return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (FP16 block-SELL factors)");
+        return CFD_OK;
+    }
     TM *&lv = SellVals<TM>::l(m), *&uv = SellVals<TM>::u(m), *&dv = SellVals<TM>::d(m);
     if (!lv) lv = (TM *)cfd_dalloc(m, sizeof(TM) * 25 * (size_t)std::max<int64_t>(m->mcl_sell.nblk, 64));
     if (!uv) uv = (TM *)cfd_dalloc(m, sizeof(TM) * 25 * (size_t)std::max<int64_t>(m->mcu_sell.nblk, 64));
@@ -2177,12 +2292,15 @@ int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
         return cfd_set_error(m->ctx, CFD_E_ARG, "krylov_m must be in [1, %d]", kMaxKrylov);
     if (o->precond < CFD_PREC_NONE || o->precond > CFD_PREC_ILU0_MC)
         return cfd_set_error(m->ctx, CFD_E_ARG, "unknown preconditioner %d", o->precond);
+    if (o->mixed < 0 || o->mixed > 2) return cfd_set_error(m->ctx, CFD_E_ARG, "mixed must be 0, 1 or 2");
     const bool mixed = o->mixed != 0;
+    // FP16 factor copies: mixed = 2, multicolour ILU(0) on the block-SELL kernels
+    const bool half = o->mixed == 2 && o->precond == CFD_PREC_ILU0_MC && m->use_sell;
     if (mixed != m->jac_mixed)
         return cfd_set_error(m->ctx, CFD_E_ARG, "Jacobian precision does not match linsolve precision");
     int rc = ensure_krylov(m, o->krylov_m, mixed);
     if (rc) return rc;
-    rc = mixed ? ensure_sell<float>(m, o->precond) : ensure_sell<double>(m, o->precond);
+    rc = mixed ? ensure_sell<float>(m, o->precond, half) : ensure_sell<double>(m, o->precond, false);
     if (rc) return rc;
     if (m->use_sell && !m->asell_valid) {
         if (mixed)
@@ -2194,19 +2312,21 @@ int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
         CFD_HIP(m, hipGetLastError());
         m->asell_valid = true;
     }
-    if (!m->prec_valid || m->prec_kind != o->precond || m->prec_mixed != mixed) {
+    if (!m->prec_valid || m->prec_kind != o->precond || m->prec_mixed != mixed || m->prec_half != half) {
+        if (m->prec_half != half) drop_krylov_graphs(m);  // captured steps launch the other sweep kernels
         m->prec_kind = o->precond;
         m->prec_mixed = mixed;
+        m->prec_half = half;
         if (mixed) {
             if ((o->precond == CFD_PREC_ILU0 || o->precond == CFD_PREC_ILU0_MC) && !m->luf)
                 m->luf = (float *)cfd_dalloc(m, sizeof(float) * 25 * (size_t)m->nnz);
             if (o->precond != CFD_PREC_NONE && !m->dinvf) m->dinvf = (float *)cfd_dalloc(m, sizeof(float) * 25 * (size_t)m->N);
-            rc = build_prec<float>(m, o->precond, m->af, m->luf, m->dinvf);
+            rc = build_prec<float>(m, o->precond, m->af, m->luf, m->dinvf, half);
         } else {
             if ((o->precond == CFD_PREC_ILU0 || o->precond == CFD_PREC_ILU0_MC) && !m->lu)
                 m->lu = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->nnz);
             if (o->precond != CFD_PREC_NONE && !m->dinv) m->dinv = (double *)cfd_dalloc(m, sizeof(double) * 25 * (size_t)m->N);
-            rc = build_prec<double>(m, o->precond, m->a, m->lu, m->dinv);
+            rc = build_prec<double>(m, o->precond, m->a, m->lu, m->dinv, false);
         }
         if (rc) return rc;
         m->prec_valid = true;

@@ -3,6 +3,8 @@ algorithm (backward Euler, approximate Roe Jacobian, FGMRES + ILU(0)):
 Newton iteration counts within +-1, Krylov totals within 10%, converged
 forces within 1e-8 relative; mixed precision (FP32 Jacobian, ILU(0) and
 Krylov basis) converges to the same state within the stated tolerance."""
+import os
+
 import numpy as np
 import pytest
 
@@ -134,3 +136,34 @@ def test_finite_difference_jacobian_solve(ctx):
     fa, ff = np.array(a.force), np.array(f.force)
     assert np.max(np.abs(ff - fa)) <= 1e-8 * np.max(np.abs(fa))
     assert abs(f.lift - a.lift) <= 1e-8 * np.max(np.abs(fa)) and abs(f.drag - a.drag) <= 1e-8 * np.max(np.abs(fa))
+
+
+def test_mixed_fp16_factor_solve(ctx):
+    """mixed = 2: the multicolour ILU(0) factors of the block-SELL sweeps in FP16 with per-block FP32 scales.  The
+    steady solve converges to the FP64 solve's state (forces, lift and drag within 1e-8) in the same number of
+    Newton iterations (+-1), and its iterates differ from the FP32-factor solve's, so the FP16 sweeps ran."""
+    print("mixed solve with FP16 factors", flush=True)
+    old = os.environ.get("CFD_SELL")
+    os.environ["CFD_SELL"] = "1"  # block-SELL kernels on this small mesh
+    try:
+        m = cmesh.bump_channel(24, 8, 8)
+        p = cfd.make_params(mach=0.6, alpha_deg=2.0, flux="roe", order=1)
+        res = {}
+        for mixed in (False, True, cfd.MIXED_FP16):
+            dm = cfd.DeviceMesh(ctx, m, p)
+            r, hist = dm.solve_steady(p, alpha_deg=2.0, max_iter=300, tol=1e-10, krylov_m=30, max_restarts=4,
+                                      rtol=1e-3, precond=cfd.PREC_ILU0_MC, mixed=mixed)
+            res[int(mixed)] = (r, hist.tolist())
+            dm.close()
+    finally:
+        if old is None:
+            del os.environ["CFD_SELL"]
+        else:
+            os.environ["CFD_SELL"] = old
+    (r64, _), (r32, h32), (r16, h16) = res[0], res[1], res[2]
+    assert r64.converged and r32.converged and r16.converged
+    assert abs(r16.iterations - r64.iterations) <= 1
+    scale = np.max(np.abs(r64.force))
+    assert np.max(np.abs(np.array(r16.force) - np.array(r64.force))) <= 1e-8 * scale
+    assert abs(r16.lift - r64.lift) <= 1e-8 * scale and abs(r16.drag - r64.drag) <= 1e-8 * scale
+    assert h16 != h32
