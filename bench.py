@@ -308,8 +308,8 @@ def solve3_leg(ctx):
     """Bump channel 192x64x64 (815,425 nodes), second-order HLLC with Green-Gauss gradients and the
     Venkatakrishnan limiter, M 0.7, AoA 2 deg, FGMRES(40) with the multicolour ILU(0), to RMS 1e-10: plain FP64 vs
     mixed precision (FP32 Jacobian, ILU(0) and Krylov basis), each alone on the GPU, with the approximate Roe
-    Jacobian; the mixed solve with the ILU(0) factors of the sweeps stored in FP16 (mixed = 2); and the mixed
-    solve with the finite-difference Jacobian of the HLLC flux."""
+    Jacobian; the mixed solve with the block-SELL copies of the Jacobian and the ILU(0) factors stored in FP16
+    (mixed = 2); and the mixed solve with the finite-difference Jacobian of the HLLC flux."""
     from cfdsolutions_amd import cfd, mesh
 
     m = mesh.bump_channel(192, 64, 64, bump_h=0.1)
@@ -562,8 +562,8 @@ def main():
     ap.add_argument("--batch-fp64-sample", type=int, default=16,
                     help="cases of the batch also solved one at a time in FP64 for the baseline rate (0: none)")
     ap.add_argument("--fp16-factors", action="store_true",
-                    help="the serving and batch legs' mixed solves store the multicolour ILU(0) factors of the "
-                         "block-SELL sweeps in FP16 (mixed = 2)")
+                    help="the serving and batch legs' mixed solves store the block-SELL copies of the Jacobian "
+                         "and of the multicolour ILU(0) factors in FP16 (mixed = 2)")
     ap.add_argument("--autotune", action="store_true",
                     help="solve legs tune each mesh's same-bits kernel variants at its first Newton iteration "
                          "(FP64 and mixed paths alike)")

@@ -1095,6 +1095,45 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, const int *__restri
     }
 }
 
+// k_spmv_sell with A stored in FP16 (per-block FP32 scales): each block's
+// product is formed in FP32 and scaled once.
+template <class TX, class TY>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell_half(int N, const int *__restrict__ ia,
+                                                           const int *__restrict__ off, const int *__restrict__ col,
+                                                           const _Float16 *__restrict__ val,
+                                                           const float *__restrict__ vsc, const TX *__restrict__ x,
+                                                           TY *__restrict__ y, const double *__restrict__ bsub) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int l = i & 63;
+    const int len = ia[i + 1] - ia[i];
+    size_t p = (size_t)off[i >> 6] + l;
+    TY s[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) s[r] = TY(0.0);
+    for (int k = 0; k < len; k++, p += 64) {
+        const TX *xj = x + 5 * (size_t)col[p];
+        float xv[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) xv[c] = (float)xj[c];
+        const _Float16 *A = val + 25 * (p - l) + l;
+        const float sc = vsc[p];
+#pragma unroll
+        for (int r = 0; r < 5; r++) {
+            float t = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 5; c++) t += (float)A[64 * (5 * r + c)] * xv[c];
+            s[r] += (TY)(sc * t);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        TY v = s[r];
+        if (bsub) v = (TY)bsub[5 * (size_t)i + r] - v;
+        y[5 * (size_t)i + r] = v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Reference sweeps (MATH/src/MC_Iterative_Jacobi.c), one level per launch
 // ---------------------------------------------------------------------------
@@ -1593,7 +1632,10 @@ int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv, bool half) 
 // y = A x, or y = bsub - A x, with the block-SELL copy of the Jacobian.
 template <class TM, class TX, class TY>
 void spmv(cfd_mesh *m, const TX *x, TY *y, const double *bsub) {
-    if (m->use_sell)
+    if (std::is_same<TM, float>::value && m->use_sell && m->asell_half)
+        hipLaunchKernelGGL((k_spmv_sell_half<TX, TY>), dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           m->stream, m->N, m->ia, m->a_sell.off, m->a_sell.col, m->asvh, m->asvs, x, y, bsub);
+    else if (m->use_sell)
         hipLaunchKernelGGL((k_spmv_sell<TM, TX, TY>), dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream,
                            m->N, m->ia, m->a_sell.off, m->a_sell.col, (const TM *)SellVals<TM>::a(m), x, y, bsub);
     else
@@ -2269,6 +2311,9 @@ int ensure_sell(cfd_mesh *m, int precond, bool half) {
     if (half) {
         const size_t nl = std::max<int64_t>(m->mcl_sell.nblk, 64), nu = std::max<int64_t>(m->mcu_sell.nblk, 64),
                      nd = ((size_t)m->N + 63) / 64 * 64;
+        if (!m->asvh) m->asvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * (size_t)m->a_sell.nblk);
+        if (!m->asvs) m->asvs = (float *)cfd_dalloc(m, sizeof(float) * (size_t)m->a_sell.nblk);
+        if (!m->asvh || !m->asvs) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (FP16 A)");
         if (!m->lsvh) m->lsvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * nl);
         if (!m->usvh) m->usvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * nu);
         if (!m->dsvh) m->dsvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * nd);
@@ -2294,7 +2339,8 @@ int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
         return cfd_set_error(m->ctx, CFD_E_ARG, "unknown preconditioner %d", o->precond);
     if (o->mixed < 0 || o->mixed > 2) return cfd_set_error(m->ctx, CFD_E_ARG, "mixed must be 0, 1 or 2");
     const bool mixed = o->mixed != 0;
-    // FP16 factor copies: mixed = 2, multicolour ILU(0) on the block-SELL kernels
+    // FP16 copies of A and of the factors: mixed = 2, multicolour ILU(0) on
+    // the block-SELL kernels
     const bool half = o->mixed == 2 && o->precond == CFD_PREC_ILU0_MC && m->use_sell;
     if (mixed != m->jac_mixed)
         return cfd_set_error(m->ctx, CFD_E_ARG, "Jacobian precision does not match linsolve precision");
@@ -2302,8 +2348,13 @@ int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
     if (rc) return rc;
     rc = mixed ? ensure_sell<float>(m, o->precond, half) : ensure_sell<double>(m, o->precond, false);
     if (rc) return rc;
-    if (m->use_sell && !m->asell_valid) {
-        if (mixed)
+    if (m->use_sell && (!m->asell_valid || m->asell_half != half)) {
+        if (m->asell_half != half) drop_krylov_graphs(m);
+        m->asell_half = half;
+        if (half)
+            hipLaunchKernelGGL(k_to_sell_half, dim3(grid_cap((size_t)m->a_sell.nblk, 8192)), dim3(kBlock), 0,
+                               m->stream, (size_t)m->a_sell.nblk, m->a_sell.src, m->af, m->asvh, m->asvs);
+        else if (mixed)
             hipLaunchKernelGGL(k_to_sell<float>, dim3(grid_cap((size_t)m->a_sell.nblk, 8192)), dim3(kBlock), 0,
                                m->stream, (size_t)m->a_sell.nblk, m->a_sell.src, m->af, m->asvf);
         else

@@ -139,9 +139,9 @@ def test_finite_difference_jacobian_solve(ctx):
 
 
 def test_mixed_fp16_factor_solve(ctx):
-    """mixed = 2: the multicolour ILU(0) factors of the block-SELL sweeps in FP16 with per-block FP32 scales.  The
-    steady solve converges to the FP64 solve's state (forces, lift and drag within 1e-8) in the same number of
-    Newton iterations (+-1), and its iterates differ from the FP32-factor solve's, so the FP16 sweeps ran."""
+    """mixed = 2: the block-SELL copies of the Jacobian and of the multicolour ILU(0) factors in FP16 with per-block
+    FP32 scales.  The steady solve converges to the FP64 solve's state (forces, lift and drag within 1e-8) in the
+    same number of Newton iterations (+-1), and its iterates differ from the FP32 solve's, so the FP16 kernels ran."""
     print("mixed solve with FP16 factors", flush=True)
     old = os.environ.get("CFD_SELL")
     os.environ["CFD_SELL"] = "1"  # block-SELL kernels on this small mesh
