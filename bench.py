@@ -20,9 +20,13 @@ restatement of the reference) on the same mesh and state and checks both
 GPU builds against it.
 
 Also measured: HLLC and AUSM+, FP32, unfused gradient/limiter and first-order
-variants ("variants"); a converged steady solve of the 48x16x16 bump channel,
-plain FP64 vs mixed precision ("solves"); and the serving loop against the
-plain FP64 path on a stream of solve requests ("serving").
+variants ("variants"); converged steady solves of the 48x16x16 and 192x64x64
+bump channels, plain FP64 vs mixed precision ("solves", "solve_192x64x64");
+the serving loop against the plain FP64 path on a stream of solve requests
+("serving"); and 512 small cases solved as one batch ("batched_small_cases").
+The serving and batch legs' mixed solves use mixed = 2 by default (FP32
+Jacobian, factors and Krylov basis, with the block-SELL copies of the Jacobian
+and of the multicolour ILU(0) factors in FP16); --mixed-level 1 keeps FP32.
 """
 import argparse
 import ctypes
@@ -256,7 +260,7 @@ def bump_dims(n_nodes):
 
 
 SOLVE_KW = dict(max_iter=400, tol=1e-10, krylov_m=30, max_restarts=4, rtol=1e-3)
-MIXED_LEVEL = True  # mixed= of the serving and batch legs' mixed solves (2 with --fp16-factors)
+MIXED_LEVEL = 2  # mixed= of the serving and batch legs' mixed solves (--mixed-level)
 
 
 def solve_leg(ctx, cpu=True):
@@ -341,7 +345,7 @@ def solve3_leg(ctx):
                                                                               np.array(r0.force))) /
                                                                  np.max(np.abs(r0.force))),
                                   "state_rel_err_vs_fp64": float(np.max(np.abs(q2 - q0)) / np.max(np.abs(q0)))},
-            "mixed_fp16_factors": {"seconds": r3.seconds, "newton": r3.iterations, "krylov": r3.krylov_total,
+            "mixed_fp16_storage": {"seconds": r3.seconds, "newton": r3.iterations, "krylov": r3.krylov_total,
                                    "converged": bool(r3.converged), "lift": r3.lift, "drag": r3.drag,
                                    "force_rel_err_vs_fp64": float(np.max(np.abs(np.array(r3.force) -
                                                                                np.array(r0.force))) /
@@ -561,9 +565,10 @@ def main():
                     help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")
     ap.add_argument("--batch-fp64-sample", type=int, default=16,
                     help="cases of the batch also solved one at a time in FP64 for the baseline rate (0: none)")
-    ap.add_argument("--fp16-factors", action="store_true",
-                    help="the serving and batch legs' mixed solves store the block-SELL copies of the Jacobian "
-                         "and of the multicolour ILU(0) factors in FP16 (mixed = 2)")
+    ap.add_argument("--mixed-level", type=int, default=2, choices=(1, 2),
+                    help="mixed= of the serving and batch legs' mixed solves: 1 FP32 Jacobian, factors and Krylov "
+                         "basis; 2 (default) as 1 with the block-SELL copies of the Jacobian and of the multicolour "
+                         "ILU(0) factors in FP16")
     ap.add_argument("--autotune", action="store_true",
                     help="solve legs tune each mesh's same-bits kernel variants at its first Newton iteration "
                          "(FP64 and mixed paths alike)")
@@ -582,7 +587,7 @@ def main():
     if args.autotune:
         SOLVE_KW["autotune"] = True
     global MIXED_LEVEL
-    MIXED_LEVEL = cfd.MIXED_FP16 if args.fp16_factors else True
+    MIXED_LEVEL = args.mixed_level
     if args.calibrate_copy:
         import torch
 
