@@ -25,8 +25,9 @@ bump channels, plain FP64 vs mixed precision ("solves", "solve_192x64x64");
 the serving loop against the plain FP64 path on a stream of solve requests
 ("serving"); and 512 small cases solved as one batch ("batched_small_cases").
 The serving and batch legs' mixed solves use mixed = 2 by default (FP32
-Jacobian, factors and Krylov basis, with the block-SELL copies of the Jacobian
-and of the multicolour ILU(0) factors in FP16); --mixed-level 1 keeps FP32.
+Jacobian, factors and Krylov basis, with the block-SELL copies of the
+multicolour ILU(0) factors in FP16); --mixed-level 1 keeps FP32, 3 also puts
+the Jacobian's block-SELL copy in FP16.
 """
 import argparse
 import ctypes
@@ -312,8 +313,9 @@ def solve3_leg(ctx):
     """Bump channel 192x64x64 (815,425 nodes), second-order HLLC with Green-Gauss gradients and the
     Venkatakrishnan limiter, M 0.7, AoA 2 deg, FGMRES(40) with the multicolour ILU(0), to RMS 1e-10: plain FP64 vs
     mixed precision (FP32 Jacobian, ILU(0) and Krylov basis), each alone on the GPU, with the approximate Roe
-    Jacobian; the mixed solve with the block-SELL copies of the Jacobian and the ILU(0) factors stored in FP16
-    (mixed = 2); and the mixed solve with the finite-difference Jacobian of the HLLC flux."""
+    Jacobian; the mixed solve with the block-SELL copies of the ILU(0) factors stored in FP16 (mixed = 2), and of
+    the factors and the Jacobian (mixed = 3); and the mixed solve with the finite-difference Jacobian of the HLLC
+    flux."""
     from cfdsolutions_amd import cfd, mesh
 
     m = mesh.bump_channel(192, 64, 64, bump_h=0.1)
@@ -321,7 +323,8 @@ def solve3_leg(ctx):
     kw = dict(SOLVE_KW, krylov_m=40, alpha_deg=2.0, precond=cfd.PREC_ILU0_MC)
     out = {}
     runs = (("fp64", False, cfd.SOLVE_JAC_APPROX), ("mixed", True, cfd.SOLVE_JAC_APPROX),
-            ("mixed_fp16", cfd.MIXED_FP16, cfd.SOLVE_JAC_APPROX), ("mixed_fd", True, cfd.SOLVE_JAC_FD))
+            ("mixed_fp16", cfd.MIXED_FP16, cfd.SOLVE_JAC_APPROX),
+            ("mixed_fp16_all", cfd.MIXED_FP16_ALL, cfd.SOLVE_JAC_APPROX), ("mixed_fd", True, cfd.SOLVE_JAC_FD))
     for name, mixed, jac in runs:
         dm = cfd.DeviceMesh(ctx, m, p)
         r, _ = dm.solve_steady(p, mixed=mixed, check=False, jacobian=jac, **kw)
@@ -329,7 +332,16 @@ def solve3_leg(ctx):
         print("bench: solve 192x64x64 %-6s %.2f s (%d newton / %d krylov), converged %d, rms %.2e" % (
             name, r.seconds, r.iterations, r.krylov_total, r.converged, r.final_rms), file=sys.stderr, flush=True)
         dm.close()
-    (r0, q0), (r1, q1), (r2, q2), (r3, q3) = out["fp64"], out["mixed"], out["mixed_fd"], out["mixed_fp16"]
+    (r0, q0), (r1, q1), (r2, q2) = out["fp64"], out["mixed"], out["mixed_fd"]
+
+    def level(name):
+        r, q = out[name]
+        return {"seconds": r.seconds, "newton": r.iterations, "krylov": r.krylov_total, "converged": bool(r.converged),
+                "lift": r.lift, "drag": r.drag,
+                "force_rel_err_vs_fp64": float(np.max(np.abs(np.array(r.force) - np.array(r0.force))) /
+                                               np.max(np.abs(r0.force))),
+                "state_rel_err_vs_fp64": float(np.max(np.abs(q - q0)) / np.max(np.abs(q0))),
+                "speedup_vs_fp64": r0.seconds / r.seconds}
     return {"workload": "bump channel 192x64x64 (%d nodes), 2nd-order HLLC + GG + Venkatakrishnan, M 0.7, AoA 2, "
                         "FGMRES(40) + multicolour ILU(0), RMS < 1e-10" % m.n_node,
             "fp64": {"seconds": r0.seconds, "newton": r0.iterations, "krylov": r0.krylov_total,
@@ -345,13 +357,7 @@ def solve3_leg(ctx):
                                                                               np.array(r0.force))) /
                                                                  np.max(np.abs(r0.force))),
                                   "state_rel_err_vs_fp64": float(np.max(np.abs(q2 - q0)) / np.max(np.abs(q0)))},
-            "mixed_fp16_storage": {"seconds": r3.seconds, "newton": r3.iterations, "krylov": r3.krylov_total,
-                                   "converged": bool(r3.converged), "lift": r3.lift, "drag": r3.drag,
-                                   "force_rel_err_vs_fp64": float(np.max(np.abs(np.array(r3.force) -
-                                                                               np.array(r0.force))) /
-                                                                  np.max(np.abs(r0.force))),
-                                   "state_rel_err_vs_fp64": float(np.max(np.abs(q3 - q0)) / np.max(np.abs(q0))),
-                                   "speedup_vs_fp64": r0.seconds / r3.seconds},
+            "mixed_fp16_factors": level("mixed_fp16"), "mixed_fp16_factors_and_matrix": level("mixed_fp16_all"),
             "speedup_mixed": r0.seconds / r1.seconds}
 
 
@@ -565,10 +571,10 @@ def main():
                     help="small cases (8k-40k nodes, Roe, M 0.3-0.8, AoA -2..4) solved as one batch (0: skip)")
     ap.add_argument("--batch-fp64-sample", type=int, default=16,
                     help="cases of the batch also solved one at a time in FP64 for the baseline rate (0: none)")
-    ap.add_argument("--mixed-level", type=int, default=2, choices=(1, 2),
+    ap.add_argument("--mixed-level", type=int, default=2, choices=(1, 2, 3),
                     help="mixed= of the serving and batch legs' mixed solves: 1 FP32 Jacobian, factors and Krylov "
-                         "basis; 2 (default) as 1 with the block-SELL copies of the Jacobian and of the multicolour "
-                         "ILU(0) factors in FP16")
+                         "basis; 2 (default) as 1 with the block-SELL copies of the multicolour ILU(0) factors in "
+                         "FP16; 3 as 2 with the Jacobian's block-SELL copy in FP16 too")
     ap.add_argument("--autotune", action="store_true",
                     help="solve legs tune each mesh's same-bits kernel variants at its first Newton iteration "
                          "(FP64 and mixed paths alike)")

@@ -20,8 +20,9 @@ FP64, FP32 = 0, 1
 ARITH_EXACT, ARITH_FAST = 0, 1
 SOLVE_JAC_APPROX, SOLVE_JAC_FD = 0, 1
 PREC_NONE, PREC_BJACOBI, PREC_ILU0, PREC_ILU0_MC = 0, 1, 2, 3
-# mixed= of the linear solves: False/0 FP64, True/1 FP32 storage, MIXED_FP16 FP32 with FP16-stored multicolour factors
-MIXED_FP16 = 2
+# mixed= of the linear solves: False/0 FP64; True/1 FP32 storage; MIXED_FP16 as 1 with the multicolour factors'
+# block-SELL copies in FP16; MIXED_FP16_ALL as MIXED_FP16 with the matrix's block-SELL copy in FP16 too
+MIXED_FP16, MIXED_FP16_ALL = 2, 3
 SOLVER_FGMRES, SOLVER_BLOCK_GS, SOLVER_LUSGS = 0, 1, 2
 JAC_FD_CENTRAL, JAC_FD_FORWARD, JAC_FD_BACKWARD, JAC_FD_ALTERNATE = 0, 1, 2, 3
 FIELD_Q, FIELD_GRAD, FIELD_PHI, FIELD_RES_CONV, FIELD_RES_DISS, FIELD_DT_ACC, FIELD_DT, FIELD_JAC_B, FIELD_X = range(9)

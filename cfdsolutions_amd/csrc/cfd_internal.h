@@ -232,9 +232,9 @@ struct cfd_mesh {
     BlockSell a_sell, mcl_sell, mcu_sell;
     double *asv = nullptr, *lsv = nullptr, *usv = nullptr, *dsv = nullptr;
     float *asvf = nullptr, *lsvf = nullptr, *usvf = nullptr, *dsvf = nullptr;
-    // FP16 block-SELL copies of the multicolour factors and of A for linear
-    // solves with mixed = 2: each block's entries over its largest magnitude,
-    // with that scale kept per block in FP32
+    // FP16 block-SELL copies of the multicolour factors (linear solves with
+    // mixed >= 2) and of A (mixed = 3): each block's entries over its largest
+    // magnitude, with that scale kept per block in FP32
     _Float16 *lsvh = nullptr, *usvh = nullptr, *dsvh = nullptr, *asvh = nullptr;
     float *lsvs = nullptr, *usvs = nullptr, *dsvs = nullptr, *asvs = nullptr;
     bool asell_half = false;  // the SELL copy of A is the FP16 one

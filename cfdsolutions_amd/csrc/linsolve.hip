@@ -2302,7 +2302,7 @@ namespace {
 // Allocates the block-SELL value arrays of storage type TM: A always, the
 // multicolour factors for that preconditioner.
 template <class TM>
-int ensure_sell(cfd_mesh *m, int precond, bool half) {
+int ensure_sell(cfd_mesh *m, int precond, bool half, bool half_a) {
     if (!m->use_sell) return CFD_OK;
     TM *&av = SellVals<TM>::a(m);
     if (!av) av = (TM *)cfd_dalloc(m, sizeof(TM) * 25 * (size_t)m->a_sell.nblk);
@@ -2311,9 +2311,10 @@ int ensure_sell(cfd_mesh *m, int precond, bool half) {
     if (half) {
         const size_t nl = std::max<int64_t>(m->mcl_sell.nblk, 64), nu = std::max<int64_t>(m->mcu_sell.nblk, 64),
                      nd = ((size_t)m->N + 63) / 64 * 64;
-        if (!m->asvh) m->asvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * (size_t)m->a_sell.nblk);
-        if (!m->asvs) m->asvs = (float *)cfd_dalloc(m, sizeof(float) * (size_t)m->a_sell.nblk);
-        if (!m->asvh || !m->asvs) return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (FP16 A)");
+        if (half_a && !m->asvh) m->asvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * (size_t)m->a_sell.nblk);
+        if (half_a && !m->asvs) m->asvs = (float *)cfd_dalloc(m, sizeof(float) * (size_t)m->a_sell.nblk);
+        if (half_a && (!m->asvh || !m->asvs))
+            return cfd_set_error(m->ctx, CFD_E_OOM, "out of device memory (FP16 A)");
         if (!m->lsvh) m->lsvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * nl);
         if (!m->usvh) m->usvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * nu);
         if (!m->dsvh) m->dsvh = (_Float16 *)cfd_dalloc(m, sizeof(_Float16) * 25 * nd);
@@ -2337,21 +2338,23 @@ int prepare_fgmres(cfd_mesh *m, const cfd_linsolve_opts *o) {
         return cfd_set_error(m->ctx, CFD_E_ARG, "krylov_m must be in [1, %d]", kMaxKrylov);
     if (o->precond < CFD_PREC_NONE || o->precond > CFD_PREC_ILU0_MC)
         return cfd_set_error(m->ctx, CFD_E_ARG, "unknown preconditioner %d", o->precond);
-    if (o->mixed < 0 || o->mixed > 2) return cfd_set_error(m->ctx, CFD_E_ARG, "mixed must be 0, 1 or 2");
+    if (o->mixed < 0 || o->mixed > 3) return cfd_set_error(m->ctx, CFD_E_ARG, "mixed must be 0, 1, 2 or 3");
     const bool mixed = o->mixed != 0;
-    // FP16 copies of A and of the factors: mixed = 2, multicolour ILU(0) on
-    // the block-SELL kernels
-    const bool half = o->mixed == 2 && o->precond == CFD_PREC_ILU0_MC && m->use_sell;
+    // FP16 copies of the factors (mixed >= 2) and of A (mixed = 3), for the
+    // multicolour ILU(0) on the block-SELL kernels
+    const bool sell_mc = o->precond == CFD_PREC_ILU0_MC && m->use_sell;
+    const bool half = o->mixed >= 2 && sell_mc, half_a = o->mixed == 3 && sell_mc;
     if (mixed != m->jac_mixed)
         return cfd_set_error(m->ctx, CFD_E_ARG, "Jacobian precision does not match linsolve precision");
     int rc = ensure_krylov(m, o->krylov_m, mixed);
     if (rc) return rc;
-    rc = mixed ? ensure_sell<float>(m, o->precond, half) : ensure_sell<double>(m, o->precond, false);
+    rc = mixed ? ensure_sell<float>(m, o->precond, half, half_a)
+               : ensure_sell<double>(m, o->precond, false, false);
     if (rc) return rc;
-    if (m->use_sell && (!m->asell_valid || m->asell_half != half)) {
-        if (m->asell_half != half) drop_krylov_graphs(m);
-        m->asell_half = half;
-        if (half)
+    if (m->use_sell && (!m->asell_valid || m->asell_half != half_a)) {
+        if (m->asell_half != half_a) drop_krylov_graphs(m);  // captured steps launch the other SpMV kernel
+        m->asell_half = half_a;
+        if (half_a)
             hipLaunchKernelGGL(k_to_sell_half, dim3(grid_cap((size_t)m->a_sell.nblk, 8192)), dim3(kBlock), 0,
                                m->stream, (size_t)m->a_sell.nblk, m->a_sell.src, m->af, m->asvh, m->asvs);
         else if (mixed)

@@ -136,10 +136,12 @@ typedef struct cfd_linsolve_opts {
     double rtol;            /* FGMRES: stop at ||r|| <= rtol*||b|| */
     int32_t sweeps;         /* block GS / LU-SGS sweeps */
     int32_t mixed;          /* 0: FP64; 1: FP32 matrix, preconditioner and Krylov basis;
-                             * 2: as 1, with the block-SELL copies of the matrix and
-                             *    of the multicolour ILU(0) factors (systems of
-                             *    131,072 rows or more) stored in FP16 (per-block
-                             *    FP32 scales, FP32 arithmetic) */
+                             * 2: as 1, with the block-SELL copies of the multicolour
+                             *    ILU(0) factors (systems of 131,072 rows or more)
+                             *    stored in FP16 (per-block FP32 scales, FP32
+                             *    arithmetic);
+                             * 3: as 2, with the block-SELL copy of the matrix for
+                             *    the SpMV in FP16 as well */
 } cfd_linsolve_opts;
 
 typedef struct cfd_linsolve_stats {

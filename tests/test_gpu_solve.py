@@ -139,9 +139,10 @@ def test_finite_difference_jacobian_solve(ctx):
 
 
 def test_mixed_fp16_factor_solve(ctx):
-    """mixed = 2: the block-SELL copies of the Jacobian and of the multicolour ILU(0) factors in FP16 with per-block
-    FP32 scales.  The steady solve converges to the FP64 solve's state (forces, lift and drag within 1e-8) in the
-    same number of Newton iterations (+-1), and its iterates differ from the FP32 solve's, so the FP16 kernels ran."""
+    """mixed = 2 (the block-SELL copies of the multicolour ILU(0) factors in FP16 with per-block FP32 scales) and
+    mixed = 3 (the Jacobian's block-SELL copy in FP16 too).  Each steady solve converges to the FP64 solve's state
+    (forces, lift and drag within 1e-8) in the same number of Newton iterations (+-1), and its iterates differ from
+    the FP32 solve's, so the FP16 kernels ran."""
     print("mixed solve with FP16 factors", flush=True)
     old = os.environ.get("CFD_SELL")
     os.environ["CFD_SELL"] = "1"  # block-SELL kernels on this small mesh
@@ -149,7 +150,7 @@ def test_mixed_fp16_factor_solve(ctx):
         m = cmesh.bump_channel(24, 8, 8)
         p = cfd.make_params(mach=0.6, alpha_deg=2.0, flux="roe", order=1)
         res = {}
-        for mixed in (False, True, cfd.MIXED_FP16):
+        for mixed in (False, True, cfd.MIXED_FP16, cfd.MIXED_FP16_ALL):
             dm = cfd.DeviceMesh(ctx, m, p)
             r, hist = dm.solve_steady(p, alpha_deg=2.0, max_iter=300, tol=1e-10, krylov_m=30, max_restarts=4,
                                       rtol=1e-3, precond=cfd.PREC_ILU0_MC, mixed=mixed)
@@ -160,10 +161,14 @@ def test_mixed_fp16_factor_solve(ctx):
             del os.environ["CFD_SELL"]
         else:
             os.environ["CFD_SELL"] = old
-    (r64, _), (r32, h32), (r16, h16) = res[0], res[1], res[2]
-    assert r64.converged and r32.converged and r16.converged
-    assert abs(r16.iterations - r64.iterations) <= 1
+    (r64, _), (r32, h32) = res[0], res[1]
+    assert r64.converged and r32.converged
     scale = np.max(np.abs(r64.force))
-    assert np.max(np.abs(np.array(r16.force) - np.array(r64.force))) <= 1e-8 * scale
-    assert abs(r16.lift - r64.lift) <= 1e-8 * scale and abs(r16.drag - r64.drag) <= 1e-8 * scale
-    assert h16 != h32
+    for level in (2, 3):
+        r16, h16 = res[level]
+        assert r16.converged
+        assert abs(r16.iterations - r64.iterations) <= 1
+        assert np.max(np.abs(np.array(r16.force) - np.array(r64.force))) <= 1e-8 * scale
+        assert abs(r16.lift - r64.lift) <= 1e-8 * scale and abs(r16.drag - r64.drag) <= 1e-8 * scale
+        assert h16 != h32
+    assert res[2][1] != res[3][1]
