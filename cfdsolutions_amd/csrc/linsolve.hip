@@ -1342,15 +1342,133 @@ __global__ __launch_bounds__(64) void k_mdot_final(int nparts, int nv, const dou
     if (threadIdx.x == 0) out[k] = s;
 }
 
-// w -= sum_k h_k V_k (k ascending, as CGS in the oracle)
-template <class TV>
-__global__ __launch_bounds__(kBlock) void k_maxpy(size_t n, int nv, const TV *__restrict__ V, size_t ld,
-                                                  const double *__restrict__ h, TV *__restrict__ w) {
-    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
-        double wi = (double)w[i];
-        for (int k = 0; k < nv; k++) wi = wi - h[k] * (double)V[ld * k + i];
-        w[i] = (TV)wi;
+// U entries i, i + stride, ... of the pass: their updates (the loads of 8
+// vectors for all U entries issued together), stores, and products.
+template <class TV, int NVMAX, bool NORM, int U>
+__device__ __forceinline__ void cgs_entries(size_t o, size_t i, size_t stride, int nv, const TV *__restrict__ V,
+                                            size_t ld, const double *h, TV *__restrict__ w, double *acc) {
+    constexpr int NC = (NVMAX + 7) / 8;
+    double wi[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) wi[u] = (double)w[o + i + u * stride];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if (8 * c < nv) {
+            TV vv[U][8];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++)
+                    if (8 * c + kk < nv) vv[u][kk] = V[ld * (8 * c + kk) + o + i + u * stride];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++)
+                    if (8 * c + kk < nv) wi[u] = wi[u] - h[8 * c + kk] * (double)vv[u][kk];
+        }
     }
+    double wd[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const TV wt = (TV)wi[u];
+        w[o + i + u * stride] = wt;
+        wd[u] = (double)wt;
+    }
+    if (NORM) {
+#pragma unroll
+        for (int u = 0; u < U; u++) acc[0] += wd[u] * wd[u];
+        return;
+    }
+    // read V again rather than hold its values across the update
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if (8 * c < nv) {
+            TV vv[U][8];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++)
+                    if (8 * c + kk < nv) vv[u][kk] = V[ld * (8 * c + kk) + o + i + u * stride];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++)
+                    if (8 * c + kk < nv) acc[8 * c + kk] += (double)vv[u][kk] * wd[u];
+        }
+    }
+}
+
+// One fused CGS pass over this thread's strided entries of [o + i0, o + n)
+// (the entries block_dots visits, in its order): each entry first loses its
+// projections on V_0..V_{nv-1} (w_i -= h_k V_ki for k ascending, as CGS in
+// the oracle, rounded once to TV) and is stored; then its products with
+// V_0..V_{nv-1} (or, NORM, its square) are added to the partial sums, which
+// go through block_dots' block tree into part[0..nv) (part[0]).  The results
+// are those of the update over all entries followed by block_dots, bit for
+// bit, with one pass over w instead of three and the second read of each
+// V_ki served by the cache.  DEEP (threads with many entries to walk):
+// entries go four at a time (two for long bases) while four remain, so their
+// loads overlap; it costs registers, so short walks take one entry at a time.
+// h: nv coefficients in LDS; nv <= NVMAX.
+template <class TV, int NVMAX, bool NORM, bool DEEP>
+__device__ __forceinline__ void block_cgs_pass(size_t o, size_t i0, size_t n, size_t stride, int nv,
+                                               const TV *__restrict__ V, size_t ld, const double *h,
+                                               TV *__restrict__ w, double *__restrict__ part) {
+    constexpr int NA = NORM ? 1 : NVMAX;
+    constexpr int U = !DEEP ? 1 : NVMAX > 24 ? 2 : 4;
+    __shared__ double wsum[NA][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[NA];
+#pragma unroll
+    for (int k = 0; k < NA; k++) acc[k] = 0.0;
+    size_t i = i0;
+    if (U > 1)
+        for (; i + (U - 1) * stride < n; i += U * stride)
+            cgs_entries<TV, NVMAX, NORM, U>(o, i, stride, nv, V, ld, h, w, acc);
+    for (; i < n; i += stride) cgs_entries<TV, NVMAX, NORM, 1>(o, i, stride, nv, V, ld, h, w, acc);
+    const int nd = NORM ? 1 : nv;
+#pragma unroll
+    for (int k = 0; k < NA; k++) {
+        if (k < nd) {
+            double v = acc[k];
+#pragma unroll
+            for (int sh = 32; sh > 0; sh >>= 1) v += __shfl_xor(v, sh, 64);
+            if (lane == 0) wsum[k][wave] = v;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nd; k += kBlock) part[k] = ((wsum[k][0] + wsum[k][1]) + wsum[k][2]) + wsum[k][3];
+}
+
+// w -= V h, then the dots of w with V_0..V_{nv-1} (DOTS) or with itself
+// (NORM) into part[block][..] (grid: dot_parts(n) blocks, as k_mdot).
+template <class TV, int NVMAX, bool NORM, bool DEEP>
+__global__ __launch_bounds__(kBlock) void k_cgs_pass(size_t n, int nv, const TV *__restrict__ V, size_t ld,
+                                                     const double *__restrict__ hdev, TV *__restrict__ w,
+                                                     double *__restrict__ part) {
+    __shared__ double h[kMaxKrylov + 1];
+    for (int k = threadIdx.x; k < nv; k += kBlock) h[k] = hdev[k];
+    __syncthreads();
+    block_cgs_pass<TV, NVMAX, NORM, DEEP>(0, blockIdx.x * (size_t)kBlock + threadIdx.x, n, (size_t)gridDim.x * kBlock, nv,
+                                    V, ld, h, w, part + (size_t)blockIdx.x * (kMaxKrylov + 1));
+}
+
+// The same over a union of cases: block (c, b) of the segment map updates
+// and reduces case c's range with case c's coefficients out[c][slot..].
+template <class TV, int NVMAX, bool NORM>
+__global__ __launch_bounds__(kBlock) void k_cgs_pass_seg(const int2 *__restrict__ blk, const int *__restrict__ first,
+                                                         const int *__restrict__ off, const int *__restrict__ len,
+                                                         int nv, const TV *__restrict__ V, size_t ld,
+                                                         const double *__restrict__ out, int slot, TV *__restrict__ w,
+                                                         double *__restrict__ part) {
+    const int2 cb = blk[blockIdx.x];
+    __shared__ double h[kMaxKrylov + 1];
+    for (int k = threadIdx.x; k < nv; k += kBlock) h[k] = out[(size_t)cb.x * kBOut + slot + k];
+    __syncthreads();
+    const size_t P = (size_t)(first[cb.x + 1] - first[cb.x]);
+    block_cgs_pass<TV, NVMAX, NORM, false>((size_t)off[cb.x], cb.y * (size_t)kBlock + threadIdx.x, (size_t)len[cb.x],
+                                    P * kBlock, nv, V, ld, h, w, part + (size_t)blockIdx.x * (kMaxKrylov + 1));
 }
 
 // dst = src * (1/scale) with scale read from the device (scale = out[idx])
@@ -1420,19 +1538,6 @@ __global__ __launch_bounds__(64) void k_mdot_seg_final(const int *first, int nv,
     const int k = blockIdx.x, c = blockIdx.y;
     const double s = wave_sum_parts(part, first[c], first[c + 1] - first[c], k);
     if (threadIdx.x == 0) out[(size_t)c * kBOut + slot + k] = s;
-}
-
-// w -= sum_k h_{c,k} V_k with the coefficients of each entry's case
-template <class TV>
-__global__ __launch_bounds__(kBlock) void k_maxpy_seg(size_t n, int nv, const int *__restrict__ ecase,
-                                                      const TV *__restrict__ V, size_t ld,
-                                                      const double *__restrict__ out, int slot, TV *__restrict__ w) {
-    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
-        const double *h = out + (size_t)ecase[i / 5] * kBOut + slot;
-        double wi = (double)w[i];
-        for (int k = 0; k < nv; k++) wi = wi - h[k] * (double)V[ld * k + i];
-        w[i] = (TV)wi;
-    }
 }
 
 // dst = src / sqrt(out[c][slot]) (Arnoldi normalisation, per case)
@@ -1733,6 +1838,34 @@ void mdot(cfd_mesh *m, int nv, const TV *V, size_t ld, const TV *w, int off) {
     hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(64), 0, m->stream, np, nv, m->red, m->redout + off);
 }
 
+// w -= V_{0..nv-1} h (h = m->redout + hoff on the device), then the dots of
+// the new w with V_0..V_{nv-1} (or, NORM, its squared norm) into
+// m->redout + off: the update and mdot in one pass (the same results).
+template <class TV, bool NORM>
+void cgs_pass(cfd_mesh *m, int nv, const TV *V, size_t ld, int hoff, TV *w, int off) {
+    const size_t L = 5 * (size_t)m->N;
+    const int np = dot_parts(L);
+    const double *h = m->redout + hoff;
+    const bool deep = L >= 4 * (size_t)np * kBlock;  // each thread walks four strides or more
+    dim3 g(np), b(kBlock);
+    // NVMAX: the fewest unrolled vectors that cover nv
+#define CFD_CGS(NVMAX)                                                                                          \
+    do {                                                                                                         \
+        if (deep) hipLaunchKernelGGL((k_cgs_pass<TV, NVMAX, NORM, true>), g, b, 0, m->stream, L, nv, V, ld, h, w, \
+                                     m->red);                                                                    \
+        else hipLaunchKernelGGL((k_cgs_pass<TV, NVMAX, NORM, false>), g, b, 0, m->stream, L, nv, V, ld, h, w,     \
+                                m->red);                                                                         \
+    } while (0)
+    if (nv <= 8) CFD_CGS(8);
+    else if (nv <= 16) CFD_CGS(16);
+    else if (nv <= 32) CFD_CGS(32);
+    else if (nv <= 41) CFD_CGS(41);
+    else CFD_CGS(kMaxKrylov + 1);
+#undef CFD_CGS
+    const int nd = NORM ? 1 : nv;
+    hipLaunchKernelGGL(k_mdot_final, dim3(nd), dim3(64), 0, m->stream, np, nd, m->red, m->redout + off);
+}
+
 // Krylov basis of one storage type for restart length mk.  Each precision
 // keeps its own basis, so alternating FP64 and mixed solves on one mesh
 // allocates each once; a basis too short for a larger mk is replaced (the
@@ -1791,13 +1924,11 @@ void arnoldi_step(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, TV *V,
                            m->redout + 2 * (kMaxKrylov + 1));
     apply_prec<TM, TV>(m, m->prec_kind, lu, dinv, vj, zj);
     spmv<TM, TV, TV>(m, zj, w, nullptr);
-    // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
+    // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2, ||w||^2 (each
+    // update fused with the reduction after it)
     mdot<TV>(m, j + 1, V, L, w, 0);
-    hipLaunchKernelGGL(k_maxpy<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, V, L, m->redout, w);
-    mdot<TV>(m, j + 1, V, L, w, kMaxKrylov + 1);
-    hipLaunchKernelGGL(k_maxpy<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, V, L, m->redout + kMaxKrylov + 1,
-                       w);
-    mdot<TV>(m, 1, w, 0, w, 2 * (kMaxKrylov + 1));
+    cgs_pass<TV, false>(m, j + 1, V, L, 0, w, kMaxKrylov + 1);
+    cgs_pass<TV, true>(m, j + 1, V, L, kMaxKrylov + 1, w, 2 * (kMaxKrylov + 1));
     hipMemcpyAsync(m->hred, m->redout, sizeof(double) * (2 * (kMaxKrylov + 1) + 1), hipMemcpyDeviceToHost, m->stream);
 }
 
@@ -1866,6 +1997,26 @@ void mdot_seg(cfd_mesh *m, int nv, const TV *V, size_t ld, const TV *w, int slot
                        slot);
 }
 
+// Per case: w -= V h_c (h_c = m->bred[c][hslot..]), then the dots of the new
+// w with V_0..V_{nv-1} (or, NORM, its squared norm) into m->bred[c][slot..].
+template <class TV, bool NORM>
+void cgs_pass_seg(cfd_mesh *m, int nv, const TV *V, size_t ld, int hslot, TV *w, int slot) {
+    const SegMap &sg = m->seg_dot;
+    dim3 g(sg.nblocks), b(kBlock);
+#define CFD_CGS_SEG(NVMAX, NORM_)                                                                                   \
+    hipLaunchKernelGGL((k_cgs_pass_seg<TV, NVMAX, NORM_>), g, b, 0, m->stream, sg.blk, sg.first, sg.off, sg.len, nv, \
+                       V, ld, m->bred, hslot, w, m->bpart)
+    if (nv <= 8) CFD_CGS_SEG(8, NORM);
+    else if (nv <= 16) CFD_CGS_SEG(16, NORM);
+    else if (nv <= 32) CFD_CGS_SEG(32, NORM);
+    else if (nv <= 41) CFD_CGS_SEG(41, NORM);
+    else CFD_CGS_SEG(kMaxKrylov + 1, NORM);
+#undef CFD_CGS_SEG
+    const int nd = NORM ? 1 : nv;
+    hipLaunchKernelGGL(k_mdot_seg_final, dim3(nd, m->ncase), dim3(64), 0, m->stream, sg.first, nd, m->bpart, m->bred,
+                       slot);
+}
+
 // Arnoldi step j of the batched FGMRES: the operations of arnoldi_step with
 // per-case dots, coefficients and norms.
 template <class TM, class TV>
@@ -1880,11 +2031,8 @@ void arnoldi_step_batch(cfd_mesh *m, const TM *a, const TM *lu, const TM *dinv, 
     apply_prec<TM, TV>(m, m->prec_kind, lu, dinv, vj, zj);
     spmv<TM, TV, TV>(m, zj, w, nullptr);
     mdot_seg<TV>(m, j + 1, V, L, w, 0);
-    hipLaunchKernelGGL(k_maxpy_seg<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, m->ecase, V, L, m->bred, 0, w);
-    mdot_seg<TV>(m, j + 1, V, L, w, kMaxKrylov + 1);
-    hipLaunchKernelGGL(k_maxpy_seg<TV>, dim3(gN), dim3(kBlock), 0, m->stream, L, j + 1, m->ecase, V, L, m->bred,
-                       kMaxKrylov + 1, w);
-    mdot_seg<TV>(m, 1, w, 0, w, 2 * (kMaxKrylov + 1));
+    cgs_pass_seg<TV, false>(m, j + 1, V, L, 0, w, kMaxKrylov + 1);
+    cgs_pass_seg<TV, true>(m, j + 1, V, L, kMaxKrylov + 1, w, 2 * (kMaxKrylov + 1));
     hipMemcpyAsync(m->hbred, m->bred, sizeof(double) * kBOut * (size_t)m->ncase, hipMemcpyDeviceToHost, m->stream);
 }
 
