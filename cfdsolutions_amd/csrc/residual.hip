@@ -295,13 +295,16 @@ __device__ __forceinline__ bool near_vacuum(const T *Q, T P) {
 // multiplication, so a node divides once per variable (limiter_finish)
 // instead of once per neighbour; the exact build divides per neighbour as the
 // reference does and leaves pd alone.
-template <class T, int KIND>  // 1 Venkatakrishnan, 2 Barth-Jespersen
+// F0 (fast build only): the reconstruction fraction is 0, so the face value
+// is qc + grad . R and the neighbour's state is not read.
+template <class T, int KIND, bool F0 = false>  // KIND 1 Venkatakrishnan, 2 Barth-Jespersen
 __device__ __forceinline__ void limiter_nbr(const T *qj, T xj, T yj, T zj, const T *qc, T xc, T yc, T zc, const T *gr,
                                             const T *qmin, const T *qmax, T eps2, T frac, T *ph, T *pd) {
     const T Rx = T(0.5) * (xj - xc), Ry = T(0.5) * (yj - yc), Rz = T(0.5) * (zj - zc);
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-        const T qf = qc[k] + frac * (qj[k] - qc[k]) + (T(1.0) - frac) * (gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz);
+        const T gR = gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz;
+        const T qf = F0 ? qc[k] + gR : qc[k] + frac * (qj[k] - qc[k]) + (T(1.0) - frac) * gR;
         const T dM = qf - qc[k];
         // The three cases (dM > 0: towards the maximum, dM < 0: towards the
         // minimum, otherwise no bound) as selects, so the five variables'
@@ -347,14 +350,16 @@ __device__ __forceinline__ void limiter_finish(T *ph, const T *pd) {
 // Pressure of the face state towards one neighbour with the limiter ph
 // (Compute_Limiter_PressureCorrection, Limiters.cpp:273-304): a negative
 // pressure zeroes the limiter; a near-vacuum one marks the node as risky.
-template <class T>
+template <class T, bool F0 = false>
 __device__ __forceinline__ void pressure_nbr(const T *qj, T xj, T yj, T zj, const T *qc, T xc, T yc, T zc,
                                              const T *gr, T frac, T g, T *ph, bool &zeroed, bool &risky) {
     const T Rx = T(0.5) * (xj - xc), Ry = T(0.5) * (yj - yc), Rz = T(0.5) * (zj - zc);
     T Q[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++)
-        Q[k] = qc[k] + ph[k] * (frac * (qj[k] - qc[k]) + (T(1.0) - frac) * (gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz));
+    for (int k = 0; k < 5; k++) {
+        const T gR = gr[3 * k] * Rx + gr[3 * k + 1] * Ry + gr[3 * k + 2] * Rz;
+        Q[k] = F0 ? qc[k] + ph[k] * gR : qc[k] + ph[k] * (frac * (qj[k] - qc[k]) + (T(1.0) - frac) * gR);
+    }
     const T P = (g - T(1.0)) * (Q[4] - (T(0.5) / Q[0]) * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]));
     if (P < T(0.0)) {
 #pragma unroll
@@ -385,7 +390,7 @@ __device__ __forceinline__ bool pressure_final(const T *qc, T g, bool zeroed, bo
 // bound of |E|; the slack covers the rounding of both this bound and the face
 // states.  When it holds, the pressure pass would change nothing: every face
 // has a positive density and a pressure above the vacuum margin.
-template <class T>
+template <class T, bool F0 = false>
 __device__ __forceinline__ bool faces_safe(const T *qc, const T *gr, const T *qmin, const T *qmax, const T *ph,
                                            T frac, T rmax, T g) {
     const T slack = sizeof(T) == 8 ? T(1e-12) : T(1e-5);
@@ -394,7 +399,7 @@ __device__ __forceinline__ bool faces_safe(const T *qc, const T *gr, const T *qm
     for (int k = 0; k < 5; k++) {
         const T A = tmax(qmax[k] - qc[k], qc[k] - qmin[k]);
         const T G = sqrt(gr[3 * k] * gr[3 * k] + gr[3 * k + 1] * gr[3 * k + 1] + gr[3 * k + 2] * gr[3 * k + 2]);
-        e[k] = ph[k] * (fabs(frac) * A + fabs(T(1.0) - frac) * G * rmax);
+        e[k] = F0 ? ph[k] * (G * rmax) : ph[k] * (fabs(frac) * A + fabs(T(1.0) - frac) * G * rmax);
         e[k] = e[k] * (T(1.0) + slack) + slack * fabs(qc[k]);
     }
     const T rho = qc[0] - e[0];
@@ -413,7 +418,7 @@ __device__ __forceinline__ bool faces_safe(const T *qc, const T *gr, const T *qm
 // when faces_safe shows it cannot act.  Returns whether a final face state of
 // the node is near vacuum (the only case in which the Roe pressure correction
 // can act).  Neither pass depends on the order of the neighbours.
-template <class T, int KIND>  // 1 Venkatakrishnan, 2 Barth-Jespersen
+template <class T, int KIND, bool F0 = false>  // KIND 1 Venkatakrishnan, 2 Barth-Jespersen
 __device__ __forceinline__ bool limiter_passes(int N, int deg, size_t base, const int *__restrict__ nbr,
                                                const T *__restrict__ qa, const T *qc, T xc, T yc, T zc, const T *gr,
                                                const T *qmin, const T *qmax, T v, T g, T frac, T k3, T *ph) {
@@ -435,24 +440,24 @@ __device__ __forceinline__ bool limiter_passes(int N, int deg, size_t base, cons
             if (jr >= N) continue;
             const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
 #pragma unroll
-            for (int e = 0; e < 8; e++) r[u][e] = rj[e];
+            for (int e = F0 ? 5 : 0; e < 8; e++) r[u][e] = rj[e];
         }
 #pragma unroll
         for (int u = 0; u < 2; u++) {
             if ((u ? j1 : j0) >= N) continue;
-            limiter_nbr<T, KIND>(r[u], r[u][5], r[u][6], r[u][7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph, pd);
+            limiter_nbr<T, KIND, F0>(r[u], r[u][5], r[u][6], r[u][7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph, pd);
             const T Rx = T(0.5) * (r[u][5] - xc), Ry = T(0.5) * (r[u][6] - yc), Rz = T(0.5) * (r[u][7] - zc);
             r2 = tmax(r2, Rx * Rx + Ry * Ry + Rz * Rz);
         }
     }
     limiter_finish<T, KIND>(ph, pd);
-    if (faces_safe<T>(qc, gr, qmin, qmax, ph, frac, sqrt(r2), g)) return false;
+    if (faces_safe<T, F0>(qc, gr, qmin, qmax, ph, frac, sqrt(r2), g)) return false;
     bool zeroed = false, risky = false;
     for (int n = 0; n < deg; n++) {
         const int jr = nbr[base + (size_t)n * kSlice];
         if (jr >= N) continue;
         const T *rj = qa + kQA * (size_t)(jr >= 0 ? jr : ~jr);
-        pressure_nbr<T>(rj, rj[5], rj[6], rj[7], qc, xc, yc, zc, gr, frac, g, ph, zeroed, risky);
+        pressure_nbr<T, F0>(rj, rj[5], rj[6], rj[7], qc, xc, yc, zc, gr, frac, g, ph, zeroed, risky);
     }
     return pressure_final<T>(qc, g, zeroed, risky);
 }
@@ -576,7 +581,7 @@ __device__ __forceinline__ void staged_walk(const T *qa, const T *isx, const T *
     }
 }
 
-template <class T, int KIND, bool STAGE>
+template <class T, int KIND, bool STAGE, bool F0>
 __global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restrict__ ip, const int *__restrict__ sptr,
                                                        const int *__restrict__ nbr, const T *__restrict__ isx,
                                                        const T *__restrict__ isy, const T *__restrict__ isz,
@@ -638,7 +643,7 @@ __global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restr
         staged_walk<T, false>(qa, isx, isy, isz, nbr, base, deg, slot, aux, lane,
                               [&](int jr, const T *r, T, T, T) {
             if (jr >= N) return;
-            limiter_nbr<T, KIND>(r, r[5], r[6], r[7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph, pd);
+            limiter_nbr<T, KIND, F0>(r, r[5], r[6], r[7], qc, xc, yc, zc, gr, qmin, qmax, eps2, frac, ph, pd);
             const T Rx = T(0.5) * (r[5] - xc), Ry = T(0.5) * (r[6] - yc), Rz = T(0.5) * (r[7] - zc);
             r2 = tmax(r2, Rx * Rx + Ry * Ry + Rz * Rz);
         });
@@ -646,11 +651,11 @@ __global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restr
         // the pressure pass, unless the bounds show it would change nothing
         // (a lane with nothing to check walks no incidences)
         bool zeroed = false, risky = false;
-        const bool skip = faces_safe<T>(qc, gr, qmin, qmax, ph, frac, sqrt(r2), g);
+        const bool skip = faces_safe<T, F0>(qc, gr, qmin, qmax, ph, frac, sqrt(r2), g);
         staged_walk<T, false>(qa, isx, isy, isz, nbr, base, skip ? 0 : deg, slot, aux, lane,
                               [&](int jr, const T *r, T, T, T) {
             if (jr >= N) return;
-            pressure_nbr<T>(r, r[5], r[6], r[7], qc, xc, yc, zc, gr, frac, g, ph, zeroed, risky);
+            pressure_nbr<T, F0>(r, r[5], r[6], r[7], qc, xc, yc, zc, gr, frac, g, ph, zeroed, risky);
         });
         if (pressure_final<T>(qc, g, zeroed, risky)) atomicAdd(risk, 1);
 #pragma unroll
@@ -685,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_gg_limiter(int N, const int *__restr
         rb[f] = gr[f];
     }
     T ph[5];
-    const bool risky = limiter_passes<T, KIND>(N, deg, base, nbr, qa, qc, xc, yc, zc, gr, qmin, qmax, v, g, frac, k3, ph);
+    const bool risky = limiter_passes<T, KIND, F0>(N, deg, base, nbr, qa, qc, xc, yc, zc, gr, qmin, qmax, v, g, frac, k3, ph);
     if (risky) atomicAdd(risk, 1);
 #pragma unroll
     for (int k = 0; k < 5; k++) rb[kPHI + k] = ph[k];
@@ -1454,16 +1459,28 @@ int launch_gradient_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p
     const bool stage = env && (env[0] == '0' || env[0] == '1') ? env[0] == '1'
                        : tuned >= 0                            ? tuned == 1
                                                                : sizeof(T) == 4;
-#define CFD_GL_LAUNCH(KIND, STAGE)                                                                                 \
-    hipLaunchKernelGGL((k_gg_limiter<T, KIND, STAGE>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr,     \
+    // the fast build drops the fraction's terms when it is 0
+#if CFD_FAST_MATH
+    const bool f0 = frac == T(0.0);
+#else
+    const bool f0 = false;
+#endif
+#define CFD_GL_LAUNCH(KIND, STAGE, F0)                                                                             \
+    hipLaunchKernelGGL((k_gg_limiter<T, KIND, STAGE, F0>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, \
                        m->sptr, m->nbr, b.isx, b.isy, b.isz, b.vol, b.qa, b.qb, g, frac, k3, m->risk)
+#define CFD_GL_STAGE(KIND, F0)          \
+    do {                                \
+        if (stage) CFD_GL_LAUNCH(KIND, true, F0); \
+        else CFD_GL_LAUNCH(KIND, false, F0);      \
+    } while (0)
     if (p->limiter == CFD_LIM_VENKAT) {
-        if (stage) CFD_GL_LAUNCH(1, true);
-        else CFD_GL_LAUNCH(1, false);
+        if (f0) CFD_GL_STAGE(1, true);
+        else CFD_GL_STAGE(1, false);
     } else {
-        if (stage) CFD_GL_LAUNCH(2, true);
-        else CFD_GL_LAUNCH(2, false);
+        if (f0) CFD_GL_STAGE(2, true);
+        else CFD_GL_STAGE(2, false);
     }
+#undef CFD_GL_STAGE
 #undef CFD_GL_LAUNCH
     CFD_HIP(m, hipGetLastError());
     return launch_roe_correction<T>(m, b, p, nb, g, frac);
