@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "cfd_internal.h"
+#include "setup_levels.h"
 
 // ---------------------------------------------------------------------------
 // errors and allocation
@@ -156,38 +157,6 @@ static void build_block_sell(int n, RB rb, RE re, const std::vector<int> &ja, st
 
 static int upload_block_sell(cfd_mesh *m, const std::vector<int> &off, const std::vector<int> &col,
                              const std::vector<int> &src, BlockSell &bs);
-
-// ILU(0) merge map of a BSR pattern: for lower entry kk = (i, k) and the t-th
-// upper block of row k, mqq[moff[kk] + t] is the slot of row i in the same
-// column, or -1.
-static bool build_merge_map(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau, int N,
-                            std::vector<int> &moff, std::vector<int> &mqq, int64_t &total) {
-    moff.assign(ia[N], 0);
-    int64_t tot = 0;
-    for (int i = 0; i < N; i++)
-        for (int kk = ia[i]; kk < iau[i]; kk++) {
-            moff[kk] = (int)tot;
-            const int k = ja[kk];
-            tot += ia[k + 1] - (iau[k] + 1);
-            if (tot > INT32_MAX) return false;
-        }
-    mqq.assign((size_t)tot, -1);
-    cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
-        for (int i = (int)i0; i < (int)i1; i++)
-            for (int kk = ia[i]; kk < iau[i]; kk++) {
-                const int k = ja[kk];
-                int qq = kk + 1;
-                const int qe = ia[i + 1];
-                for (int pp = iau[k] + 1, t2 = 0; pp < ia[k + 1]; pp++, t2++) {
-                    while (qq < qe && ja[qq] < ja[pp]) qq++;
-                    if (qq < qe && ja[qq] == ja[pp]) mqq[(size_t)moff[kk] + t2] = qq;
-                }
-            }
-    });
-    total = tot;
-    if (mqq.empty()) mqq.push_back(-1);  // keeps the device array non-empty
-    return true;
-}
 
 // Rows of each level followed by -1 up to a multiple of w.
 static void pad_levels(const std::vector<int> &ptr, const std::vector<int> &rows, int w, std::vector<int> &out) {
@@ -597,13 +566,8 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             },
             64);
         phase("off-diagonal slots");
-        // ILU(0) merge map (see cfd_mesh::ilu_mqq)
-        std::vector<int> moff, mqq;
-        if (!build_merge_map(m->h_ia, m->h_ja, m->h_iau, N, moff, mqq, m->ilu_nmap)) {
-            cfd_mesh_destroy(m);
-            return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: ILU(0) merge map too large");
-        }
-        phase("merge map");
+        // (the natural-order ILU(0) merge map is built on first use,
+        // cfd_ensure_ilu_map)
         std::vector<int> lrows, urows;
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, false, m->lower.ptr, lrows, m->lower.maxwidth);
         build_levels(m->h_ia, m->h_ja, m->h_iau, N, true, m->upper.ptr, urows, m->upper.maxwidth);
@@ -681,24 +645,28 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
                 }
             });
             int64_t tot = 0;
-            if (!build_merge_map(mia, mja, miau, N, mmoff, mmqq, tot)) {
+            if (!build_merge_map_par(mia, mja, miau, N, mmoff, mmqq, tot)) {
                 cfd_mesh_destroy(m);
                 return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: ILU(0) merge map too large");
             }
-            build_levels(mia, mja, miau, N, false, m->mc_lower.ptr, mlrows, m->mc_lower.maxwidth);
-            build_levels(mia, mja, miau, N, true, m->mc_upper.ptr, murows, m->mc_upper.maxwidth);
+            build_levels_by_colour(mia, mja, miau, N, false, m->mc_cstart, m->mc_lower.ptr, mlrows,
+                                   m->mc_lower.maxwidth);
+            build_levels_by_colour(mia, mja, miau, N, true, m->mc_cstart, m->mc_upper.ptr, murows,
+                                   m->mc_upper.maxwidth);
             m->mc_lower.nlevels = (int)m->mc_lower.ptr.size() - 1;
             m->mc_upper.nlevels = (int)m->mc_upper.ptr.size() - 1;
         }
         phase("multicolour");
         // neighbour lists (reference order within each list), internal numbering
         std::vector<int> nia(N + 1, 0), nja((size_t)d->n2n_ia[N]);
-        for (int ii = 0; ii < N; ii++) {
-            const int i = order[ii];
-            nia[ii + 1] = nia[ii] + (d->n2n_ia[i + 1] - d->n2n_ia[i]);
-            int k = nia[ii];
-            for (int q = d->n2n_ia[i]; q < d->n2n_ia[i + 1]; q++) nja[k++] = iperm[d->n2n_ja[q]];
-        }
+        for (int ii = 0; ii < N; ii++) nia[ii + 1] = nia[ii] + (d->n2n_ia[order[ii] + 1] - d->n2n_ia[order[ii]]);
+        cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
+            for (int ii = (int)i0; ii < (int)i1; ii++) {
+                const int i = order[ii];
+                int k = nia[ii];
+                for (int q = d->n2n_ia[i]; q < d->n2n_ia[i + 1]; q++) nja[k++] = iperm[d->n2n_ja[q]];
+            }
+        });
 
         // packed node records: q (freestream), coordinates
         HostArray<double> qa((size_t)kQA * m->NT);
@@ -742,8 +710,6 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         if (!rc) m->ia = upload(m, m->h_ia.data(), m->h_ia.size(), &rc);
         if (!rc) m->ja = upload(m, m->h_ja.data(), m->h_ja.size(), &rc);
         if (!rc) m->iau = upload(m, m->h_iau.data(), m->h_iau.size(), &rc);
-        if (!rc) m->ilu_moff = upload(m, moff.data(), moff.size(), &rc);
-        if (!rc) m->ilu_mqq = upload(m, mqq.data(), mqq.size(), &rc);
         if (!rc) m->mc_perm = upload(m, mperm.data(), mperm.size(), &rc);
         if (!rc) m->mc_ia = upload(m, mia.data(), mia.size(), &rc);
         if (!rc) m->mc_ja = upload(m, mja.data(), mja.size(), &rc);
@@ -838,6 +804,25 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
     }
     phase("lsq");
     *out = m;
+    return CFD_OK;
+}
+
+// The natural-order ILU(0) merge map (cfd_mesh::ilu_moff, ilu_mqq), built
+// and uploaded on the first natural-order factorisation: the multicolour
+// factorisation, the sweeps and the Gauss-Seidel solvers do not use it.
+int cfd_ensure_ilu_map(cfd_mesh *m) {
+    if (m->ilu_moff) return CFD_OK;
+    std::vector<int> moff, mqq;
+    int64_t total = 0;
+    if (!build_merge_map_par(m->h_ia, m->h_ja, m->h_iau, m->N, moff, mqq, total))
+        return cfd_set_error(m->ctx, CFD_E_ARG, "ILU(0) merge map too large");
+    int rc = CFD_OK;
+    int *mo = upload(m, moff.data(), moff.size(), &rc);
+    int *mq = rc ? nullptr : upload(m, mqq.data(), mqq.size(), &rc);
+    if (rc) return rc;
+    m->ilu_moff = mo;
+    m->ilu_mqq = mq;
+    m->ilu_nmap = total;
     return CFD_OK;
 }
 

@@ -203,7 +203,7 @@ struct cfd_mesh {
     // ILU(0) merge map: for lower entry kk = (i, k) of the BSR pattern and the
     // t-th upper block of row k, ilu_mqq[ilu_moff[kk] + t] is the slot of the
     // block of row i in the same column, or -1
-    int *ilu_moff = nullptr, *ilu_mqq = nullptr;
+    int *ilu_moff = nullptr, *ilu_mqq = nullptr;  // built on first use (cfd_ensure_ilu_map)
     int64_t ilu_nmap = 0;
     double *a = nullptr, *b = nullptr, *x = nullptr;
     float *af = nullptr;      // FP32 Jacobian (mixed)
@@ -320,6 +320,7 @@ int cfd_ensure_fp32(cfd_mesh *m);
 // incidence lists (once per mesh) and the FP32 copy of their area vectors
 // when fp32 is set.
 int cfd_ensure_patches(cfd_mesh *m, bool fp32);
+int cfd_ensure_ilu_map(cfd_mesh *m);  // natural-order ILU(0) merge map, on first use
 // Whether the residual of these parameters runs the flux-once patch kernel:
 // the fast build at second order, unless CFD_FLUX_ONCE=0 or 1 forces it.
 bool cfd_residual_uses_patches(const cfd_params *p);

@@ -1719,6 +1719,8 @@ int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv, bool half) 
                                (size_t)m->N, nullptr, dinv, SellVals<TM>::d(m));
         }
     } else if (kind == CFD_PREC_ILU0) {
+        const int rc = cfd_ensure_ilu_map(m);
+        if (rc) return rc;
         CFD_HIP(m, hipMemcpyAsync(lu, a, sizeof(TM) * 25 * (size_t)m->nnz, hipMemcpyDeviceToDevice, m->stream));
         for (int l = 0; l < m->lower.nlevels; l++) {
             const int b0 = m->lower.ptr[l], nr = m->lower.ptr[l + 1] - b0;
