@@ -1,0 +1,110 @@
+// Host check of the mesh set-up's parallel ILU(0) merge map and colour-parallel
+// level schedules (cfdsolutions_amd/csrc/setup_levels.h): on random symmetric
+// patterns in multicolour order they must equal one sequential pass over the
+// rows (the definitions restated below).  Built and run by
+// tests/test_host_setup.py; prints one line per trial, exits 1 on a mismatch.
+#include <algorithm>
+#include <cstdio>
+#include <cstdint>
+#include <random>
+#include <thread>
+#include <vector>
+#include <climits>
+template <class F>
+void cfd_parallel_for(int64_t n, F fn, int64_t grain = 1 << 14) {
+    const int64_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nt = (int)std::max<int64_t>(1, std::min(hw, n / std::max<int64_t>(1, grain)));
+    if (nt <= 1) { fn((int64_t)0, n); return; }
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+    fn((int64_t)0, n / nt);
+    for (auto &x : th) x.join();
+}
+// ---- serial (current) ----
+static bool mm_serial(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau, int N,
+                      std::vector<int> &moff, std::vector<int> &mqq, int64_t &total) {
+    moff.assign(ia[N], 0);
+    int64_t tot = 0;
+    for (int i = 0; i < N; i++)
+        for (int kk = ia[i]; kk < iau[i]; kk++) {
+            moff[kk] = (int)tot;
+            const int k = ja[kk];
+            tot += ia[k + 1] - (iau[k] + 1);
+            if (tot > INT32_MAX) return false;
+        }
+    mqq.assign((size_t)tot, -1);
+    for (int i = 0; i < N; i++)
+        for (int kk = ia[i]; kk < iau[i]; kk++) {
+            const int k = ja[kk];
+            int qq = kk + 1;
+            const int qe = ia[i + 1];
+            for (int pp = iau[k] + 1, t2 = 0; pp < ia[k + 1]; pp++, t2++) {
+                while (qq < qe && ja[qq] < ja[pp]) qq++;
+                if (qq < qe && ja[qq] == ja[pp]) mqq[(size_t)moff[kk] + t2] = qq;
+            }
+        }
+    total = tot;
+    if (mqq.empty()) mqq.push_back(-1);
+    return true;
+}
+static void lev_serial(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau, int N,
+                       bool upper, std::vector<int> &ptr, std::vector<int> &rows, int &maxw) {
+    std::vector<int> lev(N, 0);
+    int nl = 0;
+    if (!upper) {
+        for (int i = 0; i < N; i++) { int l = 0; for (int k = ia[i]; k < iau[i]; k++) l = std::max(l, lev[ja[k]] + 1); lev[i] = l; nl = std::max(nl, l + 1); }
+    } else {
+        for (int i = N - 1; i >= 0; i--) { int l = 0; for (int k = iau[i] + 1; k < ia[i + 1]; k++) l = std::max(l, lev[ja[k]] + 1); lev[i] = l; nl = std::max(nl, l + 1); }
+    }
+    ptr.assign(nl + 1, 0);
+    for (int i = 0; i < N; i++) ptr[lev[i] + 1]++;
+    maxw = 0;
+    for (int l = 0; l < nl; l++) { maxw = std::max(maxw, ptr[l + 1]); ptr[l + 1] += ptr[l]; }
+    rows.assign(N, 0);
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < N; i++) rows[pos[lev[i]]++] = i;
+}
+#include "setup_levels.h"
+int main() {
+    std::mt19937 rng(3);
+    for (int trial = 0; trial < 6; trial++) {
+        // random symmetric graph with a greedy colouring order
+        const int N = trial < 3 ? 2000 + 997 * trial : 200000;
+        std::vector<std::vector<int>> nb(N);
+        for (int i = 0; i < N; i++) {
+            int d = 3 + rng() % 8;
+            for (int t = 0; t < d; t++) {
+                int j = (i + 1 + rng() % 300) % N;
+                if (j == i) continue;
+                nb[i].push_back(j); nb[j].push_back(i);
+            }
+        }
+        for (auto &v : nb) { std::sort(v.begin(), v.end()); v.erase(std::unique(v.begin(), v.end()), v.end()); }
+        // greedy colouring, then rows sorted by colour (the multicolour order)
+        std::vector<int> col(N, -1); int nc = 0;
+        for (int i = 0; i < N; i++) { std::vector<char> mk(nc + 2, 0); for (int j : nb[i]) if (col[j] >= 0) mk[col[j]] = 1; int c = 0; while (mk[c]) c++; col[i] = c; nc = std::max(nc, c + 1); }
+        std::vector<int> perm, iperm(N), cstart(nc + 1, 0);
+        for (int c = 0; c < nc; c++) { for (int i = 0; i < N; i++) if (col[i] == c) perm.push_back(i); cstart[c + 1] = (int)perm.size(); }
+        for (int t = 0; t < N; t++) iperm[perm[t]] = t;
+        std::vector<int> ia(N + 1, 0), ja, iau(N);
+        for (int t = 0; t < N; t++) {
+            std::vector<int> r{t}; for (int j : nb[perm[t]]) r.push_back(iperm[j]);
+            std::sort(r.begin(), r.end());
+            for (int x : r) { if (x == t) iau[t] = (int)ja.size(); ja.push_back(x); }
+            ia[t + 1] = (int)ja.size();
+        }
+        std::vector<int> m1, q1, m2, q2; int64_t t1, t2;
+        mm_serial(ia, ja, iau, N, m1, q1, t1);
+        build_merge_map_par(ia, ja, iau, N, m2, q2, t2);
+        bool ok = m1 == m2 && q1 == q2 && t1 == t2;
+        for (int up = 0; up < 2; up++) {
+            std::vector<int> p1, r1, p2, r2; int w1, w2;
+            lev_serial(ia, ja, iau, N, up, p1, r1, w1);
+            build_levels_by_colour(ia, ja, iau, N, up, cstart, p2, r2, w2);
+            ok = ok && p1 == p2 && r1 == r2 && w1 == w2;
+        }
+        printf("trial %d N=%d colours=%d %s\n", trial, N, nc, ok ? "same" : "DIFFERENT");
+        if (!ok) return 1;
+    }
+    return 0;
+}
