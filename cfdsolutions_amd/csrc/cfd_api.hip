@@ -14,6 +14,8 @@
 #include <new>
 #include <vector>
 
+#include <atomic>
+
 #include "cfd_internal.h"
 #include "setup_levels.h"
 
@@ -133,8 +135,8 @@ struct HostArray {
 // Pattern of a block-SELL layout (BlockSell) over rows 0..n-1: row i holds
 // the BSR slots rb(i) .. re(i)-1 in that order, with block columns ja[slot].
 template <class RB, class RE>
-static void build_block_sell(int n, RB rb, RE re, const std::vector<int> &ja, std::vector<int> &off,
-                             std::vector<int> &col, std::vector<int> &src) {
+static void build_block_sell(int n, RB rb, RE re, const int *ja, std::vector<int> &off, HostArray<int> &col,
+                             HostArray<int> &src) {
     const int ns = (n + 63) / 64;
     off.assign(ns + 1, 0);
     for (int sl = 0; sl < ns; sl++) {
@@ -142,21 +144,28 @@ static void build_block_sell(int n, RB rb, RE re, const std::vector<int> &ja, st
         for (int i = 64 * sl; i < std::min(n, 64 * sl + 64); i++) w = std::max(w, re(i) - rb(i));
         off[sl + 1] = off[sl] + 64 * w;
     }
-    col.assign(off[ns], 0);
-    src.assign(off[ns], -1);
-    cfd_parallel_for(n, [&](int64_t i0, int64_t i1) {
-        for (int i = (int)i0; i < (int)i1; i++) {
-            size_t q = (size_t)off[i / 64] + i % 64;
-            for (int kk = rb(i); kk < re(i); kk++, q += 64) {
-                col[q] = ja[kk];
-                src[q] = kk;
+    col = HostArray<int>((size_t)off[ns]);
+    src = HostArray<int>((size_t)off[ns]);
+    // every slot of a slice written by the thread that owns the slice; padding
+    // slots get column 0 and source -1
+    cfd_parallel_for(ns, [&](int64_t s0, int64_t s1) {
+        for (int sl = (int)s0; sl < (int)s1; sl++) {
+            const int w = (off[sl + 1] - off[sl]) / 64;
+            for (int l = 0; l < 64; l++) {
+                const int i = 64 * sl + l;
+                const int len = i < n ? re(i) - rb(i) : 0;
+                for (int k = 0; k < w; k++) {
+                    const size_t q = (size_t)off[sl] + 64 * (size_t)k + l;
+                    col[q] = k < len ? ja[rb(i) + k] : 0;
+                    src[q] = k < len ? rb(i) + k : -1;
+                }
             }
         }
-    });
+    }, 1 << 8);
 }
 
-static int upload_block_sell(cfd_mesh *m, const std::vector<int> &off, const std::vector<int> &col,
-                             const std::vector<int> &src, BlockSell &bs);
+static int upload_block_sell(cfd_mesh *m, const std::vector<int> &off, const HostArray<int> &col,
+                             const HostArray<int> &src, BlockSell &bs);
 
 // Rows of each level followed by -1 up to a multiple of w.
 static void pad_levels(const std::vector<int> &ptr, const std::vector<int> &rows, int w, std::vector<int> &out) {
@@ -212,13 +221,13 @@ static T *upload(cfd_mesh *m, const T *h, size_t n, int *rc) {
     return d;
 }
 
-static int upload_block_sell(cfd_mesh *m, const std::vector<int> &off, const std::vector<int> &col,
-                             const std::vector<int> &src, BlockSell &bs) {
+static int upload_block_sell(cfd_mesh *m, const std::vector<int> &off, const HostArray<int> &col,
+                             const HostArray<int> &src, BlockSell &bs) {
     int rc = 0;
     bs.nblk = (int64_t)col.size();
     bs.off = upload(m, off.data(), off.size(), &rc);
-    if (!rc && !col.empty()) bs.col = upload(m, col.data(), col.size(), &rc);
-    if (!rc && !src.empty()) bs.src = upload(m, src.data(), src.size(), &rc);
+    if (!rc && col.size()) bs.col = upload(m, col.data(), col.size(), &rc);
+    if (!rc && src.size()) bs.src = upload(m, src.data(), src.size(), &rc);
     return rc;
 }
 
@@ -576,7 +585,10 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         phase("levels");
         // multicolour ordering for ILU(0): greedy colouring of the node graph in
         // reference order; rows sorted by (colour, index)
-        std::vector<int> mperm(N), mia(N + 1, 0), mja(m->nnz), miau(N), msrc(m->nnz), mmoff, mmqq, mlrows, murows;
+        std::vector<int> mperm(N), mia(N + 1, 0), miau(N), mlrows, murows;
+        HostArray<int> mja(m->nnz), msrc(m->nnz);  // filled in full below
+        std::unique_ptr<int[]> mmoff, mmqq;
+        int64_t mc_nmap = 0;
         {
             // Greedy colouring in reference order.  The cases of a union hold
             // ascending node ranges with no edge between them, so each range
@@ -589,12 +601,22 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
             }
             if (!ranges) cut.assign(1, 0);
             cut.push_back(N);
-            for (int r = 0; r + 1 < (int)cut.size() && cut.size() > 2; r++)  // no edge may leave its range
-                for (int q = d->n2n_ia[cut[r]]; q < d->n2n_ia[cut[r + 1]]; q++)
-                    if (d->n2n_ja[q] < cut[r] || d->n2n_ja[q] >= cut[r + 1]) {
-                        cut.assign({0, N});
-                        break;
-                    }
+            if (cut.size() > 2) {  // no edge may leave its range (checked range by range, in parallel)
+                std::atomic<bool> leaves{false};
+                cfd_parallel_for(
+                    (int64_t)cut.size() - 1,
+                    [&](int64_t r0, int64_t r1) {
+                        for (int64_t r = r0; r < r1 && !leaves.load(std::memory_order_relaxed); r++)
+                            for (int q = d->n2n_ia[cut[r]]; q < d->n2n_ia[cut[r + 1]]; q++)
+                                if (d->n2n_ja[q] < cut[r] || d->n2n_ja[q] >= cut[r + 1]) {
+                                    leaves = true;
+                                    break;
+                                }
+                    },
+                    1);
+                if (leaves) cut.assign({0, N});
+            }
+            phase("multicolour: ranges");
             std::vector<int> ncs(cut.size() - 1, 0);
             cfd_parallel_for(
                 (int64_t)cut.size() - 1,
@@ -617,6 +639,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
                     }
                 },
                 1);
+            phase("multicolour: colouring");
             int nc = 1;
             for (int v : ncs) nc = std::max(nc, v);
             m->ncolor = nc;
@@ -644,19 +667,20 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
                     }
                 }
             });
-            int64_t tot = 0;
-            if (!build_merge_map_par(mia, mja, miau, N, mmoff, mmqq, tot)) {
+            phase("multicolour: pattern");
+            if (!build_merge_map_par(mia.data(), mja.data(), miau.data(), N, mmoff, mmqq, mc_nmap)) {
                 cfd_mesh_destroy(m);
                 return cfd_set_error(ctx, CFD_E_ARG, "cfd_mesh_create: ILU(0) merge map too large");
             }
-            build_levels_by_colour(mia, mja, miau, N, false, m->mc_cstart, m->mc_lower.ptr, mlrows,
+            phase("multicolour: merge map");
+            build_levels_by_colour(mia.data(), mja.data(), miau.data(), N, false, m->mc_cstart, m->mc_lower.ptr, mlrows,
                                    m->mc_lower.maxwidth);
-            build_levels_by_colour(mia, mja, miau, N, true, m->mc_cstart, m->mc_upper.ptr, murows,
+            build_levels_by_colour(mia.data(), mja.data(), miau.data(), N, true, m->mc_cstart, m->mc_upper.ptr, murows,
                                    m->mc_upper.maxwidth);
             m->mc_lower.nlevels = (int)m->mc_lower.ptr.size() - 1;
             m->mc_upper.nlevels = (int)m->mc_upper.ptr.size() - 1;
         }
-        phase("multicolour");
+        phase("multicolour: levels");
         // neighbour lists (reference order within each list), internal numbering
         std::vector<int> nia(N + 1, 0), nja((size_t)d->n2n_ia[N]);
         for (int ii = 0; ii < N; ii++) nia[ii + 1] = nia[ii] + (d->n2n_ia[order[ii] + 1] - d->n2n_ia[order[ii]]);
@@ -715,21 +739,23 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         if (!rc) m->mc_ja = upload(m, mja.data(), mja.size(), &rc);
         if (!rc) m->mc_iau = upload(m, miau.data(), miau.size(), &rc);
         if (!rc) m->mc_src = upload(m, msrc.data(), msrc.size(), &rc);
-        if (!rc) m->mc_moff = upload(m, mmoff.data(), mmoff.size(), &rc);
-        if (!rc) m->mc_mqq = upload(m, mmqq.data(), mmqq.size(), &rc);
+        if (!rc) m->mc_moff = upload(m, mmoff.get(), (size_t)m->nnz, &rc);
+        if (!rc) m->mc_mqq = upload(m, mmqq.get(), (size_t)std::max<int64_t>(mc_nmap, 1), &rc);
         phase("uploads");
         if (!rc) {  // block-SELL patterns of A and of the multicolour factors
-            std::vector<int> off, col, src;
+            std::vector<int> off;
+            HostArray<int> col(0), src(0);
             const std::vector<int> &hia = m->h_ia;
-            build_block_sell(N, [&](int i) { return hia[i]; }, [&](int i) { return hia[i + 1]; }, m->h_ja, off, col,
+            build_block_sell(N, [&](int i) { return hia[i]; }, [&](int i) { return hia[i + 1]; }, m->h_ja.data(), off, col,
                              src);
             rc = upload_block_sell(m, off, col, src, m->a_sell);
             if (!rc) {
-                build_block_sell(N, [&](int t) { return mia[t]; }, [&](int t) { return miau[t]; }, mja, off, col, src);
+                build_block_sell(N, [&](int t) { return mia[t]; }, [&](int t) { return miau[t]; }, mja.data(), off, col,
+                                 src);
                 rc = upload_block_sell(m, off, col, src, m->mcl_sell);
             }
             if (!rc) {
-                build_block_sell(N, [&](int t) { return miau[t] + 1; }, [&](int t) { return mia[t + 1]; }, mja, off,
+                build_block_sell(N, [&](int t) { return miau[t] + 1; }, [&](int t) { return mia[t + 1]; }, mja.data(), off,
                                  col, src);
                 rc = upload_block_sell(m, off, col, src, m->mcu_sell);
             }
@@ -812,13 +838,13 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
 // factorisation, the sweeps and the Gauss-Seidel solvers do not use it.
 int cfd_ensure_ilu_map(cfd_mesh *m) {
     if (m->ilu_moff) return CFD_OK;
-    std::vector<int> moff, mqq;
+    std::unique_ptr<int[]> moff, mqq;
     int64_t total = 0;
-    if (!build_merge_map_par(m->h_ia, m->h_ja, m->h_iau, m->N, moff, mqq, total))
+    if (!build_merge_map_par(m->h_ia.data(), m->h_ja.data(), m->h_iau.data(), m->N, moff, mqq, total))
         return cfd_set_error(m->ctx, CFD_E_ARG, "ILU(0) merge map too large");
     int rc = CFD_OK;
-    int *mo = upload(m, moff.data(), moff.size(), &rc);
-    int *mq = rc ? nullptr : upload(m, mqq.data(), mqq.size(), &rc);
+    int *mo = upload(m, moff.get(), (size_t)m->nnz, &rc);
+    int *mq = rc ? nullptr : upload(m, mqq.get(), (size_t)std::max<int64_t>(total, 1), &rc);
     if (rc) return rc;
     m->ilu_moff = mo;
     m->ilu_mqq = mq;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 // ILU(0) merge map of a BSR pattern (ia, ja sorted per row, iau the diagonal
@@ -13,11 +14,12 @@
 // the map entries of the upper blocks (k, j) of row k, and mqq[moff[kk] + t]
 // is the slot of block (i, j) in row i for the t-th of them, or -1 when (i, j)
 // is outside the pattern (ILU(0) drops it).  Entries are numbered row by row,
-// lower block by lower block, as a sequential pass would number them.
+// lower block by lower block, as a sequential pass would number them; moff is
+// 0 at the other slots.  Every entry of both arrays is written once, by the
+// thread that owns its row (no separate initialisation pass over them).
 // Returns false when there are more than INT32_MAX entries.
-inline bool build_merge_map_par(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau,
-                                int N, std::vector<int> &moff, std::vector<int> &mqq, int64_t &total) {
-    moff.assign(ia[N], 0);
+inline bool build_merge_map_par(const int *ia, const int *ja, const int *iau, int N, std::unique_ptr<int[]> &moff,
+                                std::unique_ptr<int[]> &mqq, int64_t &total) {
     auto count = [&](int kk) {
         const int k = ja[kk];
         return (int64_t)(ia[k + 1] - (iau[k] + 1));
@@ -33,30 +35,26 @@ inline bool build_merge_map_par(const std::vector<int> &ia, const std::vector<in
     for (int i = 0; i < N; i++) start[i + 1] += start[i];
     const int64_t tot = start[N];
     if (tot > INT32_MAX) return false;
+    moff.reset(new int[(size_t)ia[N] + 1]);
+    mqq.reset(new int[(size_t)std::max<int64_t>(tot, 1)]);
+    mqq[0] = -1;  // keeps the array non-empty when there are no entries
     cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
         for (int i = (int)i0; i < (int)i1; i++) {
             int64_t t = start[i];
             for (int kk = ia[i]; kk < iau[i]; kk++) {
                 moff[kk] = (int)t;
-                t += count(kk);
-            }
-        }
-    });
-    mqq.assign((size_t)tot, -1);
-    cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
-        for (int i = (int)i0; i < (int)i1; i++)
-            for (int kk = ia[i]; kk < iau[i]; kk++) {
                 const int k = ja[kk];
                 int qq = kk + 1;
                 const int qe = ia[i + 1];
-                for (int pp = iau[k] + 1, t2 = 0; pp < ia[k + 1]; pp++, t2++) {
+                for (int pp = iau[k] + 1; pp < ia[k + 1]; pp++, t++) {
                     while (qq < qe && ja[qq] < ja[pp]) qq++;
-                    if (qq < qe && ja[qq] == ja[pp]) mqq[(size_t)moff[kk] + t2] = qq;
+                    mqq[(size_t)t] = qq < qe && ja[qq] == ja[pp] ? qq : -1;
                 }
             }
+            for (int kk = iau[i]; kk < ia[i + 1]; kk++) moff[kk] = 0;
+        }
     });
     total = tot;
-    if (mqq.empty()) mqq.push_back(-1);  // keeps the device array non-empty
     return true;
 }
 
@@ -67,9 +65,9 @@ inline bool build_merge_map_par(const std::vector<int> &ia, const std::vector<in
 // in reverse), the rows of each in parallel.  level(i) = 1 + the largest
 // level of the rows it depends on (0 for none); ptr[l]..ptr[l+1] index the
 // rows of level l in rows, ascending within a level; maxw is the widest level.
-inline void build_levels_by_colour(const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<int> &iau,
-                                   int N, bool upper, const std::vector<int> &cstart, std::vector<int> &ptr,
-                                   std::vector<int> &rows, int &maxw) {
+inline void build_levels_by_colour(const int *ia, const int *ja, const int *iau, int N, bool upper,
+                                   const std::vector<int> &cstart, std::vector<int> &ptr, std::vector<int> &rows,
+                                   int &maxw) {
     std::vector<int> lev((size_t)N, 0);
     const int nc = (int)cstart.size() - 1;
     for (int q = 0; q < nc; q++) {

@@ -10,6 +10,7 @@
 #include <thread>
 #include <vector>
 #include <climits>
+#include <memory>
 template <class F>
 void cfd_parallel_for(int64_t n, F fn, int64_t grain = 1 << 14) {
     const int64_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -93,14 +94,16 @@ int main() {
             for (int x : r) { if (x == t) iau[t] = (int)ja.size(); ja.push_back(x); }
             ia[t + 1] = (int)ja.size();
         }
-        std::vector<int> m1, q1, m2, q2; int64_t t1, t2;
+        std::vector<int> m1, q1; int64_t t1, t2;
+        std::unique_ptr<int[]> m2, q2;
         mm_serial(ia, ja, iau, N, m1, q1, t1);
-        build_merge_map_par(ia, ja, iau, N, m2, q2, t2);
-        bool ok = m1 == m2 && q1 == q2 && t1 == t2;
+        build_merge_map_par(ia.data(), ja.data(), iau.data(), N, m2, q2, t2);
+        bool ok = t1 == t2 && std::equal(m1.begin(), m1.end(), m2.get()) &&
+                  std::equal(q1.begin(), q1.end(), q2.get());
         for (int up = 0; up < 2; up++) {
             std::vector<int> p1, r1, p2, r2; int w1, w2;
             lev_serial(ia, ja, iau, N, up, p1, r1, w1);
-            build_levels_by_colour(ia, ja, iau, N, up, cstart, p2, r2, w2);
+            build_levels_by_colour(ia.data(), ja.data(), iau.data(), N, up, cstart, p2, r2, w2);
             ok = ok && p1 == p2 && r1 == r2 && w1 == w2;
         }
         printf("trial %d N=%d colours=%d %s\n", trial, N, nc, ok ? "same" : "DIFFERENT");
