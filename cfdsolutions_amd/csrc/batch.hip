@@ -204,9 +204,11 @@ int build_union(cfd_ctx *ctx, const std::vector<int> &cases, const cfd_mesh_desc
         boff[c + 1] = boff[c] + d.n_bedge;
     }
     const int N = noff[K], E = eoff[K], B = boff[K];
-    std::vector<int> edge(2 * (size_t)E), bedge(2 * (size_t)B), btype(B), btag(B), ia(N + 1, 0), ja(2 * (size_t)E),
-        node_case(N), bcase(B);
-    std::vector<double> earea(3 * (size_t)E), barea(3 * (size_t)B), xyz(3 * (size_t)N), vol(N);
+    // every entry is written below, case by case in parallel
+    HostArray<int> edge(2 * (size_t)E), bedge(2 * (size_t)B), btype(B), btag(B), ia((size_t)N + 1),
+        ja(2 * (size_t)E), node_case(N), bcase(B);
+    HostArray<double> earea(3 * (size_t)E), barea(3 * (size_t)B), xyz(3 * (size_t)N), vol(N);
+    ia[0] = 0;
     // each case fills its own ranges; its n2n lists start at entry 2 * eoff[c]
     cfd_parallel_for(
         K,
@@ -303,8 +305,8 @@ int build_union(cfd_ctx *ctx, const std::vector<int> &cases, const cfd_mesh_desc
                                 (size_t)m->seg_forces.nblocks * 4});
     m->bpart = (double *)cfd_dalloc(m, sizeof(double) * np);
     if (!m->bpart) return cfd_set_error(ctx, CFD_E_OOM, "cfd_batch_solve: out of device memory");
-    // initial state of every case, nodes and ghosts
-    std::vector<double> q(5 * (size_t)(N + B));
+    // initial state of every case, nodes and ghosts (every entry written below)
+    HostArray<double> q(5 * (size_t)(N + B));
     for (int c = 0; c < K; c++) {
         const size_t nc = (size_t)(noff[c + 1] - noff[c]), bc = (size_t)(boff[c + 1] - boff[c]), tc = nc + bc;
         const std::vector<double> &qi = q_init[cases[c]];

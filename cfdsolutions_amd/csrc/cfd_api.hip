@@ -118,19 +118,6 @@ extern "C" const char *cfd_last_error(const cfd_ctx *ctx) {
 // ---------------------------------------------------------------------------
 // mesh creation
 // ---------------------------------------------------------------------------
-// Host array without value initialisation, for large set-up arrays that a
-// parallel pass writes in full.
-template <class T>
-struct HostArray {
-    std::unique_ptr<T[]> p;
-    size_t n;
-    explicit HostArray(size_t count) : p(new T[count ? count : 1]), n(count) {}
-    T *data() { return p.get(); }
-    const T *data() const { return p.get(); }
-    size_t size() const { return n; }
-    T &operator[](size_t i) { return p[i]; }
-    const T &operator[](size_t i) const { return p[i]; }
-};
 
 // Pattern of a block-SELL layout (BlockSell) over rows 0..n-1: row i holds
 // the BSR slots rb(i) .. re(i)-1 in that order, with block columns ja[slot].
@@ -406,7 +393,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         });
         for (int b = 0; b < B; b++) ip[iperm[d->bedge[2 * b]] + 1]++;
         for (int i = 0; i < N; i++) ip[i + 1] += ip[i];
-        std::vector<int2> inc(ip[N]);
+        HostArray<int2> inc(ip[N]);  // every entry written below (checked by the list ends)
         {
             // Edges and boundary edges in order, each appended to the lists of
             // its nodes.  A union's cases hold ascending ranges of nodes, edges
@@ -535,13 +522,16 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         });
         phase("bsr pattern");
         // BSR block of the off-diagonal (row of the node, column of the neighbour)
-        std::vector<int> off(inc.size(), -1);
+        HostArray<int> off(inc.size());  // every entry written below (unless a neighbour is missing)
         std::vector<int> missing(N, -1);  // per node: a neighbour missing from its n2n list
         cfd_parallel_for(N, [&](int64_t i0, int64_t i1) {
             for (int ii = (int)i0; ii < (int)i1; ii++) {
                 const int i = order[ii];
                 for (int k = ip[ii]; k < ip[ii + 1]; k++) {
-                    if (inc[k].x >= N) continue;
+                    if (inc[k].x >= N) {
+                        off[k] = -1;
+                        continue;
+                    }
                     const int j = order[inc[k].x];
                     auto b0 = m->h_ja.begin() + m->h_ia[i], b1 = m->h_ja.begin() + m->h_ia[i + 1];
                     auto it = std::lower_bound(b0, b1, j);

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <shared_mutex>
 #include <string>
@@ -124,6 +125,20 @@ void cfd_parallel_for(int64_t n, F fn, int64_t grain = 1 << 14) {
     fn((int64_t)0, n / nt);
     for (auto &x : th) x.join();
 }
+
+// Host array without value initialisation, for large set-up arrays that a
+// parallel pass writes in full.
+template <class T>
+struct HostArray {
+    std::unique_ptr<T[]> p;
+    size_t n;
+    explicit HostArray(size_t count) : p(new T[count ? count : 1]), n(count) {}
+    T *data() { return p.get(); }
+    const T *data() const { return p.get(); }
+    size_t size() const { return n; }
+    T &operator[](size_t i) { return p[i]; }
+    const T &operator[](size_t i) const { return p[i]; }
+};
 
 // CFD_PROGRESS=1: the Newton loops of cfd_solve_steady and cfd_batch_solve
 // print one line per iteration to stderr (long solves stay observable).
