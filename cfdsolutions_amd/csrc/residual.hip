@@ -295,8 +295,8 @@ __device__ __forceinline__ bool near_vacuum(const T *Q, T P) {
 // multiplication, so a node divides once per variable (limiter_finish)
 // instead of once per neighbour; the exact build divides per neighbour as the
 // reference does and leaves pd alone.
-// F0 (fast build only): the reconstruction fraction is 0, so the face value
-// is qc + grad . R and the neighbour's state is not read.
+// F0: the reconstruction fraction is 0, so the face value is qc + grad . R
+// and the neighbour's state is not read.
 template <class T, int KIND, bool F0 = false>  // KIND 1 Venkatakrishnan, 2 Barth-Jespersen
 __device__ __forceinline__ void limiter_nbr(const T *qj, T xj, T yj, T zj, const T *qc, T xc, T yc, T zc, const T *gr,
                                             const T *qmin, const T *qmax, T eps2, T frac, T *ph, T *pd) {
@@ -1459,12 +1459,11 @@ int launch_gradient_limiter(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p
     const bool stage = env && (env[0] == '0' || env[0] == '1') ? env[0] == '1'
                        : tuned >= 0                            ? tuned == 1
                                                                : sizeof(T) == 4;
-    // the fast build drops the fraction's terms when it is 0
-#if CFD_FAST_MATH
+    // with the fraction at 0 its terms drop out: 0 (q_j - q_c) is a zero, and
+    // q_c + 0 + 1 g.R equals q_c + g.R, so the exact build keeps its bits
+    // (for finite states; zeros may differ only in sign where every use
+    // compares them with zero or squares them)
     const bool f0 = frac == T(0.0);
-#else
-    const bool f0 = false;
-#endif
 #define CFD_GL_LAUNCH(KIND, STAGE, F0)                                                                             \
     hipLaunchKernelGGL((k_gg_limiter<T, KIND, STAGE, F0>), dim3(nb), dim3(kBlock), 0, m->stream, m->N, m->inc_ptr, \
                        m->sptr, m->nbr, b.isx, b.isy, b.isz, b.vol, b.qa, b.qb, g, frac, k3, m->risk)
