@@ -669,6 +669,21 @@ def main():
                               l_gbs=by["L"] / tl / 1e6 if tl > 0 else None, bytes=total)
         print("bench: %-22s %8.3f ms/step %8.1f GB/s  (G %.3f L %.3f R %.3f ms)" %
               (name, wall, total / wall / 1e6, tg, tl, tr), file=sys.stderr, flush=True)
+    # the exact build with the flux-once patch kernel (work edges in the reference's edge order: the same bits as
+    # the node-centred kernel; the exact build runs it when CFD_FLUX_ONCE=1)
+    once_name = "roe-fp64-GL+R2-flux-once"
+    os.environ["CFD_FLUX_ONCE"] = "1"
+    try:
+        pv = cfd.make_params(mach=0.8, flux="roe", order=2, limiter=cfd.LIM_VENKAT, venkat_k=5.0,
+                             gradient=cfd.GRAD_GG, arith=EX)
+        wall, tg, tl, tr, tmed = time_variant(dm, ev, pv, max(10, args.steps // 4), 5, True, True)
+    finally:
+        del os.environ["CFD_FLUX_ONCE"]
+    total = bytes64["G"] + bytes64["L"] + bytes64["R2split"]
+    variants[once_name] = dict(ms_per_step=wall, median_step_ms=tmed, gbs=total / wall / 1e6, g_ms=tg, l_ms=tl,
+                               r_ms=tr, r_gbs=bytes64["R2split"] / tr / 1e6, bytes=total)
+    print("bench: %-22s %8.3f ms/step %8.1f GB/s  (G %.3f L %.3f R %.3f ms)" %
+          (once_name, wall, total / wall / 1e6, tg, tl, tr), file=sys.stderr, flush=True)
     head_name, exact_name = "roe-fp64fast-GL+R2", "roe-fp64-GL+R2"
     head = variants[head_name]
     # parity spot check of both builds of the measured configuration against the oracle: the exact build to the
@@ -676,31 +691,38 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and not args.only:
         gpu = {}
-        for arith in (FA, EX):
+        for arith, once in ((FA, None), (EX, None), (EX, "1")):
             pv = cfd.make_params(mach=0.8, flux="roe", order=2, limiter=cfd.LIM_VENKAT, venkat_k=5.0,
                                  gradient=cfd.GRAD_GG, arith=arith)
-            dm.gradient_limiter(pv)
-            dm.residual(pv)
-            gpu[arith] = (dm.get(cfd.FIELD_RES_CONV), dm.get(cfd.FIELD_RES_DISS))
+            if once:
+                os.environ["CFD_FLUX_ONCE"] = once
+            try:
+                dm.gradient_limiter(pv)
+                dm.residual(pv)
+            finally:
+                os.environ.pop("CFD_FLUX_ONCE", None)
+            gpu[(arith, once)] = (dm.get(cfd.FIELD_RES_CONV), dm.get(cfd.FIELD_RES_DISS))
 
-        def rel_to(rc, rd, arith):
-            gc, gd = gpu[arith]
+        def rel_to(rc, rd, arith, once=None):
+            gc, gd = gpu[(arith, once)]
             return float(max(np.max(np.abs(gc - rc)) / np.max(np.abs(rc)),
                              np.max(np.abs(gd - rd)) / np.max(np.abs(rd))))
 
         secs, rc, rd = cpu_baseline(m, pv, q)
-        rel, rel_ex = rel_to(rc, rd, FA), rel_to(rc, rd, EX)
+        rel, rel_ex, rel_once = rel_to(rc, rd, FA), rel_to(rc, rd, EX), rel_to(rc, rd, EX, "1")
         secs_f, rc, rd = cpu_baseline(m, pv, q, build="speed")
         rel_f = rel_to(rc, rd, FA)
         cpu = dict(value=head["bytes"] / secs / 1e9, unit="GB/s", cores=1, kind="port",
                    sample="one full G+L+R2 Roe FP64 evaluation of the same 160^3 mesh and state (%.1f s)" % secs,
                    build="oracle.c -O2 -ffp-contract=off (parity build)",
                    parity_max_rel_diff=rel, parity_tolerance=1e-12, exact_build_max_rel_diff=rel_ex,
+                   exact_flux_once_max_rel_diff=rel_once,
                    speed_build=dict(value=head["bytes"] / secs_f / 1e9, seconds=secs_f,
                                     build="oracle.c -O3 -march=x86-64-v3", max_rel_diff_vs_gpu=rel_f))
-        print("bench: cpu oracle %.2f s (speed build %.2f s), GPU-vs-oracle max rel diff: fast %.3e, exact %.3e "
-              "(speed build vs fast %.1e)" % (secs, secs_f, rel, rel_ex, rel_f), file=sys.stderr, flush=True)
-        if rel > 1e-12 or rel_ex != 0.0:
+        print("bench: cpu oracle %.2f s (speed build %.2f s), GPU-vs-oracle max rel diff: fast %.3e, exact %.3e, "
+              "exact flux-once %.3e (speed build vs fast %.1e)" % (secs, secs_f, rel, rel_ex, rel_once, rel_f),
+              file=sys.stderr, flush=True)
+        if rel > 1e-12 or rel_ex != 0.0 or rel_once != 0.0:
             print("bench: PARITY FAILURE of the measured configuration", file=sys.stderr, flush=True)
             sys.exit(1)
     dm.close()
@@ -745,7 +767,12 @@ def main():
                    "solve_autotune": bool(args.autotune)},
         "exact_build": {"value": round(exact["gbs"], 2), "ms_per_step": round(exact["ms_per_step"], 4),
                         "residual_kernel": "k_residual<double,Roe,2nd order,limited>",
-                        "residual_gbs": round(exact["r_gbs"], 1), "residual_ms": round(exact["r_ms"], 4)}
+                        "residual_gbs": round(exact["r_gbs"], 1), "residual_ms": round(exact["r_ms"], 4),
+                        "flux_once": {"value": round(variants[once_name]["gbs"], 2),
+                                      "ms_per_step": round(variants[once_name]["ms_per_step"], 4),
+                                      "residual_kernel": "k_residual_patch<double,Roe,2nd order,limited> "
+                                                         "(work edges in the reference's edge order, CFD_FLUX_ONCE=1)",
+                                      "residual_ms": round(variants[once_name]["r_ms"], 4)}}
         if exact else None,
         "roofline": {"bound": "hbm", "kernel": "k_residual_patch<double,Roe,2nd order,limited> (fast build)",
                      "achieved": round(r_achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -467,6 +467,7 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
         // once (padding included), in parallel, instead of a serial zero fill
         HostArray<int> nbr(m->nsell);
         HostArray<double> isx(m->nsell), isy(m->nsell), isz(m->nsell);
+        m->h_sell_edge = HostArray<int>(m->nsell);  // every entry written below
         cfd_parallel_for(
             nsl,
             [&](int64_t s0, int64_t s1) {
@@ -480,9 +481,11 @@ int cfd_mesh_create_cases(cfd_ctx *ctx, const cfd_mesh_desc *d, const cfd_params
                             if (k >= cnt) {  // padding, never read
                                 nbr[idx] = 0;
                                 isx[idx] = isy[idx] = isz[idx] = 0.0;
+                                m->h_sell_edge[idx] = -1;
                                 continue;
                             }
                             const int2 en = inc[ip[i] + k];
+                            m->h_sell_edge[idx] = en.y >= 0 ? en.y : ~en.y;  // boundary: E + b
                             const double *S;
                             if (en.x >= N) {
                                 nbr[idx] = en.x;
@@ -914,22 +917,30 @@ int cfd_ensure_fp32(cfd_mesh *m) {
 }
 
 // ---------------------------------------------------------------------------
-// Work-edge patches of the flux-once residual (CFD_ARITH_FAST).  Built from
-// the device incidence lists: a node's incidence with neighbour j is an
-// interior edge (node is L when nbr >= 0, R when nbr = ~j) or boundary edge b
-// (nbr = N + b), with the edge's area vector.  An edge inside a patch becomes
-// one work edge (taken from its L end); an edge between patches becomes a
-// work edge of each; a boundary edge one of its node's patch.  Work edges of
-// a patch are ordered by (first end in the patch, L, R), and each node lists
-// the slots of its incidences in ascending order (sign bit: the node is R).
-// Patches hold kPatch consecutive nodes, restarting at every case of a union.
+// Work-edge patches of the flux-once residual.  Built from the device
+// incidence lists: a node's incidence with neighbour j is an interior edge
+// (node is L when nbr >= 0, R when nbr = ~j) or boundary edge b (nbr = N + b),
+// with the edge's area vector.  An edge inside a patch becomes one work edge
+// (taken from its L end); an edge between patches becomes a work edge of
+// each; a boundary edge one of its node's patch.  Work edges of a patch are
+// ordered by reference edge index (boundary edges after the interior ones),
+// and each node lists the slots of its incidences in ascending order (sign
+// bit: the node is R), which is the order of its incidence list: each node
+// adds its fluxes in the reference's order.  Patches hold kPatch consecutive
+// nodes, restarting at every case of a union.
 // ---------------------------------------------------------------------------
 constexpr int kPatch = 256;  // nodes per patch: one per thread of the patch kernel's workgroup
 
+// The fast build runs the flux-once patch kernel at second order; the exact
+// build runs the node-centred kernel, since its patches (whose work edges go
+// in the reference's edge order, giving the same bits) cost a host build of
+// about 0.15 s per million nodes, more than a steady solve's residual
+// evaluations save.  CFD_FLUX_ONCE=0/1 (read at each call) forces either
+// kernel in both builds.
 bool cfd_residual_uses_patches(const cfd_params *p) {
-    if (p->arith != CFD_ARITH_FAST) return false;
-    static const char *const once = getenv("CFD_FLUX_ONCE");
-    return once && (once[0] == '0' || once[0] == '1') ? once[0] == '1' : p->order == 2;
+    const char *const once = getenv("CFD_FLUX_ONCE");
+    if (once && (once[0] == '0' || once[0] == '1')) return once[0] == '1';
+    return p->arith == CFD_ARITH_FAST && p->order == 2;
 }
 
 int64_t cfd_patch_footprint_bytes(int64_t n, int64_t e, int64_t b, int64_t cases) {
@@ -939,8 +950,10 @@ int64_t cfd_patch_footprint_bytes(int64_t n, int64_t e, int64_t b, int64_t cases
     return inc * (4 + 4 + 24 + 12) + 4 * nsell + 8 * (n / kPatch + cases + 2);
 }
 
-int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
-    if (!m->pw_ptr) {
+int cfd_ensure_patches(cfd_mesh *m, bool fp32, int order) {
+    PatchSet &ps = m->patch[order];
+    if (!ps.pw_ptr) {
+        const auto t0 = std::chrono::steady_clock::now();
         const int N = m->N, nsl = (N + kSlice - 1) / kSlice;
         // patches of kPatch nodes from the start of the mesh, or of each case
         // of a union (so a case's patches, and its sums, are those it has alone)
@@ -951,8 +964,9 @@ int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
             for (int n = cut[c]; n < cut[c + 1]; n += kPatch) pstart.push_back(n);
         pstart.push_back(N);
         const int P = (int)pstart.size() - 1;
-        std::vector<int> ip(N + 1), sptr(nsl + 1), nbr(m->nsell);
-        std::vector<double> sx(m->nsell), sy(m->nsell), sz(m->nsell);
+        std::vector<int> ip(N + 1), sptr(nsl + 1);
+        HostArray<int> nbr(m->nsell);  // (downloaded in full)
+        HostArray<double> sx(m->nsell), sy(m->nsell), sz(m->nsell);
         CFD_HIP(m, hipStreamSynchronize(m->stream));
         CFD_HIP(m, cfd_memcpy(ip.data(), m->inc_ptr, sizeof(int) * ip.size(), hipMemcpyDeviceToHost));
         CFD_HIP(m, cfd_memcpy(sptr.data(), m->sptr, sizeof(int) * sptr.size(), hipMemcpyDeviceToHost));
@@ -961,7 +975,12 @@ int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
         CFD_HIP(m, cfd_memcpy(sy.data(), m->d.isy, sizeof(double) * sy.size(), hipMemcpyDeviceToHost));
         CFD_HIP(m, cfd_memcpy(sz.data(), m->d.isz, sizeof(double) * sz.size(), hipMemcpyDeviceToHost));
         auto sell = [&](int i, int k) { return (size_t)sptr[i / kSlice] + (size_t)kSlice * k + i % kSlice; };
-        // work edges of each patch: (key, l, r, incidence of the area vector)
+        // work edges of each patch: (key, l, r, incidence of the area vector);
+        // key: the first end in the patch (node order) or the reference edge
+        // index (edge order: interior edges by index, then boundary edges by
+        // index, the order of each node's incidence list, along which its
+        // slots then ascend)
+        const bool by_edge = order == kPatchEdgeOrder;
         struct Work {
             int key, l, r;
             size_t inc;
@@ -978,17 +997,18 @@ int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
                             const size_t x = sell(i, k);
                             const int jr = nbr[x];
                             if (jr >= N) {
-                                w.push_back({i - n0, i, jr, x});
+                                w.push_back({by_edge ? m->h_sell_edge[x] : i - n0, i, jr, x});
                                 continue;
                             }
                             const bool isL = jr >= 0;
                             const int j = isL ? jr : ~jr;
                             const bool inside = j >= n0 && j < n1;
                             if (inside && !isL) continue;  // taken from its L end
-                            const int key = inside ? std::min(i, j) - n0 : i - n0;
+                            const int key = by_edge ? m->h_sell_edge[x] : (inside ? std::min(i, j) : i) - n0;
                             w.push_back({key, isL ? i : j, isL ? j : i, x});
                         }
                     }
+                    // (edge order: the keys are distinct; node order: ties by L, then R)
                     std::sort(w.begin(), w.end(), [](const Work &a, const Work &b) {
                         return a.key != b.key ? a.key < b.key : (a.l != b.l ? a.l < b.l : a.r < b.r);
                     });
@@ -998,16 +1018,24 @@ int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
         std::vector<int> wptr(P + 1, 0);
         for (int p = 0; p < P; p++) wptr[p + 1] = wptr[p] + (int)pw[p].size();
         const int64_t W = wptr[P];
-        std::vector<int> wl(W), wr(W), slot(m->nsell, 0);
-        std::vector<double> ws(3 * (size_t)W);
+        HostArray<int> wl(W), wr(W);  // every entry written below
+        HostArray<double> ws(3 * (size_t)W);
+        std::vector<int> slot(m->nsell, 0);
         bool ok = true;
         std::vector<char> bad(P, 0);
         cfd_parallel_for(
             P,
             [&](int64_t p0, int64_t p1) {
+                std::vector<int> cnt;
                 for (int p = (int)p0; p < (int)p1; p++) {
                     const int n0 = pstart[p], n1 = pstart[p + 1];
-                    std::vector<std::vector<int>> lists(n1 - n0);
+                    // each node's slots, ascending as the work edges are visited in order
+                    cnt.assign(n1 - n0, 0);
+                    auto add = [&](int i, int v) {
+                        int &c = cnt[i - n0];
+                        if (c < ip[i + 1] - ip[i]) slot[sell(i, c)] = v;
+                        c++;
+                    };
                     for (int q = 0; q < (int)pw[p].size(); q++) {
                         const Work &u = pw[p][q];
                         const size_t o = (size_t)wptr[p] + q;
@@ -1016,43 +1044,41 @@ int cfd_ensure_patches(cfd_mesh *m, bool fp32) {
                         ws[3 * o] = sx[u.inc];
                         ws[3 * o + 1] = sy[u.inc];
                         ws[3 * o + 2] = sz[u.inc];
-                        if (u.l >= n0 && u.l < n1) lists[u.l - n0].push_back(q);
-                        if (u.r >= n0 && u.r < n1) lists[u.r - n0].push_back(q | (int)0x80000000u);
+                        if (u.l >= n0 && u.l < n1) add(u.l, q);
+                        if (u.r >= n0 && u.r < n1) add(u.r, q | (int)0x80000000u);
                     }
-                    for (int i = n0; i < n1; i++) {
-                        const std::vector<int> &li = lists[i - n0];  // ascending: work edges are visited in order
-                        if ((int)li.size() != ip[i + 1] - ip[i]) {
-                            bad[p] = 1;
-                            continue;
-                        }
-                        for (int k = 0; k < (int)li.size(); k++) slot[sell(i, k)] = li[k];
-                    }
+                    for (int i = n0; i < n1; i++)
+                        if (cnt[i - n0] != ip[i + 1] - ip[i]) bad[p] = 1;
                 }
             },
             1);
         for (char b : bad) ok = ok && !b;
         if (!ok) return cfd_set_error(m->ctx, CFD_E_STATE, "work-edge patches do not cover the incidence lists");
         int rc = 0;
-        m->pw_ptr = upload(m, wptr.data(), wptr.size(), &rc);
-        if (!rc) m->pn_start = upload(m, pstart.data(), pstart.size(), &rc);
-        if (!rc) m->pw_l = upload(m, wl.data(), wl.size(), &rc);
-        if (!rc) m->pw_r = upload(m, wr.data(), wr.size(), &rc);
-        if (!rc) m->pn_slot = upload(m, slot.data(), slot.size(), &rc);
-        if (!rc) m->d.ws = upload(m, ws.data(), ws.size(), &rc);
+        ps.pw_ptr = upload(m, wptr.data(), wptr.size(), &rc);
+        if (!rc) ps.pn_start = upload(m, pstart.data(), pstart.size(), &rc);
+        if (!rc) ps.pw_l = upload(m, wl.data(), wl.size(), &rc);
+        if (!rc) ps.pw_r = upload(m, wr.data(), wr.size(), &rc);
+        if (!rc) ps.pn_slot = upload(m, slot.data(), slot.size(), &rc);
+        if (!rc) ps.ws = upload(m, ws.data(), ws.size(), &rc);
         if (rc) {
-            m->pw_ptr = nullptr;
+            ps.pw_ptr = nullptr;
             return rc;
         }
-        m->npatch = P;
-        m->npw = W;
+        ps.npatch = P;
+        ps.npw = W;
+        if (cfd_progress())
+            fprintf(stderr, "cfd_ensure_patches: %d patches, %lld work edges (%s order) %8.3f s\n", P, (long long)W,
+                    order == kPatchEdgeOrder ? "edge" : "node",
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     }
-    if (fp32 && !m->f.ws) {
-        std::vector<double> ws(3 * (size_t)m->npw);
+    if (fp32 && !ps.ws_f) {
+        std::vector<double> ws(3 * (size_t)ps.npw);
         CFD_HIP(m, hipStreamSynchronize(m->stream));
-        CFD_HIP(m, cfd_memcpy(ws.data(), m->d.ws, sizeof(double) * ws.size(), hipMemcpyDeviceToHost));
+        CFD_HIP(m, cfd_memcpy(ws.data(), ps.ws, sizeof(double) * ws.size(), hipMemcpyDeviceToHost));
         std::vector<float> wf(ws.begin(), ws.end());
         int rc = 0;
-        m->f.ws = upload(m, wf.data(), wf.size(), &rc);
+        ps.ws_f = upload(m, wf.data(), wf.size(), &rc);
         if (rc) return rc;
     }
     return CFD_OK;

@@ -48,7 +48,6 @@ struct ResidualBufs {
     T *vol = nullptr;
     T *lsq = nullptr;
     T *res = nullptr;
-    T *ws = nullptr;  // [W][3] area vector of each work edge of the flux-once residual
 };
 constexpr int kQA = 8;
 constexpr int kQB = 20;  // gradient (15) and limiter (5) of a node, unpadded
@@ -157,6 +156,28 @@ struct BlockSell {
     int *src = nullptr;   // [nblk] BSR slot it is copied from, -1 for padding
 };
 
+// Work-edge patches of the flux-once residual: patch p holds the internal
+// nodes [pn_start[p], pn_start[p+1]) (at most 256, never across cases of a
+// union) and its work edges pw_ptr[p] .. pw_ptr[p+1]: each interior edge with
+// both ends in the patch once, each edge to another patch, each boundary edge
+// of a patch node.  The work edges of a patch go in one of two orders:
+//   kPatchNodeOrder: by their first end in the patch (then L, R): the chunks
+//     of a patch touch compact groups of nodes (the fast build);
+//   kPatchEdgeOrder: by reference edge index, boundary edges last, so each
+//     node adds its fluxes in the reference's order (the exact build).
+constexpr int kPatchNodeOrder = 0, kPatchEdgeOrder = 1;
+struct PatchSet {
+    int npatch = 0;
+    int64_t npw = 0;
+    int *pn_start = nullptr;  // [npatch + 1]
+    int *pw_ptr = nullptr;    // [npatch + 1]
+    int *pw_l = nullptr;      // [npw] internal index of the L end
+    int *pw_r = nullptr;      // [npw] internal index of the R end, or N + b for boundary edge b
+    int *pn_slot = nullptr;   // SELL as nbr: patch-local slot of each incidence, ascending per node; bit 31: R end
+    double *ws = nullptr;     // [npw][3] area vector, oriented L -> R (outward for boundary edges)
+    float *ws_f = nullptr;    // the same in FP32 (made on first FP32 use)
+};
+
 struct cfd_mesh {
     cfd_ctx *ctx = nullptr;
     hipStream_t stream = nullptr;
@@ -192,19 +213,13 @@ struct cfd_mesh {
     int *risk = nullptr;      // [1] nodes with a near-vacuum face state
     int mark_epoch = 0;
 
-    // Work-edge patches of the flux-once residual (CFD_ARITH_FAST), built on
-    // first use: patch p holds the internal nodes [pn_start[p], pn_start[p+1])
-    // (at most 256, never across cases of a union) and its work edges
-    // pw_ptr[p] .. pw_ptr[p+1]: each interior edge with both ends in the
-    // patch once, each edge to another patch, each boundary edge of a patch
-    // node, ordered by their first end in the patch.
-    int npatch = 0;
-    int64_t npw = 0;
-    int *pn_start = nullptr;  // [npatch + 1]
-    int *pw_ptr = nullptr;    // [npatch + 1]
-    int *pw_l = nullptr;      // [npw] internal index of the L end
-    int *pw_r = nullptr;      // [npw] internal index of the R end, or N + b for boundary edge b
-    int *pn_slot = nullptr;   // SELL as nbr: patch-local slot of each incidence, ascending per node; bit 31: R end
+    // Work-edge patches of the flux-once residual, built on first use, one
+    // set per work-edge order (kPatchNodeOrder for the fast build,
+    // kPatchEdgeOrder for the exact build)
+    PatchSet patch[2];
+    // host, SELL as nbr: the reference edge index of each incidence (E + b for
+    // boundary edge b; -1 for padding), which orders kPatchEdgeOrder's work edges
+    HostArray<int> h_sell_edge{0};
 
     ResidualBufs<double> d;   // FP64
     ResidualBufs<float> f;    // FP32 (allocated on first use)
@@ -334,7 +349,7 @@ int cfd_ensure_fp32(cfd_mesh *m);
 // Builds the work-edge patches of the flux-once residual from the device
 // incidence lists (once per mesh) and the FP32 copy of their area vectors
 // when fp32 is set.
-int cfd_ensure_patches(cfd_mesh *m, bool fp32);
+int cfd_ensure_patches(cfd_mesh *m, bool fp32, int order);  // order: kPatchNodeOrder or kPatchEdgeOrder
 int cfd_ensure_ilu_map(cfd_mesh *m);  // natural-order ILU(0) merge map, on first use
 // Whether the residual of these parameters runs the flux-once patch kernel:
 // the fast build at second order, unless CFD_FLUX_ONCE=0 or 1 forces it.

@@ -1103,18 +1103,22 @@ __global__ __launch_bounds__(kBlock) void k_residual(ResArgs<T> a) {
     nr.store(a, i, deg);
 }
 
-#if CFD_FAST_MATH
 // ---------------------------------------------------------------------------
-// Flux-once residual (CFD_ARITH_FAST).  Workgroup p owns a patch of at most
+// Flux-once residual (second order).  Workgroup p owns a patch of at most
 // kBlock consecutive internal nodes (a compact piece of the Morton order;
 // patches restart at each case of a union) and its work edges: each interior edge with both ends in the patch
 // once, each edge to another patch (both patches evaluate it, each for its
 // own end) and each boundary edge of a patch node.  Chunks of kBlock work
 // edges are evaluated one per thread into LDS; then every node adds the
 // chunk's fluxes of its incidences, whose slots it lists in ascending order
-// (held in LDS), so each node's sum has a fixed order (deterministic
-// results).  Interior fluxes are evaluated once per patch instead of once per
-// end, and every flux stays in the patch's LDS.
+// (held in LDS).  The work edges are in the reference's edge order, so each
+// node adds its fluxes in the order its incidence list gives (that of the
+// reference's edge loop), and a flux evaluated once for both ends has the
+// bits of the node-centred kernel's two evaluations (the R end's face state
+// q_R - phi (g . R) equals q_R + phi (g . (-R)) exactly): the exact build's
+// results are bit-identical to the node-centred kernel's.  Interior fluxes are
+// evaluated once per patch instead of once per end, and every flux stays in
+// the patch's LDS.
 // ---------------------------------------------------------------------------
 template <class T>
 struct PatchArgs {
@@ -1268,7 +1272,6 @@ __global__ __launch_bounds__(kBlock) void k_residual_patch(PatchArgs<T> a) {
         }
     }
 }
-#endif  // CFD_FAST_MATH
 
 // ---------------------------------------------------------------------------
 // Local time step, RMS, update, forces
@@ -1503,10 +1506,9 @@ static void launch_res_flux(cfd_mesh *m, const ResArgs<T> &a, int order, bool li
     }
 }
 
-#if CFD_FAST_MATH
 template <class T, int FLUX>
 static void launch_patch_flux(cfd_mesh *m, const PatchArgs<T> &a, int order, bool lim, bool stats) {
-    dim3 g(xcd_grid(m->npatch)), bl(kBlock);
+    dim3 g(xcd_grid(a.npatch)), bl(kBlock);
     if (order == 2) {
         if (lim) {
             if (stats) hipLaunchKernelGGL((k_residual_patch<T, FLUX, 2, true, true>), g, bl, 0, m->stream, a);
@@ -1525,9 +1527,20 @@ static void launch_patch_flux(cfd_mesh *m, const PatchArgs<T> &a, int order, boo
 // first call).
 template <class T>
 static int launch_patch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats) {
-    int rc = cfd_ensure_patches(m, sizeof(T) == 4);
+    // the fast build evaluates its work edges in node order; the exact build
+    // in the reference's edge order, so each node's sum keeps its order
+#if CFD_FAST_MATH
+    const int order = kPatchNodeOrder;
+#else
+    const int order = kPatchEdgeOrder;
+#endif
+    int rc = cfd_ensure_patches(m, sizeof(T) == 4, order);
     if (rc) return rc;
-    PatchArgs<T> a{m->N, m->npatch, m->pn_start, m->inc_ptr, m->sptr, m->pn_slot, m->pw_ptr, m->pw_l, m->pw_r, b.ws, b.qa, b.qb,
+    const PatchSet &ps = m->patch[order];
+    const T *ws;
+    if constexpr (sizeof(T) == 4) ws = ps.ws_f;
+    else ws = ps.ws;
+    PatchArgs<T> a{m->N, ps.npatch, ps.pn_start, m->inc_ptr, m->sptr, ps.pn_slot, ps.pw_ptr, ps.pw_l, ps.pw_r, ws, b.qa, b.qb,
                    b.res, (T)p->gamma, (p->limiter_order == 2) ? T(0.5) : T(0.0), p->entropy_fix, m->eigbits,
                    m->flag, m->ncase ? m->case_flag : nullptr, m->ecase};
     const bool lim = p->limiter != CFD_LIM_NONE;
@@ -1538,7 +1551,6 @@ static int launch_patch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_para
     CFD_HIP(m, hipGetLastError());
     return CFD_OK;
 }
-#endif
 
 template <class T>
 static int launch_node_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats) {
@@ -1555,17 +1567,16 @@ static int launch_node_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_param
     return CFD_OK;
 }
 
-// The exact build always runs the node-centred kernel.  The fast build runs
-// the flux-once patch kernel for second order, where each edge's
-// reconstruction and flux are worth computing once (1.3-1.7x faster on a
-// 4.2M-node cube), and the node-centred kernel for first order, where the
-// patch kernel's extra traffic (work-edge lists, LDS partial sums) costs more
-// than the second flux evaluation (cfd_residual_uses_patches).
+// The fast build runs the flux-once patch kernel for second order, where each
+// edge's reconstruction and flux are worth computing once, and the
+// node-centred kernel for first order, where the patch kernel's extra traffic
+// (work-edge lists, LDS partial sums) costs more than the second flux
+// evaluation.  The exact build runs the node-centred kernel unless asked
+// (cfd_residual_uses_patches; CFD_FLUX_ONCE=0/1 forces either); both kernels
+// give the same bits there.
 template <class T>
 int launch_residual(cfd_mesh *m, ResidualBufs<T> &b, const cfd_params *p, bool stats) {
-#if CFD_FAST_MATH
     if (cfd_residual_uses_patches(p)) return launch_patch_residual<T>(m, b, p, stats);
-#endif
     return launch_node_residual<T>(m, b, p, stats);
 }
 
