@@ -24,6 +24,9 @@
 
 using namespace cfd;
 
+// The kernels live in the build's namespace too, so that profiles tell the
+// two builds' kernels apart (res_fast::... and res_exact::...).
+namespace CFD_RES_NS {
 namespace {
 
 constexpr int kBlock = 256;
@@ -1369,6 +1372,7 @@ int grid_for(size_t n) {
 }
 
 }  // namespace
+}  // namespace CFD_RES_NS
 
 // ---------------------------------------------------------------------------
 // launchers
