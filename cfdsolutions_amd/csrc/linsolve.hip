@@ -535,41 +535,46 @@ __global__ __launch_bounds__(kBlock) void k_ilu0_level(int nrows, const int *__r
     const int i = active ? rows[t] : 0;
     TM drow[5] = {TM(0.0), TM(0.0), TM(0.0), TM(0.0), TM(0.0)};
     if (active) {
+        // the lanes of a block row hold one row each of every block they
+        // load; a product needs the other rows, which come from the group's
+        // lanes by shuffles (the five lanes of a row run every loop together)
+        const int gbase = lane - r;
         for (int kk = ia[i]; kk < iau[i]; kk++) {
             const int k = ja[kk];
             TM *Lk = lu + 25 * (size_t)kk + 5 * r;
-            const TM *Dk = dinvk + 25 * (size_t)k;
-            TM Lr[5], U[5];
+            const TM *Dk = dinvk + 25 * (size_t)k + 5 * r;
+            TM Lr[5], U[5], Dr[5];
 #pragma unroll
             for (int q = 0; q < 5; q++) U[q] = Lk[q];
+#pragma unroll
+            for (int c = 0; c < 5; c++) Dr[c] = Dk[c];
 #pragma unroll
             for (int c = 0; c < 5; c++) {
                 TM s = TM(0.0);
 #pragma unroll
-                for (int q = 0; q < 5; q++) s += U[q] * Dk[5 * q + c];
+                for (int q = 0; q < 5; q++) s += U[q] * __shfl(Dr[c], gbase + q, 64);
                 Lr[c] = s;
             }
 #pragma unroll
             for (int c = 0; c < 5; c++) Lk[c] = Lr[c];
             // upper blocks of row k that meet a block of row i, in ascending
-            // column order (the precomputed merge of the two rows)
-            // CH upper blocks at a time (two in FP32; FP64 has no registers to
-            // spare): their slots, blocks and targets are loaded before any
+            // column order (the precomputed merge of the two rows), CH at a
+            // time: their slots, rows and targets are loaded before any
             // target is updated.  The targets are distinct blocks of row i, so
             // each update is the one-at-a-time arithmetic.
-            constexpr int CH = sizeof(TM) == 4 ? 2 : 1;
+            constexpr int CH = sizeof(TM) == 4 ? 8 : 4;
             const int p0 = iau[k] + 1, np = ia[k + 1] - p0, m0 = moff[kk];
             for (int t2 = 0; t2 < np; t2 += CH) {
                 int qs[CH];
-                TM up[CH][25], tv[CH][5];
+                TM ur[CH][5], tv[CH][5];
 #pragma unroll
                 for (int u = 0; u < CH; u++) qs[u] = t2 + u < np ? mqq[m0 + t2 + u] : -1;
 #pragma unroll
                 for (int u = 0; u < CH; u++)
                     if (qs[u] >= 0) {
-                        const TM *Up = luk + 25 * (size_t)(p0 + t2 + u);
+                        const TM *Up = luk + 25 * (size_t)(p0 + t2 + u) + 5 * r;
 #pragma unroll
-                        for (int e = 0; e < 25; e++) up[u][e] = Up[e];
+                        for (int c = 0; c < 5; c++) ur[u][c] = Up[c];
 #pragma unroll
                         for (int c = 0; c < 5; c++) tv[u][c] = lu[25 * (size_t)qs[u] + 5 * r + c];
                     }
@@ -581,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_ilu0_level(int nrows, const int *__r
                     for (int c = 0; c < 5; c++) {
                         TM s = TM(0.0);
 #pragma unroll
-                        for (int q = 0; q < 5; q++) s += Lr[q] * up[u][5 * q + c];
+                        for (int q = 0; q < 5; q++) s += Lr[q] * __shfl(ur[u][c], gbase + q, 64);
                         T[c] = tv[u][c] - s;
                     }
                 }
