@@ -887,27 +887,45 @@ __global__ __launch_bounds__(kBlock) void k_trsv_mc_sell(int r0, int r1, const i
                                                          const int *__restrict__ off, const int *__restrict__ col,
                                                          const TM *__restrict__ val, const TM *__restrict__ dv,
                                                          const TV *__restrict__ rhs, TV *w, TV *z, int backward) {
-    const int i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+    const int i = (r0 & ~63) + xcd_block() * kBlock + threadIdx.x;
     if (i < r0 || i >= r1) return;
     const int l = i & 63;
     const int len = backward ? ia[i + 1] - iau[i] - 1 : iau[i] - ia[i];
     TV y[5];
 #pragma unroll
     for (int r = 0; r < 5; r++) y[r] = backward ? w[5 * (size_t)i + r] : rhs[5 * (size_t)perm[i] + r];
+    // U blocks at a time: their column indices, then their x values and
+    // entries, are all loaded before the products (a colour has few rows, so
+    // the waves are few and each must keep several loads in flight).
+    constexpr int U = sizeof(TM) == 8 ? 2 : 4;
     size_t p = (size_t)off[i >> 6] + l;
-    for (int k = 0; k < len; k++, p += 64) {
-        const TV *wj = w + 5 * (size_t)col[p];
-        TV wv[5];
+    for (int k0 = 0; k0 < len; k0 += U, p += 64 * U) {
+        int jj[U];
 #pragma unroll
-        for (int c = 0; c < 5; c++) wv[c] = wj[c];
-        const TM *A = val + 25 * (p - l) + l;
+        for (int u = 0; u < U; u++) jj[u] = k0 + u < len ? col[p + 64 * u] : -1;
+        TV wv[U][5];
+        TM av[U][25];
 #pragma unroll
-        for (int r = 0; r < 5; r++) {
-            TV s = TV(0.0);
+        for (int u = 0; u < U; u++)
+            if (jj[u] >= 0) {
+                const TV *wj = w + 5 * (size_t)jj[u];
 #pragma unroll
-            for (int c = 0; c < 5; c++) s += (TV)A[64 * (5 * r + c)] * wv[c];
-            y[r] = y[r] - s;
-        }
+                for (int c = 0; c < 5; c++) wv[u][c] = wj[c];
+                const TM *A = val + 25 * (p + 64 * u - l) + l;
+#pragma unroll
+                for (int e = 0; e < 25; e++) av[u][e] = ld_stream(A + 64 * e);
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (jj[u] >= 0) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) {
+                    TV s = TV(0.0);
+#pragma unroll
+                    for (int c = 0; c < 5; c++) s += (TV)av[u][5 * r + c] * wv[u][c];
+                    y[r] = y[r] - s;
+                }
+            }
     }
     if (!backward) {
 #pragma unroll
@@ -957,28 +975,43 @@ __global__ __launch_bounds__(kBlock) void k_trsv_mc_sell_half(int r0, int r1, co
                                                               const float *__restrict__ dsc,
                                                               const float *__restrict__ rhs, float *w, float *z,
                                                               int backward) {
-    const int i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+    const int i = (r0 & ~63) + xcd_block() * kBlock + threadIdx.x;
     if (i < r0 || i >= r1) return;
     const int l = i & 63;
     const int len = backward ? ia[i + 1] - iau[i] - 1 : iau[i] - ia[i];
     float y[5];
 #pragma unroll
     for (int r = 0; r < 5; r++) y[r] = backward ? w[5 * (size_t)i + r] : rhs[5 * (size_t)perm[i] + r];
+    constexpr int U = 4;  // blocks whose loads are in flight together, as in k_trsv_mc_sell
     size_t p = (size_t)off[i >> 6] + l;
-    for (int k = 0; k < len; k++, p += 64) {
-        const float *wj = w + 5 * (size_t)col[p];
-        float wv[5];
+    for (int k0 = 0; k0 < len; k0 += U, p += 64 * U) {
+        int jj[U];
 #pragma unroll
-        for (int c = 0; c < 5; c++) wv[c] = wj[c];
-        const _Float16 *A = val + 25 * (p - l) + l;
-        const float sc = vsc[p];
+        for (int u = 0; u < U; u++) jj[u] = k0 + u < len ? col[p + 64 * u] : -1;
+        float wv[U][5], sc[U];
+        _Float16 av[U][25];
 #pragma unroll
-        for (int r = 0; r < 5; r++) {
-            float s = 0.0f;
+        for (int u = 0; u < U; u++)
+            if (jj[u] >= 0) {
+                const float *wj = w + 5 * (size_t)jj[u];
 #pragma unroll
-            for (int c = 0; c < 5; c++) s += (float)A[64 * (5 * r + c)] * wv[c];
-            y[r] = y[r] - sc * s;
-        }
+                for (int c = 0; c < 5; c++) wv[u][c] = wj[c];
+                const _Float16 *A = val + 25 * (p + 64 * u - l) + l;
+#pragma unroll
+                for (int e = 0; e < 25; e++) av[u][e] = ld_stream(A + 64 * e);
+                sc[u] = ld_stream(vsc + p + 64 * u);
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (jj[u] >= 0) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) {
+                    float s = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < 5; c++) s += (float)av[u][5 * r + c] * wv[u][c];
+                    y[r] = y[r] - sc[u] * s;
+                }
+            }
     }
     if (!backward) {
 #pragma unroll
@@ -1786,7 +1819,7 @@ void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r
                 for (int t = 0; t < m->ncolor; t++) {
                     const int c = pass ? m->ncolor - 1 - t : t;
                     const int r0 = m->mc_cstart[c], r1 = m->mc_cstart[c + 1];
-                    const int g = (r1 - (r0 & ~63) + kBlock - 1) / kBlock;
+                    const int g = xcd_grid((r1 - (r0 & ~63) + kBlock - 1) / kBlock);
                     hipLaunchKernelGGL(k_trsv_mc_sell_half, dim3(g), dim3(kBlock), 0, m->stream, r0, r1, m->mc_ia,
                                        m->mc_iau, m->mc_perm, bs.off, bs.col, val, vsc, m->dsvh, m->dsvs, r, wk, z,
                                        pass);
@@ -1803,7 +1836,7 @@ void apply_prec(cfd_mesh *m, int kind, const TM *lu, const TM *dinv, const TV *r
             for (int t = 0; t < m->ncolor; t++) {
                 const int c = pass ? m->ncolor - 1 - t : t;
                 const int r0 = m->mc_cstart[c], r1 = m->mc_cstart[c + 1];
-                const int g = (r1 - (r0 & ~63) + kBlock - 1) / kBlock;
+                const int g = xcd_grid((r1 - (r0 & ~63) + kBlock - 1) / kBlock);
                 hipLaunchKernelGGL((k_trsv_mc_sell<TM, TV>), dim3(g), dim3(kBlock), 0, m->stream, r0, r1, m->mc_ia,
                                    m->mc_iau, m->mc_perm, bs.off, bs.col, val, dv, r, wk, z, pass);
             }
