@@ -1106,7 +1106,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, const int *__restri
                                                       const int *__restrict__ col, const TA *__restrict__ val,
                                                       const TX *__restrict__ x, TY *__restrict__ y,
                                                       const double *__restrict__ bsub) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= N) return;
     const int l = i & 63;
     const int len = ia[i + 1] - ia[i];
@@ -1123,7 +1123,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, const int *__restri
 #pragma unroll
         for (int r = 0; r < 5; r++)
 #pragma unroll
-            for (int c = 0; c < 5; c++) s[r] += (TY)A[64 * (5 * r + c)] * (TY)xv[c];
+            for (int c = 0; c < 5; c++) s[r] += (TY)ld_stream(A + 64 * (5 * r + c)) * (TY)xv[c];
     }
 #pragma unroll
     for (int r = 0; r < 5; r++) {
@@ -1141,7 +1141,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_half(int N, const int *__r
                                                            const _Float16 *__restrict__ val,
                                                            const float *__restrict__ vsc, const TX *__restrict__ x,
                                                            TY *__restrict__ y, const double *__restrict__ bsub) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= N) return;
     const int l = i & 63;
     const int len = ia[i + 1] - ia[i];
@@ -1155,12 +1155,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_half(int N, const int *__r
 #pragma unroll
         for (int c = 0; c < 5; c++) xv[c] = (float)xj[c];
         const _Float16 *A = val + 25 * (p - l) + l;
-        const float sc = vsc[p];
+        const float sc = ld_stream(vsc + p);
 #pragma unroll
         for (int r = 0; r < 5; r++) {
             float t = 0.0f;
 #pragma unroll
-            for (int c = 0; c < 5; c++) t += (float)A[64 * (5 * r + c)] * xv[c];
+            for (int c = 0; c < 5; c++) t += (float)ld_stream(A + 64 * (5 * r + c)) * xv[c];
             s[r] += (TY)(sc * t);
         }
     }
@@ -1778,10 +1778,11 @@ int build_prec(cfd_mesh *m, int kind, const TM *a, TM *lu, TM *dinv, bool half) 
 template <class TM, class TX, class TY>
 void spmv(cfd_mesh *m, const TX *x, TY *y, const double *bsub) {
     if (std::is_same<TM, float>::value && m->use_sell && m->asell_half)
-        hipLaunchKernelGGL((k_spmv_sell_half<TX, TY>), dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0,
+        hipLaunchKernelGGL((k_spmv_sell_half<TX, TY>), dim3(xcd_grid((m->N + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            m->stream, m->N, m->ia, m->a_sell.off, m->a_sell.col, m->asvh, m->asvs, x, y, bsub);
     else if (m->use_sell)
-        hipLaunchKernelGGL((k_spmv_sell<TM, TX, TY>), dim3((m->N + kBlock - 1) / kBlock), dim3(kBlock), 0, m->stream,
+        hipLaunchKernelGGL((k_spmv_sell<TM, TX, TY>), dim3(xcd_grid((m->N + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           m->stream,
                            m->N, m->ia, m->a_sell.off, m->a_sell.col, (const TM *)SellVals<TM>::a(m), x, y, bsub);
     else
         hipLaunchKernelGGL((k_spmv<TM, TX, TY>), dim3((int)((5 * (size_t)m->N + kBlock - 1) / kBlock)), dim3(kBlock),
