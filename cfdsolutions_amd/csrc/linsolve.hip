@@ -880,7 +880,9 @@ __global__ __launch_bounds__(kBlock) void k_trsv_mc(int nrows, const int *__rest
 // other), one row per lane.  Forward: w_i = r_P(i) - sum over the strict lower
 // blocks L_ij w_j; backward: w_i = D_i^-1 (w_i - sum over the strict upper
 // blocks U_ij w_j), also scattered to z in original order.  Same operation
-// order as k_trsv_mc.
+// order as k_trsv_mc.  Workgroups are numbered XCD by XCD (the grid is a
+// multiple of the XCD count), so each XCD sweeps a contiguous range of the
+// colour's rows and gathers from fewer lines of w.
 template <class TM, class TV>
 __global__ __launch_bounds__(kBlock) void k_trsv_mc_sell(int r0, int r1, const int *__restrict__ ia,
                                                          const int *__restrict__ iau, const int *__restrict__ perm,
@@ -895,8 +897,8 @@ __global__ __launch_bounds__(kBlock) void k_trsv_mc_sell(int r0, int r1, const i
 #pragma unroll
     for (int r = 0; r < 5; r++) y[r] = backward ? w[5 * (size_t)i + r] : rhs[5 * (size_t)perm[i] + r];
     // U blocks at a time: their column indices, then their x values and
-    // entries, are all loaded before the products (a colour has few rows, so
-    // the waves are few and each must keep several loads in flight).
+    // entries (streamed: each is read once per sweep) are loaded before the
+    // products, which are then subtracted in block order as before.
     constexpr int U = sizeof(TM) == 8 ? 2 : 4;
     size_t p = (size_t)off[i >> 6] + l;
     for (int k0 = 0; k0 < len; k0 += U, p += 64 * U) {
@@ -1100,7 +1102,9 @@ __global__ __launch_bounds__(kBlock) void k_to_sell(size_t nblk, const int *__re
 
 // y = A x (or y = b - A x) with A in block-SELL layout, one row per lane.  Each
 // row's sum runs over its blocks in BSR order and each block's columns in
-// order, as k_spmv and the oracle add them, so the bits are the same.
+// order, as k_spmv and the oracle add them, so the bits are the same.  Rows
+// go to the XCDs in contiguous ranges (grid a multiple of the XCD count) and
+// the entries, read once per product, are streamed past the L2.
 template <class TA, class TX, class TY>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(int N, const int *__restrict__ ia, const int *__restrict__ off,
                                                       const int *__restrict__ col, const TA *__restrict__ val,
