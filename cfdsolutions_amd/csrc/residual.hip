@@ -364,13 +364,16 @@ __device__ __forceinline__ void pressure_nbr(const T *qj, T xj, T yj, T zj, cons
         Q[k] = F0 ? qc[k] + ph[k] * gR : qc[k] + ph[k] * (frac * (qj[k] - qc[k]) + (T(1.0) - frac) * gR);
     }
     const T P = (g - T(1.0)) * (Q[4] - (T(0.5) / Q[0]) * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]));
-    if (P < T(0.0)) {
+    const bool neg = P < T(0.0);
+    const bool vac = !neg && near_vacuum(Q, P);
+    if (neg) {
 #pragma unroll
         for (int k = 0; k < 5; k++) ph[k] = T(0.0);
-        zeroed = true;
-    } else if (near_vacuum(Q, P)) {
-        risky = true;
     }
+    // both flags written on every path: a store to one flag or the other
+    // made the compiler keep the pair in scratch memory
+    zeroed = zeroed || neg;
+    risky = risky || vac;
 }
 
 // After every neighbour: if a face zeroed the limiter, every face state is
