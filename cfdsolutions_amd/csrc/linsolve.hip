@@ -1386,9 +1386,14 @@ __global__ __launch_bounds__(64) void k_mdot_final(int nparts, int nv, const dou
 
 // U entries i, i + stride, ... of the pass: their updates (the loads of 8
 // vectors for all U entries issued together), stores, and products.
-template <class TV, int NVMAX, bool NORM, int U>
+// hold (LDS, [NVMAX][kBlock], one entry per call): the entry's V values are
+// kept there between the update and the products instead of being read again
+// from memory (the same values).
+template <class TV, int NVMAX, bool NORM, int U, bool LHOLD = false>
 __device__ __forceinline__ void cgs_entries(size_t o, size_t i, size_t stride, int nv, const TV *__restrict__ V,
-                                            size_t ld, const double *h, TV *__restrict__ w, double *acc) {
+                                            size_t ld, const double *h, TV *__restrict__ w, double *acc,
+                                            TV *hold = nullptr) {
+    static_assert(!LHOLD || (U == 1 && !NORM), "hold: one entry, dots pass");
     constexpr int NC = (NVMAX + 7) / 8;
     double wi[U];
 #pragma unroll
@@ -1407,6 +1412,11 @@ __device__ __forceinline__ void cgs_entries(size_t o, size_t i, size_t stride, i
 #pragma unroll
                 for (int kk = 0; kk < 8; kk++)
                     if (8 * c + kk < nv) wi[u] = wi[u] - h[8 * c + kk] * (double)vv[u][kk];
+            if (LHOLD) {
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++)
+                    if (8 * c + kk < nv) hold[(8 * c + kk) * kBlock + threadIdx.x] = vv[0][kk];
+            }
         }
     }
     double wd[U];
@@ -1431,7 +1441,9 @@ __device__ __forceinline__ void cgs_entries(size_t o, size_t i, size_t stride, i
             for (int u = 0; u < U; u++)
 #pragma unroll
                 for (int kk = 0; kk < 8; kk++)
-                    if (8 * c + kk < nv) vv[u][kk] = V[ld * (8 * c + kk) + o + i + u * stride];
+                    if (8 * c + kk < nv)
+                        vv[u][kk] = LHOLD ? hold[(8 * c + kk) * kBlock + threadIdx.x]
+                                          : V[ld * (8 * c + kk) + o + i + u * stride];
 #pragma unroll
             for (int u = 0; u < U; u++)
 #pragma unroll
@@ -1453,10 +1465,11 @@ __device__ __forceinline__ void cgs_entries(size_t o, size_t i, size_t stride, i
 // entries go four at a time (two for long bases) while four remain, so their
 // loads overlap; it costs registers, so short walks take one entry at a time.
 // h: nv coefficients in LDS; nv <= NVMAX.
-template <class TV, int NVMAX, bool NORM, bool DEEP>
+template <class TV, int NVMAX, bool NORM, bool DEEP, bool LHOLD = false>
 __device__ __forceinline__ void block_cgs_pass(size_t o, size_t i0, size_t n, size_t stride, int nv,
                                                const TV *__restrict__ V, size_t ld, const double *h,
                                                TV *__restrict__ w, double *__restrict__ part) {
+    static_assert(!LHOLD || (!DEEP && !NORM), "hold: one entry at a time, dots pass");
     constexpr int NA = NORM ? 1 : NVMAX;
     constexpr int U = !DEEP ? 1 : NVMAX > 24 ? 2 : 4;
     __shared__ double wsum[NA][kBlock / 64];
@@ -1468,7 +1481,12 @@ __device__ __forceinline__ void block_cgs_pass(size_t o, size_t i0, size_t n, si
     if (U > 1)
         for (; i + (U - 1) * stride < n; i += U * stride)
             cgs_entries<TV, NVMAX, NORM, U>(o, i, stride, nv, V, ld, h, w, acc);
-    for (; i < n; i += stride) cgs_entries<TV, NVMAX, NORM, 1>(o, i, stride, nv, V, ld, h, w, acc);
+    if (LHOLD) {
+        __shared__ TV hold[LHOLD ? NVMAX : 1][kBlock];
+        for (; i < n; i += stride) cgs_entries<TV, NVMAX, NORM, 1, LHOLD>(o, i, stride, nv, V, ld, h, w, acc, &hold[0][0]);
+    } else {
+        for (; i < n; i += stride) cgs_entries<TV, NVMAX, NORM, 1>(o, i, stride, nv, V, ld, h, w, acc);
+    }
     const int nd = NORM ? 1 : nv;
 #pragma unroll
     for (int k = 0; k < NA; k++) {
@@ -1509,8 +1527,12 @@ __global__ __launch_bounds__(kBlock) void k_cgs_pass_seg(const int2 *__restrict_
     for (int k = threadIdx.x; k < nv; k += kBlock) h[k] = out[(size_t)cb.x * kBOut + slot + k];
     __syncthreads();
     const size_t P = (size_t)(first[cb.x + 1] - first[cb.x]);
-    block_cgs_pass<TV, NVMAX, NORM, false>((size_t)off[cb.x], cb.y * (size_t)kBlock + threadIdx.x, (size_t)len[cb.x],
-                                    P * kBlock, nv, V, ld, h, w, part + (size_t)blockIdx.x * (kMaxKrylov + 1));
+    // (the FP32 dots pass with up to 41 vectors keeps each entry's V values
+    // in LDS: a case's blocks give each thread one or a few entries)
+    constexpr bool hold = !NORM && sizeof(TV) == 4 && NVMAX <= 41;
+    block_cgs_pass<TV, NVMAX, NORM, false, hold>((size_t)off[cb.x], cb.y * (size_t)kBlock + threadIdx.x,
+                                                 (size_t)len[cb.x], P * kBlock, nv, V, ld, h, w,
+                                                 part + (size_t)blockIdx.x * (kMaxKrylov + 1));
 }
 
 // dst = src * (1/scale) with scale read from the device (scale = out[idx])
