@@ -339,6 +339,35 @@ def test_batch_solve_matches_alone(ctx, precond):
         assert r1.lift == r0.lift and r1.drag == r0.drag
 
 
+def test_batch_long_krylov_matches_alone(ctx):
+    """FGMRES(40) with a linear tolerance the FP32 basis cannot reach (1e-12): the linear solves average 26-33
+    Krylov steps, and a case averaging more than 32 has solves past 32 steps, where the per-case Gram-Schmidt
+    passes use their widest FP32 instantiation that keeps the basis values in LDS (33-41 vectors).  The batched
+    cases reproduce their alone solves bit for bit."""
+    print("batch vs alone, FGMRES(40), linear tolerance 1e-12", flush=True)
+    cases = [((12, 4, 4), 0.08, 0.5, 0.0), ((14, 5, 5), 0.1, 0.45, -1.0), ((16, 6, 4), 0.12, 0.6, 2.0)]
+    meshes = [cmesh.bump_channel(*d, bump_h=h) for d, h, _, _ in cases]
+    params = [cfd.make_params(mach=mach, alpha_deg=a, flux="roe", order=1) for _, _, mach, a in cases]
+    kw = dict(max_iter=60, tol=1e-10, krylov_m=40, max_restarts=1, rtol=1e-12, precond=cfd.PREC_ILU0_MC,
+              mixed=True)
+    alone = []
+    for m, p, (_, _, _, a) in zip(meshes, params, cases):
+        dm = cfd.DeviceMesh(ctx, m, p)
+        r, _ = dm.solve_steady(p, alpha_deg=a, **kw)
+        alone.append(r)
+        dm.close()
+    assert any(r.krylov_total > 32 * r.iterations for r in alone)  # some linear solve past 32 steps
+    rc, batch = cfd.batch_solve(ctx, meshes, params, alphas=[a for *_, a in cases], **kw)
+    assert rc == 0, lib_error(ctx)
+    for r0, r1 in zip(alone, batch):
+        print("  alone: %d Newton, %d Krylov" % (r0.iterations, r0.krylov_total), flush=True)
+        assert r0.krylov_total > 20 * r0.iterations  # long linear solves
+        assert r1.status == r0.status and r1.converged == r0.converged
+        assert (r1.iterations, r1.krylov_total) == (r0.iterations, r0.krylov_total)
+        assert r1.first_rms == r0.first_rms and r1.final_rms == r0.final_rms
+        assert list(r1.force) == list(r0.force)
+
+
 def test_batch_solve_compaction_matches_alone(ctx):
     """Compaction: whenever the cases still iterating hold half of the union's nodes or less, they move to a new
     union of their own.  Every case still reproduces its alone solve bit for bit; the finished cases are
